@@ -1,0 +1,535 @@
+// extern "C" boundary of the library (declarations in include/graphene_amd.h).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/graphene_amd.h"
+#include "config.hpp"
+#include "device_matrix.hpp"
+#include "host_matrix.hpp"
+#include "solvers.hpp"
+
+using namespace graphene;
+using namespace graphene::matrix;
+namespace hostns = graphene::matrix::host;
+
+struct gph_host_matrix {
+  std::shared_ptr<hostns::HostMatrix> m;
+};
+struct gph_matrix {
+  std::unique_ptr<DeviceMatrix> A;
+};
+struct gph_solver {
+  const gph_matrix* A;
+  std::shared_ptr<solver::Configuration> cfg;
+  std::unique_ptr<solver::Solver> S;
+};
+
+namespace {
+thread_local std::string g_lastError;
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    g_lastError.clear();
+    return GPH_OK;
+  } catch (const gph::Error& e) {
+    g_lastError = e.what();
+    return e.code;
+  } catch (const std::bad_alloc& e) {
+    g_lastError = std::string("out of host memory: ") + e.what();
+    return GPH_ERR_RUNTIME;
+  } catch (const std::exception& e) {
+    g_lastError = e.what();
+    return GPH_ERR_RUNTIME;
+  } catch (...) {
+    g_lastError = "unknown error";
+    return GPH_ERR_RUNTIME;
+  }
+}
+
+gph::DType dt(gph_type t) {
+  switch (t) {
+    case GPH_FLOAT16:
+      return gph::DType::F16;
+    case GPH_FLOAT32:
+      return gph::DType::F32;
+    case GPH_FLOAT64:
+      return gph::DType::F64;
+  }
+  gph::fail(gph::kInvalid, "invalid element type " + std::to_string((int)t));
+}
+
+void need(const void* p, const char* what) {
+  if (!p) gph::fail(gph::kInvalid, std::string(what) + " must not be NULL");
+}
+
+int emitTriplets(hostns::TripletMatrix&& m, int64_t* count, int64_t* rows,
+                 int64_t* cols, float* vals) {
+  need(count, "count");
+  if (rows) {
+    if (*count < (int64_t)m.size())
+      gph::fail(gph::kInvalid, "output buffers too small");
+    need(cols, "cols");
+    need(vals, "vals");
+    std::memcpy(rows, m.rows.data(), m.size() * sizeof(int64_t));
+    std::memcpy(cols, m.cols.data(), m.size() * sizeof(int64_t));
+    std::memcpy(vals, m.vals.data(), m.size() * sizeof(float));
+  }
+  *count = (int64_t)m.size();
+  return 0;
+}
+
+gph_host_matrix* makeHost(hostns::TripletMatrix&& m, int32_t T,
+                          const int32_t* r2t) {
+  auto* h = new gph_host_matrix;
+  try {
+    h->m = hostns::buildHostMatrix(std::move(m), T, r2t);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  return h;
+}
+
+template <typename T>
+void permute(const hostns::HostMatrix& m, const void* in, void* out,
+             bool toLocal) {
+  const T* a = static_cast<const T*>(in);
+  T* b = static_cast<T*>(out);
+  for (int64_t p = 0; p < m.nrows; p++) {
+    int64_t g = m.newToOld[(size_t)p];
+    if (toLocal)
+      b[p] = a[g];
+    else
+      b[g] = a[p];
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* gph_last_error(void) { return g_lastError.c_str(); }
+int32_t gph_version(void) { return 100; }
+
+int gph_generate_poisson3d(int64_t nx, int64_t ny, int64_t nz, int64_t* count,
+                           int64_t* rows, int64_t* cols, float* vals) {
+  return guard([&] {
+    emitTriplets(hostns::generate3DPoissonTripletMatrix(nx, ny, nz), count,
+                 rows, cols, vals);
+  });
+}
+int gph_generate_convdiff3d27(int64_t nx, int64_t ny, int64_t nz,
+                              double peclet, uint64_t seed, int64_t* count,
+                              int64_t* rows, int64_t* cols, float* vals) {
+  return guard([&] {
+    emitTriplets(hostns::generate3DConvectionDiffusion27(nx, ny, nz, peclet, seed),
+                 count, rows, cols, vals);
+  });
+}
+int gph_generate_convdiff2d9(int64_t nx, int64_t ny, double diffusion,
+                             double peclet, uint64_t seed, int64_t* count,
+                             int64_t* rows, int64_t* cols, float* vals) {
+  return guard([&] {
+    emitTriplets(
+        hostns::generate2DConvectionDiffusion9(nx, ny, diffusion, peclet, seed),
+        count, rows, cols, vals);
+  });
+}
+int gph_generate_powerlaw(int64_t n, int32_t min_len, int32_t max_len,
+                          double exponent, int32_t band, double local_fraction,
+                          uint64_t seed, int64_t* count, int64_t* rows,
+                          int64_t* cols, float* vals) {
+  return guard([&] {
+    emitTriplets(hostns::generatePowerLaw(n, min_len, max_len, exponent, band,
+                                          local_fraction, seed),
+                 count, rows, cols, vals);
+  });
+}
+int gph_uniform_pm1(uint64_t seed, int64_t n, float* out) {
+  return guard([&] {
+    need(out, "out");
+    hostns::uniformPm1(seed, n, out);
+  });
+}
+
+int gph_partition_grid3d(int64_t nx, int64_t ny, int64_t nz, int64_t bx,
+                         int64_t by, int64_t bz, int32_t* row_to_tile,
+                         int32_t* num_tiles) {
+  return guard([&] {
+    need(row_to_tile, "row_to_tile");
+    auto v = hostns::partitionGrid3D(nx, ny, nz, bx, by, bz, num_tiles);
+    std::memcpy(row_to_tile, v.data(), v.size() * sizeof(int32_t));
+  });
+}
+
+int gph_host_matrix_from_coo(int64_t nrows, int64_t nnz, const int64_t* rows,
+                             const int64_t* cols, const float* vals,
+                             int32_t num_tiles, const int32_t* row_to_tile,
+                             gph_host_matrix** out) {
+  return guard([&] {
+    need(out, "out");
+    if (nnz > 0) {
+      need(rows, "rows");
+      need(cols, "cols");
+      need(vals, "vals");
+    }
+    hostns::TripletMatrix m;
+    m.nrows = m.ncols = nrows;
+    m.rows.assign(rows, rows + nnz);
+    m.cols.assign(cols, cols + nnz);
+    m.vals.assign(vals, vals + nnz);
+    *out = makeHost(std::move(m), num_tiles, row_to_tile);
+  });
+}
+int gph_host_matrix_poisson3d(int64_t nx, int64_t ny, int64_t nz,
+                              int32_t num_tiles, const int32_t* row_to_tile,
+                              gph_host_matrix** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = makeHost(hostns::generate3DPoissonTripletMatrix(nx, ny, nz),
+                    num_tiles, row_to_tile);
+  });
+}
+int gph_host_matrix_convdiff3d27(int64_t nx, int64_t ny, int64_t nz,
+                                 double peclet, uint64_t seed,
+                                 int32_t num_tiles, const int32_t* row_to_tile,
+                                 gph_host_matrix** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = makeHost(
+        hostns::generate3DConvectionDiffusion27(nx, ny, nz, peclet, seed),
+        num_tiles, row_to_tile);
+  });
+}
+int gph_host_matrix_powerlaw(int64_t n, int32_t min_len, int32_t max_len,
+                             double exponent, int32_t band,
+                             double local_fraction, uint64_t seed,
+                             int32_t num_tiles, const int32_t* row_to_tile,
+                             gph_host_matrix** out) {
+  return guard([&] {
+    need(out, "out");
+    *out = makeHost(hostns::generatePowerLaw(n, min_len, max_len, exponent,
+                                             band, local_fraction, seed),
+                    num_tiles, row_to_tile);
+  });
+}
+int gph_host_matrix_load_mtx(const char* path, int32_t num_tiles,
+                             const int32_t* row_to_tile,
+                             gph_host_matrix** out) {
+  return guard([&] {
+    need(path, "path");
+    need(out, "out");
+    *out = makeHost(hostns::loadTripletMatrixFromFile(path), num_tiles,
+                    row_to_tile);
+  });
+}
+void gph_host_matrix_destroy(gph_host_matrix* h) { delete h; }
+
+int gph_host_matrix_get_info(const gph_host_matrix* h,
+                             gph_host_matrix_info* info) {
+  return guard([&] {
+    need(h, "matrix");
+    need(info, "info");
+    const auto& m = *h->m;
+    info->nrows = m.nrows;
+    info->nnz_off = m.nnzOff();
+    info->num_halo = m.numHalo();
+    info->num_tiles = m.numTiles;
+    int32_t mc = 0, mr = 0;
+    for (int32_t t = 0; t < m.numTiles; t++) {
+      mc = std::max(mc, m.numColors[(size_t)t]);
+      mr = std::max<int32_t>(mr, (int32_t)m.tileRows(t));
+    }
+    info->max_colors = mc;
+    info->max_tile_rows = mr;
+    info->num_color_entries = (int64_t)m.colorStart.size();
+    info->num_upper_color_entries = (int64_t)m.upperColorStart.size();
+    info->multicolor_recommended = m.multicolorRecommended ? 1 : 0;
+  });
+}
+
+int gph_host_matrix_export(const gph_host_matrix* h, int64_t* tile_start,
+                           int64_t* new_to_old, int32_t* num_interior,
+                           int32_t* num_separator_regions, int64_t* row_ptr,
+                           int32_t* col, int32_t* col_local, float* off,
+                           float* diag, int64_t* halo_start,
+                           int64_t* halo_global) {
+  return guard([&] {
+    need(h, "matrix");
+    const auto& m = *h->m;
+    auto cp = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(tile_start, m.tileStart);
+    cp(new_to_old, m.newToOld);
+    cp(num_interior, m.numInterior);
+    cp(num_separator_regions, m.numSeparatorRegions);
+    cp(row_ptr, m.rowPtr);
+    cp(col, m.col);
+    cp(col_local, m.colLocal);
+    cp(off, m.offDiag);
+    cp(diag, m.diag);
+    cp(halo_start, m.haloStart);
+    cp(halo_global, m.haloGlobal);
+  });
+}
+
+int gph_host_matrix_export_levels(const gph_host_matrix* h, int32_t* level,
+                                  int32_t* color_sort_addr,
+                                  int64_t* color_start, int32_t* num_colors) {
+  return guard([&] {
+    need(h, "matrix");
+    const auto& m = *h->m;
+    auto cp = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(level, m.level);
+    cp(color_sort_addr, m.colorSortAddr);
+    cp(color_start, m.colorStart);
+    cp(num_colors, m.numColors);
+  });
+}
+
+int gph_host_matrix_export_ldu(const gph_host_matrix* h, int64_t* num_faces,
+                               int32_t* lower_addr, int32_t* upper_addr,
+                               int32_t* owner_start, int32_t* neighbour_start,
+                               int32_t* neighbour_col_ind, float* upper,
+                               float* lower) {
+  return guard([&] {
+    need(h, "matrix");
+    need(num_faces, "num_faces");
+    hostns::LDUHost L = hostns::buildLDU(*h->m);
+    *num_faces = L.numFaces;
+    auto cp = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    cp(lower_addr, L.lowerAddr);
+    cp(upper_addr, L.upperAddr);
+    cp(owner_start, L.ownerStart);
+    cp(neighbour_start, L.neighbourStart);
+    cp(neighbour_col_ind, L.neighbourColInd);
+    cp(upper, L.upper);
+    cp(lower, L.lower);
+  });
+}
+
+int gph_host_matrix_to_local(const gph_host_matrix* h, gph_type t,
+                             const void* global_in, void* local_out) {
+  return guard([&] {
+    need(h, "matrix");
+    need(global_in, "input");
+    need(local_out, "output");
+    switch (dt(t)) {
+      case gph::DType::F32: permute<float>(*h->m, global_in, local_out, true); break;
+      case gph::DType::F64: permute<double>(*h->m, global_in, local_out, true); break;
+      case gph::DType::F16: permute<uint16_t>(*h->m, global_in, local_out, true); break;
+    }
+  });
+}
+int gph_host_matrix_to_global(const gph_host_matrix* h, gph_type t,
+                              const void* local_in, void* global_out) {
+  return guard([&] {
+    need(h, "matrix");
+    need(local_in, "input");
+    need(global_out, "output");
+    switch (dt(t)) {
+      case gph::DType::F32: permute<float>(*h->m, local_in, global_out, false); break;
+      case gph::DType::F64: permute<double>(*h->m, local_in, global_out, false); break;
+      case gph::DType::F16: permute<uint16_t>(*h->m, local_in, global_out, false); break;
+    }
+  });
+}
+
+int gph_device_count(int32_t* count) {
+  return guard([&] {
+    need(count, "count");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+  });
+}
+int gph_malloc(size_t bytes, void** ptr) {
+  return guard([&] {
+    need(ptr, "ptr");
+    GPH_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  });
+}
+int gph_free(void* ptr) {
+  return guard([&] {
+    if (ptr) GPH_HIP(hipFree(ptr));
+  });
+}
+int gph_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  return guard([&] { GPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int gph_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  return guard([&] { GPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int gph_memset(void* dst, int value, size_t bytes) {
+  return guard([&] { GPH_HIP(hipMemset(dst, value, bytes)); });
+}
+int gph_synchronize(void) {
+  return guard([&] { GPH_HIP(hipDeviceSynchronize()); });
+}
+
+int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
+                      int32_t rows_per_block, gph_matrix** out) {
+  return guard([&] {
+    need(h, "host matrix");
+    need(out, "out");
+    if (fmt != GPH_CRS && fmt != GPH_LDU)
+      gph::fail(gph::kInvalid, "invalid matrix format");
+    auto* A = new gph_matrix;
+    try {
+      A->A = std::make_unique<DeviceMatrix>(
+          h->m, fmt == GPH_CRS ? MatrixFormat::CRS : MatrixFormat::LDU,
+          rows_per_block);
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = A;
+  });
+}
+void gph_matrix_destroy(gph_matrix* A) { delete A; }
+int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes) {
+  return guard([&] {
+    need(A, "matrix");
+    need(bytes, "bytes");
+    *bytes = A->A->deviceBytes();
+  });
+}
+
+int gph_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
+             const void* x, gph_type y_type, void* y, gph_type acc_type) {
+  return guard([&] {
+    need(A, "matrix");
+    need(x, "x");
+    need(y, "y");
+    hipStream_t s = A->A->stream();
+    A->A->spmv(Vec{const_cast<void*>(x), dt(x_type)}, Vec{y, dt(y_type)},
+               dt(acc_type), dt(value_type), s);
+    GPH_HIP(hipStreamSynchronize(s));
+  });
+}
+int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
+                 const void* x, gph_type b_type, const void* b,
+                 gph_type r_type, void* r, gph_type acc_type, double* r64) {
+  return guard([&] {
+    need(A, "matrix");
+    need(x, "x");
+    need(b, "b");
+    need(r, "r");
+    hipStream_t s = A->A->stream();
+    A->A->residual(Vec{const_cast<void*>(x), dt(x_type)},
+                   Vec{const_cast<void*>(b), dt(b_type)}, Vec{r, dt(r_type)},
+                   dt(acc_type), dt(value_type), s, r64);
+    GPH_HIP(hipStreamSynchronize(s));
+  });
+}
+int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
+                    double* out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(x, "x");
+    need(out, "out");
+    *out = A->A->norm2(Vec{const_cast<void*>(x), dt(t)}, A->A->stream());
+  });
+}
+int gph_bench_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
+                   const void* x, void* y, int32_t iters, double* avg_ms) {
+  return guard([&] {
+    need(A, "matrix");
+    need(avg_ms, "avg_ms");
+    if (iters < 1) gph::fail(gph::kInvalid, "iters must be >= 1");
+    hipStream_t s = A->A->stream();
+    gph::DType v = dt(value_type), xt = dt(x_type);
+    Vec xv{const_cast<void*>(x), xt}, yv{y, xt};
+    A->A->spmv(xv, yv, xt, v, s);  // warm-up
+    hipEvent_t e0, e1;
+    GPH_HIP(hipEventCreate(&e0));
+    GPH_HIP(hipEventCreate(&e1));
+    GPH_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; i++) A->A->spmv(xv, yv, xt, v, s);
+    GPH_HIP(hipEventRecord(e1, s));
+    GPH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+  });
+}
+
+int gph_solver_create(const gph_matrix* A, const char* json, gph_solver** out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(json, "json");
+    need(out, "out");
+    auto* S = new gph_solver;
+    try {
+      S->A = A;
+      S->cfg = solver::Configuration::fromJSON(json);
+      S->S = solver::Solver::createSolver(*A->A, S->cfg);
+    } catch (...) {
+      delete S;
+      throw;
+    }
+    *out = S;
+  });
+}
+void gph_solver_destroy(gph_solver* S) { delete S; }
+
+int gph_solve(gph_solver* S, gph_type x_type, void* x, gph_type b_type,
+              const void* b, gph_solver_stats* stats) {
+  return guard([&] {
+    need(S, "solver");
+    need(x, "x");
+    need(b, "b");
+    solver::SolverStats st = S->S->solve(Vec{x, dt(x_type)},
+                                         Vec{const_cast<void*>(b), dt(b_type)});
+    if (stats) {
+      stats->initial_residual = st.initialResidual;
+      stats->final_residual = st.finalResidual;
+      stats->iterations = st.iterations;
+      stats->inner_iterations = st.innerIterations;
+      stats->converged = st.converged ? 1 : 0;
+      stats->singular = st.singular ? 1 : 0;
+      stats->seconds = st.seconds;
+      std::memset(stats->name, 0, sizeof(stats->name));
+      std::strncpy(stats->name, st.solverName.c_str(), sizeof(stats->name) - 1);
+    }
+  });
+}
+int gph_solver_name(const gph_solver* S, char* buf, size_t len) {
+  return guard([&] {
+    need(S, "solver");
+    need(buf, "buf");
+    std::string n = S->S->name();
+    if (len == 0) gph::fail(gph::kInvalid, "buffer length 0");
+    std::strncpy(buf, n.c_str(), len - 1);
+    buf[len - 1] = 0;
+  });
+}
+int gph_solver_uses_initial_guess(const gph_solver* S, int32_t* out) {
+  return guard([&] {
+    need(S, "solver");
+    need(out, "out");
+    *out = S->S->usesInitialGuess() ? 1 : 0;
+  });
+}
+int gph_config_check(const char* json, char* name_buf, size_t len) {
+  return guard([&] {
+    need(json, "json");
+    auto c = solver::Configuration::fromJSON(json);
+    if (name_buf && len > 0) {
+      std::string n = c->solverName();
+      std::strncpy(name_buf, n.c_str(), len - 1);
+      name_buf[len - 1] = 0;
+    }
+  });
+}
+
+}  // extern "C"

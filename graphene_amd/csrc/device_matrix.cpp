@@ -1,0 +1,257 @@
+// Device-resident matrix (see device_matrix.hpp).
+#include "device_matrix.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace graphene::matrix {
+
+using gph::DevBuf;
+using gph::DType;
+using gph::fail;
+
+static std::vector<int32_t> toI32(const std::vector<int64_t>& v) {
+  std::vector<int32_t> o(v.size());
+  for (size_t i = 0; i < v.size(); i++) {
+    if (v[i] > INT32_MAX || v[i] < INT32_MIN)
+      fail(gph::kInvalid, "index does not fit int32");
+    o[i] = (int32_t)v[i];
+  }
+  return o;
+}
+
+template <typename T>
+static void up(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.upload(v.data(), v.size(), s);
+}
+
+DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
+                           MatrixFormat fmt, int32_t rowsPerBlock)
+    : host_(std::move(host)), fmt_(fmt) {
+  const host::HostMatrix& h = *host_;
+  n_ = (int32_t)h.nrows;
+  T_ = h.numTiles;
+  nnz_ = h.nnzOff();
+  GPH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  hipStream_t s = stream_;
+
+  up(rowPtr_, toI32(h.rowPtr), s);
+  up(col_, h.col, s);
+  up(off32_, h.offDiag, s);
+  up(diag32_, h.diag, s);
+
+  std::vector<int64_t> rowLen((size_t)n_);
+  for (int32_t r = 0; r < n_; r++) rowLen[(size_t)r] = h.rowPtr[(size_t)r + 1] - h.rowPtr[(size_t)r];
+  buildRowBlocks(rowLen, rowsPerBlock, blockRowCrs_, numBlocksCrs_);
+
+  if (fmt_ == MatrixFormat::LDU) {
+    host::LDUHost L = host::buildLDU(h);
+    numFaces_ = L.numFaces;
+    std::vector<int32_t> lowerCol((size_t)L.numFaces);
+    for (int64_t q = 0; q < L.numFaces; q++)
+      lowerCol[(size_t)q] = L.lowerAddr[(size_t)L.neighbourColInd[(size_t)q]];
+    up(ownerStart_, L.ownerStart, s);
+    up(upperCol_, L.upperAddr, s);
+    up(upper32_, L.upper, s);
+    up(nbrStart_, L.neighbourStart, s);
+    up(lowerCol_, lowerCol, s);
+    up(lower32_, L.lower, s);
+    std::vector<int64_t> len((size_t)n_);
+    for (int32_t r = 0; r < n_; r++)
+      len[(size_t)r] = (int64_t)(L.ownerStart[(size_t)r + 1] - L.ownerStart[(size_t)r]) +
+                       (int64_t)(L.neighbourStart[(size_t)r + 1] - L.neighbourStart[(size_t)r]);
+    buildRowBlocks(len, rowsPerBlock, blockRowLdu_, numBlocksLdu_);
+  }
+
+  // tile structure and level sets
+  up(tileStart_, toI32(h.tileStart), s);
+  up(lvlRows_, h.colorSortAddr, s);
+  up(lvlPtr_, toI32(h.colorStart), s);
+  up(lvlPtrOff_, toI32(h.colorStartOffset), s);
+  up(ulvlRows_, h.upperLevelRows, s);
+  up(ulvlPtr_, toI32(h.upperColorStart), s);
+  up(ulvlPtrOff_, toI32(h.upperColorStartOffset), s);
+  maxTileRows_ = 0;
+  for (int32_t t = 0; t < T_; t++)
+    maxTileRows_ = std::max<int32_t>(maxTileRows_, (int32_t)h.tileRows(t));
+
+  scratchState_.alloc(sizeof(gph::DevState));
+  GPH_HIP(hipMemsetAsync(scratchState_.get(), 0, sizeof(gph::DevState), s));
+  int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096});
+  partialsBytes_ = (size_t)maxBlocks * 2 * sizeof(double);
+  scratchPartials_.alloc(partialsBytes_);
+  GPH_HIP(hipStreamSynchronize(s));
+}
+
+DeviceMatrix::~DeviceMatrix() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+// Contiguous row blocks of at most 256 rows whose entries fit one staging
+// chunk (kSpmvCap) where possible; a row longer than a chunk forms its own
+// block and is streamed in several chunks.
+void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
+                                  int32_t rowsPerBlock, DevBuf& out,
+                                  int32_t& nb) {
+  const int32_t maxRows =
+      (rowsPerBlock > 0) ? std::min(rowsPerBlock, gph::kSpmvThreads)
+                         : gph::kSpmvThreads;
+  std::vector<int32_t> br;
+  br.push_back(0);
+  int64_t acc = 0;
+  int32_t rows = 0;
+  for (int32_t r = 0; r < n_; r++) {
+    int64_t l = rowLen[(size_t)r];
+    if (rows > 0 && (rows >= maxRows || acc + l > gph::kSpmvCap)) {
+      br.push_back(r);
+      acc = 0;
+      rows = 0;
+    }
+    acc += l;
+    rows++;
+  }
+  br.push_back(n_);
+  nb = (int32_t)br.size() - 1;
+  out.upload(br.data(), br.size(), stream_);
+}
+
+const void* DeviceMatrix::ensureCopy(const DevBuf& src32, DevBuf& dst,
+                                     DType vt, int64_t count) const {
+  std::lock_guard<std::mutex> lk(copyMutex_);
+  if (!dst) {
+    dst.alloc((size_t)std::max<int64_t>(count, 1) * gph::dtypeSize(vt));
+    gph::launchCast(src32.get(), DType::F32, dst.get(), vt, count, stream_);
+    GPH_HIP(hipStreamSynchronize(stream_));
+  }
+  return dst.get();
+}
+
+const void* DeviceMatrix::crsOff(DType vt) const {
+  switch (vt) {
+    case DType::F32:
+      return off32_.get();
+    case DType::F64:
+      return ensureCopy(off32_, off64_, vt, nnz_);
+    case DType::F16:
+      return ensureCopy(off32_, off16_, vt, nnz_);
+  }
+  return nullptr;
+}
+const void* DeviceMatrix::crsDiag(DType vt) const {
+  switch (vt) {
+    case DType::F32:
+      return diag32_.get();
+    case DType::F64:
+      return ensureCopy(diag32_, diag64_, vt, n_);
+    case DType::F16:
+      return ensureCopy(diag32_, diag16_, vt, n_);
+  }
+  return nullptr;
+}
+
+gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
+  gph::SpmvArgs a;
+  a.n = n_;
+  a.diag = crsDiag(vt);
+  if (fmt_ == MatrixFormat::CRS) {
+    a.numBlocks = numBlocksCrs_;
+    a.blockRow = blockRowCrs_.as<int32_t>();
+    a.nstreams = 1;
+    a.ptr[0] = rowPtr_.as<int32_t>();
+    a.col[0] = col_.as<int32_t>();
+    a.val[0] = crsOff(vt);
+  } else {
+    a.numBlocks = numBlocksLdu_;
+    a.blockRow = blockRowLdu_.as<int32_t>();
+    a.nstreams = 2;
+    a.ptr[0] = ownerStart_.as<int32_t>();
+    a.col[0] = upperCol_.as<int32_t>();
+    a.ptr[1] = nbrStart_.as<int32_t>();
+    a.col[1] = lowerCol_.as<int32_t>();
+    switch (vt) {
+      case DType::F32:
+        a.val[0] = upper32_.get();
+        a.val[1] = lower32_.get();
+        break;
+      case DType::F64:
+        a.val[0] = ensureCopy(upper32_, upper64_, vt, numFaces_);
+        a.val[1] = ensureCopy(lower32_, lower64_, vt, numFaces_);
+        break;
+      case DType::F16:
+        a.val[0] = ensureCopy(upper32_, upper16_, vt, numFaces_);
+        a.val[1] = ensureCopy(lower32_, lower16_, vt, numFaces_);
+        break;
+    }
+  }
+  return a;
+}
+
+gph::TileArgs DeviceMatrix::tileArgs() const {
+  gph::TileArgs t;
+  t.T = T_;
+  t.tileStart = tileStart_.as<int32_t>();
+  t.lvlRows = lvlRows_.as<int32_t>();
+  t.lvlPtr = lvlPtr_.as<int32_t>();
+  t.lvlPtrOff = lvlPtrOff_.as<int32_t>();
+  t.ulvlRows = ulvlRows_.as<int32_t>();
+  t.ulvlPtr = ulvlPtr_.as<int32_t>();
+  t.ulvlPtrOff = ulvlPtrOff_.as<int32_t>();
+  t.rowPtr = rowPtr_.as<int32_t>();
+  t.col = col_.as<int32_t>();
+  t.maxTileRows = maxTileRows_;
+  return t;
+}
+
+void DeviceMatrix::spmv(const Vec& x, const Vec& y, DType acc, DType vt,
+                        hipStream_t s) const {
+  gph::SpmvArgs a = spmvArgs(vt);
+  a.x = x.ptr;
+  a.out = y.ptr;
+  gph::launchSpmv(a, vt, x.type, y.type, y.type, acc, gph::kModeSpmv,
+                  gph::kRedNone, s);
+}
+
+void DeviceMatrix::residual(const Vec& x, const Vec& b, const Vec& r,
+                            DType acc, DType vt, hipStream_t s,
+                            double* r64) const {
+  gph::SpmvArgs a = spmvArgs(vt);
+  a.x = x.ptr;
+  a.b = b.ptr;
+  a.out = r.ptr;
+  a.out64 = r64;
+  gph::launchSpmv(a, vt, x.type, b.type, r.type, acc, gph::kModeResidual,
+                  gph::kRedNone, s);
+}
+
+double DeviceMatrix::norm2(const Vec& x, hipStream_t s) const {
+  gph::VecArgs v;
+  v.n = n_;
+  v.a = x.ptr;
+  v.state = scratchState();
+  v.partials = scratchPartials();
+  v.fin = gph::kFinNormAux;
+  gph::launchNorm2(v, x.type, s);
+  double out = 0;
+  GPH_HIP(hipMemcpyAsync(&out, &scratchState()->aux, sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  GPH_HIP(hipStreamSynchronize(s));
+  return out;
+}
+
+size_t DeviceMatrix::deviceBytes() const {
+  size_t b = 0;
+  for (const DevBuf* p : std::initializer_list<const DevBuf*>
+       {&rowPtr_, &col_, &off32_, &diag32_, &off64_, &diag64_, &off16_,
+        &diag16_, &blockRowCrs_, &ownerStart_, &upperCol_, &upper32_,
+        &nbrStart_, &lowerCol_, &lower32_, &upper64_, &lower64_, &upper16_,
+        &lower16_, &blockRowLdu_, &tileStart_, &lvlRows_, &lvlPtr_,
+        &lvlPtrOff_, &ulvlRows_, &ulvlPtr_, &ulvlPtrOff_, &scratchState_,
+        &scratchPartials_})
+    b += p->bytes();
+  return b;
+}
+
+}  // namespace graphene::matrix

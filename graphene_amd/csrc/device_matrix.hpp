@@ -1,0 +1,99 @@
+// Device-resident matrix: the counterpart of HostMatrix::copyToTile and of
+// the reference's CRSMatrix / LDUMatrix (lg/matrix/details/crs/CRSMatrix.hpp:35-66,
+// lg/matrix/details/ldu/LDUMatrix.hpp:49-57) plus the operations of
+// MatrixBase (spmv, residual, vectorNorm; lg/matrix/details/MatrixBase.hpp:66-82).
+#pragma once
+
+#include <memory>
+#include <mutex>
+
+#include "gph_common.hpp"
+#include "host_matrix.hpp"
+#include "kernels.hpp"
+
+namespace graphene::matrix {
+
+enum class MatrixFormat : int32_t { CRS = 0, LDU = 1 };  // MatrixFormat.hpp
+
+// A device vector: pointer + element type (the reference's Tensor in the
+// tile-major numbering; vectors "with halo" and "without halo" coincide).
+struct Vec {
+  void* ptr = nullptr;
+  gph::DType type = gph::DType::F32;
+};
+
+class DeviceMatrix {
+ public:
+  DeviceMatrix(std::shared_ptr<const host::HostMatrix> host, MatrixFormat fmt,
+               int32_t rowsPerBlock = 0);
+  ~DeviceMatrix();
+  DeviceMatrix(const DeviceMatrix&) = delete;
+  DeviceMatrix& operator=(const DeviceMatrix&) = delete;
+
+  const host::HostMatrix& hostMatrix() const { return *host_; }
+  std::shared_ptr<const host::HostMatrix> hostPtr() const { return host_; }
+  MatrixFormat format() const { return fmt_; }
+  int32_t n() const { return n_; }
+  int64_t nnzOff() const { return nnz_; }
+  int32_t numTiles() const { return T_; }
+  hipStream_t stream() const { return stream_; }
+
+  // Row-gather arguments for values stored in `vt` (copies in float16 /
+  // float64 are made on first use).
+  gph::SpmvArgs spmvArgs(gph::DType vt) const;
+  gph::TileArgs tileArgs() const;
+  const void* crsOff(gph::DType vt) const;   // CRS off-diagonal values
+  const void* crsDiag(gph::DType vt) const;
+  const int32_t* crsRowPtr() const { return rowPtr_.as<int32_t>(); }
+  const int32_t* crsCol() const { return col_.as<int32_t>(); }
+
+  // y = A*x, accumulated in `acc` (Matrix::operator*, MatrixBase::spmv).
+  void spmv(const Vec& x, const Vec& y, gph::DType acc, gph::DType vt,
+            hipStream_t s) const;
+  // r = b - A*x accumulated in `acc` (MatrixBase::residual).
+  void residual(const Vec& x, const Vec& b, const Vec& r, gph::DType acc,
+                gph::DType vt, hipStream_t s, double* r64 = nullptr) const;
+  // ||x||_2 (MatrixBase::vectorNorm), synchronous.
+  double norm2(const Vec& x, hipStream_t s) const;
+
+  // scratch for standalone reductions
+  gph::DevState* scratchState() const { return scratchState_.as<gph::DevState>(); }
+  void* scratchPartials() const { return scratchPartials_.get(); }
+  size_t partialsBytes() const { return partialsBytes_; }
+
+  size_t deviceBytes() const;
+
+ private:
+  void buildRowBlocks(const std::vector<int64_t>& rowLen, int32_t rowsPerBlock,
+                      gph::DevBuf& out, int32_t& nb);
+  const void* ensureCopy(const gph::DevBuf& src32, gph::DevBuf& dst, gph::DType vt,
+                         int64_t count) const;
+
+  std::shared_ptr<const host::HostMatrix> host_;
+  MatrixFormat fmt_;
+  int32_t n_ = 0, T_ = 0;
+  int64_t nnz_ = 0;
+  hipStream_t stream_ = nullptr;
+
+  gph::DevBuf rowPtr_, col_, off32_, diag32_;
+  mutable gph::DevBuf off64_, diag64_, off16_, diag16_;
+  gph::DevBuf blockRowCrs_;
+  int32_t numBlocksCrs_ = 0;
+
+  // LDU: owner-face stream and resolved losort stream
+  int64_t numFaces_ = 0;
+  gph::DevBuf ownerStart_, upperCol_, upper32_, nbrStart_, lowerCol_, lower32_;
+  mutable gph::DevBuf upper64_, lower64_, upper16_, lower16_;
+  gph::DevBuf blockRowLdu_;
+  int32_t numBlocksLdu_ = 0;
+
+  gph::DevBuf tileStart_, lvlRows_, lvlPtr_, lvlPtrOff_, ulvlRows_, ulvlPtr_,
+      ulvlPtrOff_;
+  int32_t maxTileRows_ = 0;
+
+  gph::DevBuf scratchState_, scratchPartials_;
+  size_t partialsBytes_ = 0;
+  mutable std::mutex copyMutex_;
+};
+
+}  // namespace graphene::matrix

@@ -1,0 +1,732 @@
+// Host-side matrix preparation (see host_matrix.hpp).
+#include "host_matrix.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+
+#include "gph_common.hpp"
+
+namespace graphene::matrix::host {
+
+using gph::fail;
+using gph::kInvalid;
+using gph::kRuntime;
+
+// ---------------------------------------------------------------------------
+// RNG: splitmix64 seeding of xoshiro256** (public-domain reference algorithms
+// by Vigna). Fixed here so that every component draws identical inputs.
+// ---------------------------------------------------------------------------
+static uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static inline uint64_t rotl(uint64_t x, int k) {
+  return (x << k) | (x >> (64 - k));
+}
+Rng::Rng(uint64_t seed) {
+  uint64_t sm = seed;
+  for (auto& w : s) w = splitmix64(sm);
+}
+uint64_t Rng::next() {
+  uint64_t result = rotl(s[1] * 5, 7) * 9;
+  uint64_t t = s[1] << 17;
+  s[2] ^= s[0];
+  s[3] ^= s[1];
+  s[1] ^= s[2];
+  s[0] ^= s[3];
+  s[2] ^= t;
+  s[3] = rotl(s[3], 45);
+  return result;
+}
+float Rng::uniformPm1() {
+  float u = (float)(next() >> 40) * (1.0f / 16777216.0f);
+  return u * 2.0f - 1.0f;
+}
+double Rng::uniform01d() { return (double)(next() >> 11) * 0x1.0p-53; }
+
+void uniformPm1(uint64_t seed, int64_t n, float* out) {
+  Rng r(seed);
+  for (int64_t i = 0; i < n; i++) out[i] = r.uniformPm1();
+}
+
+// ---------------------------------------------------------------------------
+// Generators
+// ---------------------------------------------------------------------------
+
+// lg/matrix/host/details/Poisson.cpp:26-69. Row k*nx*ny + j*nx + i; diagonal
+// 6, -1 to every in-domain face neighbour; push order diag, i-1, i+1, j-1,
+// j+1, k-1, k+1.
+TripletMatrix generate3DPoissonTripletMatrix(int64_t nx, int64_t ny,
+                                             int64_t nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) fail(kInvalid, "Poisson: empty grid");
+  TripletMatrix m;
+  m.nrows = m.ncols = nx * ny * nz;
+  size_t cap = (size_t)(7 * m.nrows);
+  m.rows.reserve(cap);
+  m.cols.reserve(cap);
+  m.vals.reserve(cap);
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        int64_t row = k * nx * ny + j * nx + i;
+        m.push(row, row, 6.0f);
+        if (i > 0) m.push(row, row - 1, -1.0f);
+        if (i < nx - 1) m.push(row, row + 1, -1.0f);
+        if (j > 0) m.push(row, row - nx, -1.0f);
+        if (j < ny - 1) m.push(row, row + nx, -1.0f);
+        if (k > 0) m.push(row, row - nx * ny, -1.0f);
+        if (k < nz - 1) m.push(row, row + nx * ny, -1.0f);
+      }
+  return m;
+}
+
+// Unit direction from a seed: v ~ U[-1,1)^dim (doubles), redrawn until
+// 0.1 <= |v| <= 1, then normalised.
+static void unitDirection(Rng& r, int dim, double* u) {
+  for (;;) {
+    double v[3] = {0, 0, 0}, nrm = 0;
+    for (int d = 0; d < dim; d++) {
+      v[d] = r.uniform01d() * 2.0 - 1.0;
+      nrm += v[d] * v[d];
+    }
+    nrm = std::sqrt(nrm);
+    if (nrm >= 0.1 && nrm <= 1.0) {
+      for (int d = 0; d < dim; d++) u[d] = v[d] / nrm;
+      return;
+    }
+  }
+}
+
+// 3D convection-diffusion, 27-point stencil, no reference counterpart.
+// Diffusion: 27-point Laplacian (diagonal 26, -1 for each of the 26
+// neighbours). Convection: first-order upwind along the unit direction u
+// drawn from `seed`, cell Peclet number `peclet`: for every axis d the
+// upwind face neighbour (-e_d if u_d > 0, +e_d if u_d < 0) gets
+// -peclet*|u_d| and the diagonal +peclet*|u_d|. Dirichlet boundaries are
+// eliminated: out-of-domain neighbours are dropped, the diagonal keeps its
+// full value. Coefficients are formed in FP64 and rounded to FP32. Entries of
+// a row are pushed diagonal first, then neighbours in ascending column order.
+TripletMatrix generate3DConvectionDiffusion27(int64_t nx, int64_t ny,
+                                              int64_t nz, double peclet,
+                                              uint64_t seed) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) fail(kInvalid, "convdiff27: empty grid");
+  Rng rng(seed);
+  double u[3];
+  unitDirection(rng, 3, u);
+  double coef[27];
+  double dg = 26.0;
+  for (int q = 0; q < 27; q++) coef[q] = -1.0;
+  auto idx = [](int di, int dj, int dk) {
+    return (dk + 1) * 9 + (dj + 1) * 3 + (di + 1);
+  };
+  for (int d = 0; d < 3; d++) {
+    double a = peclet * std::fabs(u[d]);
+    int s = u[d] > 0 ? -1 : 1;
+    int o[3] = {0, 0, 0};
+    o[d] = s;
+    if (u[d] != 0.0) coef[idx(o[0], o[1], o[2])] -= a;
+    dg += a;
+  }
+  const float diagF = (float)dg;
+  float cf[27];
+  for (int q = 0; q < 27; q++) cf[q] = (float)coef[q];
+
+  TripletMatrix m;
+  m.nrows = m.ncols = nx * ny * nz;
+  size_t cap = (size_t)(27 * m.nrows);
+  m.rows.reserve(cap);
+  m.cols.reserve(cap);
+  m.vals.reserve(cap);
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        int64_t row = k * nx * ny + j * nx + i;
+        m.push(row, row, diagF);
+        for (int dk = -1; dk <= 1; dk++) {
+          int64_t kk = k + dk;
+          if (kk < 0 || kk >= nz) continue;
+          for (int dj = -1; dj <= 1; dj++) {
+            int64_t jj = j + dj;
+            if (jj < 0 || jj >= ny) continue;
+            for (int di = -1; di <= 1; di++) {
+              int64_t ii = i + di;
+              if (ii < 0 || ii >= nx) continue;
+              if (di == 0 && dj == 0 && dk == 0) continue;
+              m.push(row, kk * nx * ny + jj * nx + ii, cf[idx(di, dj, dk)]);
+            }
+          }
+        }
+      }
+  return m;
+}
+
+// 2D convection-diffusion, 9-point stencil, no reference counterpart.
+// diffusion*(9-point Laplacian: diagonal 8, -1 for the 8 neighbours) plus
+// first-order upwind convection with cell Peclet `peclet` along a seeded unit
+// direction (coefficient peclet*diffusion*|u_d| on the upwind face
+// neighbour and the diagonal). Dirichlet boundaries eliminated.
+TripletMatrix generate2DConvectionDiffusion9(int64_t nx, int64_t ny,
+                                             double diffusion, double peclet,
+                                             uint64_t seed) {
+  if (nx <= 0 || ny <= 0) fail(kInvalid, "convdiff9: empty grid");
+  Rng rng(seed);
+  double u[3];
+  unitDirection(rng, 2, u);
+  double coef[9];
+  double dg = 8.0 * diffusion;
+  for (int q = 0; q < 9; q++) coef[q] = -diffusion;
+  auto idx = [](int di, int dj) { return (dj + 1) * 3 + (di + 1); };
+  for (int d = 0; d < 2; d++) {
+    double a = peclet * diffusion * std::fabs(u[d]);
+    int s = u[d] > 0 ? -1 : 1;
+    int o[2] = {0, 0};
+    o[d] = s;
+    if (u[d] != 0.0) coef[idx(o[0], o[1])] -= a;
+    dg += a;
+  }
+  const float diagF = (float)dg;
+  float cf[9];
+  for (int q = 0; q < 9; q++) cf[q] = (float)coef[q];
+  TripletMatrix m;
+  m.nrows = m.ncols = nx * ny;
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i) {
+      int64_t row = j * nx + i;
+      m.push(row, row, diagF);
+      for (int dj = -1; dj <= 1; dj++) {
+        int64_t jj = j + dj;
+        if (jj < 0 || jj >= ny) continue;
+        for (int di = -1; di <= 1; di++) {
+          int64_t ii = i + di;
+          if (ii < 0 || ii >= nx) continue;
+          if (di == 0 && dj == 0) continue;
+          m.push(row, jj * nx + ii, cf[idx(di, dj)]);
+        }
+      }
+    }
+  return m;
+}
+
+// Unstructured power-law matrix, no reference counterpart.
+// Row length L (stored entries including the diagonal) drawn from the
+// discrete truncated power law P(L) ~ L^-exponent on [minLen, maxLen]; the
+// L-1 off-diagonal columns are distinct and != row: with probability
+// localFraction uniform in [row-band, row+band] (redrawn when outside the
+// matrix), otherwise uniform in [0, n). Off-diagonal values -U[0,1),
+// diagonal 1.1 * sum|off| (FP64, rounded to FP32). Rows are drawn in order
+// from one RNG stream.
+TripletMatrix generatePowerLaw(int64_t n, int32_t minLen, int32_t maxLen,
+                               double exponent, int32_t band,
+                               double localFraction, uint64_t seed) {
+  if (n <= 1 || minLen < 1 || maxLen < minLen || band < 1)
+    fail(kInvalid, "power-law generator: invalid parameters");
+  Rng rng(seed);
+  std::vector<double> cdf;
+  double tot = 0;
+  for (int32_t L = minLen; L <= maxLen; L++) {
+    tot += std::pow((double)L, -exponent);
+    cdf.push_back(tot);
+  }
+  TripletMatrix m;
+  m.nrows = m.ncols = n;
+  std::vector<int64_t> cols;
+  for (int64_t r = 0; r < n; r++) {
+    double uu = rng.uniform01d() * tot;
+    int32_t L = maxLen;
+    for (size_t q = 0; q < cdf.size(); q++)
+      if (uu < cdf[q]) {
+        L = minLen + (int32_t)q;
+        break;
+      }
+    int32_t want = std::min<int64_t>(L - 1, n - 1);
+    cols.clear();
+    double asum = 0;
+    std::vector<float> vals;
+    while ((int32_t)cols.size() < want) {
+      int64_t c;
+      if (rng.uniform01d() < localFraction) {
+        int64_t off = (int64_t)(rng.next() % (uint64_t)(2 * band + 1)) - band;
+        c = r + off;
+        if (c < 0 || c >= n) continue;
+      } else {
+        c = (int64_t)(rng.next() % (uint64_t)n);
+      }
+      if (c == r) continue;
+      if (std::find(cols.begin(), cols.end(), c) != cols.end()) continue;
+      float v = -(float)rng.uniform01d();
+      cols.push_back(c);
+      vals.push_back(v);
+      asum += std::fabs((double)v);
+    }
+    m.push(r, r, (float)(1.1 * asum));
+    for (size_t q = 0; q < cols.size(); q++) m.push(r, cols[q], vals[q]);
+  }
+  return m;
+}
+
+// Matrix Market coordinate reader (real / integer / pattern; general,
+// symmetric, skew-symmetric), 1-based indices, symmetric storage mirrored.
+TripletMatrix loadTripletMatrixFromFile(const std::string& path) {
+  std::ifstream in(path);
+  if (!in.is_open()) fail(kInvalid, "Failed to open file " + path);
+  std::string line;
+  if (!std::getline(in, line)) fail(kInvalid, "empty Matrix Market file");
+  std::string lower = line;
+  for (auto& ch : lower) ch = (char)std::tolower((unsigned char)ch);
+  if (lower.rfind("%%matrixmarket", 0) != 0)
+    fail(kInvalid, "missing %%MatrixMarket header");
+  if (lower.find("coordinate") == std::string::npos)
+    fail(gph::kUnsupported, "only coordinate Matrix Market files");
+  bool pattern = lower.find("pattern") != std::string::npos;
+  bool symmetric = lower.find(" symmetric") != std::string::npos ||
+                   lower.find("hermitian") != std::string::npos;
+  bool skew = lower.find("skew-symmetric") != std::string::npos;
+  if (lower.find("complex") != std::string::npos)
+    fail(kInvalid, "complex Matrix Market files are not supported");
+  while (std::getline(in, line)) {
+    if (!line.empty() && line[0] != '%') break;
+  }
+  std::istringstream hs(line);
+  int64_t nr, nc, ne;
+  if (!(hs >> nr >> nc >> ne)) fail(kInvalid, "bad Matrix Market size line");
+  TripletMatrix m;
+  m.nrows = nr;
+  m.ncols = nc;
+  m.rows.reserve(ne);
+  m.cols.reserve(ne);
+  m.vals.reserve(ne);
+  for (int64_t e = 0; e < ne; e++) {
+    int64_t r, c;
+    double v = 1.0;
+    if (!(in >> r >> c)) fail(kInvalid, "truncated Matrix Market file");
+    if (!pattern && !(in >> v)) fail(kInvalid, "truncated Matrix Market file");
+    r -= 1;
+    c -= 1;
+    if (r < 0 || r >= nr || c < 0 || c >= nc)
+      fail(kInvalid, "Matrix Market index out of range");
+    m.push(r, c, (float)v);
+    if ((symmetric || skew) && r != c) m.push(c, r, (float)(skew ? -v : v));
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Partitioners
+// ---------------------------------------------------------------------------
+std::vector<int32_t> partitionGrid3D(int64_t nx, int64_t ny, int64_t nz,
+                                     int64_t bx, int64_t by, int64_t bz,
+                                     int32_t* numTiles) {
+  if (bx <= 0 || by <= 0 || bz <= 0) fail(kInvalid, "block size must be > 0");
+  int64_t tx = (nx + bx - 1) / bx, ty = (ny + by - 1) / by,
+          tz = (nz + bz - 1) / bz;
+  std::vector<int32_t> r2t((size_t)(nx * ny * nz));
+  for (int64_t k = 0; k < nz; k++)
+    for (int64_t j = 0; j < ny; j++)
+      for (int64_t i = 0; i < nx; i++)
+        r2t[(size_t)(k * nx * ny + j * nx + i)] =
+            (int32_t)(((k / bz) * ty + (j / by)) * tx + (i / bx));
+  if (numTiles) *numTiles = (int32_t)(tx * ty * tz);
+  return r2t;
+}
+
+// Contiguous row ranges with balanced weight (row length + 1, the vertex
+// weight the reference hands to METIS, CRSHostMatrix.cpp:197).
+std::vector<int32_t> partitionContiguous(const std::vector<int64_t>& rowPtr,
+                                         int32_t numTiles) {
+  int64_t n = (int64_t)rowPtr.size() - 1;
+  if (numTiles < 1) fail(kInvalid, "numTiles must be >= 1");
+  std::vector<int32_t> r2t((size_t)n);
+  double total = (double)(rowPtr[n] + n);
+  double acc = 0;
+  for (int64_t r = 0; r < n; r++) {
+    int32_t t = (int32_t)std::min<double>(numTiles - 1,
+                                          std::floor(acc / total * numTiles));
+    r2t[(size_t)r] = t;
+    acc += (double)(rowPtr[r + 1] - rowPtr[r] + 1);
+  }
+  return r2t;
+}
+
+// ---------------------------------------------------------------------------
+// COO -> CRS (CoordinateFormat.cpp:29-40 + CRSHostMatrix.cpp:87-137):
+// sorted by (row, col), diagonal split out, duplicates summed.
+// ---------------------------------------------------------------------------
+struct GlobalCRS {
+  std::vector<int64_t> rowPtr;
+  std::vector<int32_t> col;
+  std::vector<float> off;
+  std::vector<float> diag;
+};
+
+static GlobalCRS convertToCRS(TripletMatrix&& coo) {
+  const int64_t n = coo.nrows;
+  if (coo.ncols != n) fail(kInvalid, "matrix must be square");
+  if (n <= 0) fail(kInvalid, "matrix must have at least one row");
+  if (n >= (int64_t)INT32_MAX) fail(kInvalid, "too many rows for int32 ids");
+  const size_t ne = coo.size();
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (size_t e = 0; e < ne; e++) {
+    int64_t r = coo.rows[e], c = coo.cols[e];
+    if (r < 0 || r >= n || c < 0 || c >= n)
+      fail(kInvalid, "triplet index out of range");
+    cnt[(size_t)r + 1]++;
+  }
+  for (int64_t r = 0; r < n; r++) cnt[r + 1] += cnt[r];
+  std::vector<int64_t> ord(ne);
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for (size_t e = 0; e < ne; e++) ord[(size_t)fill[coo.rows[e]]++] = (int64_t)e;
+  }
+  GlobalCRS g;
+  g.rowPtr.assign((size_t)n + 1, 0);
+  g.diag.assign((size_t)n, 0.0f);
+  g.col.reserve(ne);
+  g.off.reserve(ne);
+  std::vector<std::pair<int64_t, float>> rowE;
+  for (int64_t r = 0; r < n; r++) {
+    rowE.clear();
+    for (int64_t q = cnt[r]; q < cnt[r + 1]; q++) {
+      int64_t e = ord[(size_t)q];
+      rowE.emplace_back(coo.cols[e], coo.vals[e]);
+    }
+    std::stable_sort(rowE.begin(), rowE.end(),
+                     [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (size_t q = 0; q < rowE.size(); q++) {
+      int64_t c = rowE[q].first;
+      float v = rowE[q].second;
+      if (c == r) {
+        g.diag[(size_t)r] = g.diag[(size_t)r] + v;
+        continue;
+      }
+      if ((int64_t)g.col.size() > g.rowPtr[(size_t)r] && g.col.back() == c) {
+        g.off.back() = g.off.back() + v;
+        continue;
+      }
+      g.col.push_back((int32_t)c);
+      g.off.push_back(v);
+    }
+    g.rowPtr[(size_t)r + 1] = (int64_t)g.col.size();
+  }
+  if (g.rowPtr[(size_t)n] >= (int64_t)INT32_MAX)
+    fail(kInvalid, "too many non-zeros for int32 offsets");
+  coo = TripletMatrix();
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Tile layouts (CRSHostMatrix.cpp:273-354, DistributedTileLayout.cpp:63-126),
+// local addressing (CRSHostMatrix.cpp:379-432) and value decomposition
+// (CRSHostMatrix.cpp:433-525). Rows are bucketed by tile instead of scanning
+// every row per tile (same per-tile ascending order, O(n) instead of O(T n)).
+// Halo regions of a tile are ordered by (source tile, source region index).
+// ---------------------------------------------------------------------------
+static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
+  const int64_t n = m.nrows;
+  const int32_t T = m.numTiles;
+  m.tileStart.assign((size_t)T + 1, 0);
+  m.newToOld.assign((size_t)n, 0);
+  m.oldToNew.assign((size_t)n, 0);
+  m.numInterior.assign((size_t)T, 0);
+  m.numSeparatorRegions.assign((size_t)T, 0);
+  m.haloStart.assign((size_t)T + 1, 0);
+  m.haloGlobal.clear();
+
+  if (T == 1) {
+    // CRSHostMatrix.cpp:151-158: a single tile holds every row, all interior.
+    std::iota(m.newToOld.begin(), m.newToOld.end(), 0);
+    std::iota(m.oldToNew.begin(), m.oldToNew.end(), 0);
+    m.tileStart[1] = n;
+    m.numInterior[0] = (int32_t)n;
+  } else {
+    struct Region {
+      std::vector<int32_t> dst;
+      std::vector<int64_t> cells;
+    };
+    std::vector<std::vector<int64_t>> interior((size_t)T);
+    std::vector<std::vector<Region>> seps((size_t)T);
+    std::vector<std::map<std::vector<int32_t>, int32_t>> lookup((size_t)T);
+    std::vector<int32_t> key;
+    for (int64_t r = 0; r < n; r++) {
+      int32_t t = m.rowToTile[(size_t)r];
+      key.clear();
+      for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
+        int32_t ct = m.rowToTile[(size_t)g.col[(size_t)a]];
+        if (ct != t) key.push_back(ct);
+      }
+      if (key.empty()) {
+        interior[(size_t)t].push_back(r);
+        continue;
+      }
+      std::sort(key.begin(), key.end());
+      key.erase(std::unique(key.begin(), key.end()), key.end());
+      auto& lk = lookup[(size_t)t];
+      auto it = lk.find(key);
+      int32_t reg;
+      if (it == lk.end()) {
+        reg = (int32_t)seps[(size_t)t].size();
+        seps[(size_t)t].push_back(Region{key, {}});
+        lk.emplace(key, reg);
+      } else {
+        reg = it->second;
+      }
+      seps[(size_t)t][(size_t)reg].cells.push_back(r);
+    }
+    std::vector<std::vector<int64_t>> halo((size_t)T);
+    for (int32_t s = 0; s < T; s++)
+      for (auto& R : seps[(size_t)s])
+        for (int32_t d : R.dst)
+          halo[(size_t)d].insert(halo[(size_t)d].end(), R.cells.begin(),
+                                 R.cells.end());
+    int64_t pos = 0;
+    for (int32_t t = 0; t < T; t++) {
+      m.tileStart[(size_t)t] = pos;
+      m.numInterior[(size_t)t] = (int32_t)interior[(size_t)t].size();
+      m.numSeparatorRegions[(size_t)t] = (int32_t)seps[(size_t)t].size();
+      for (int64_t r : interior[(size_t)t]) {
+        m.newToOld[(size_t)pos] = r;
+        m.oldToNew[(size_t)r] = pos++;
+      }
+      for (auto& R : seps[(size_t)t])
+        for (int64_t r : R.cells) {
+          m.newToOld[(size_t)pos] = r;
+          m.oldToNew[(size_t)r] = pos++;
+        }
+      if (m.tileStart[(size_t)t] == pos)
+        fail(kInvalid, "tile " + std::to_string(t) + " holds no rows");
+    }
+    m.tileStart[(size_t)T] = pos;
+    int64_t h = 0;
+    for (int32_t t = 0; t < T; t++) {
+      m.haloStart[(size_t)t] = h;
+      m.haloGlobal.insert(m.haloGlobal.end(), halo[(size_t)t].begin(),
+                          halo[(size_t)t].end());
+      h += (int64_t)halo[(size_t)t].size();
+    }
+    m.haloStart[(size_t)T] = h;
+  }
+
+  // local addressing + values in local column order
+  const int64_t nnz = g.rowPtr[(size_t)n];
+  m.rowPtr.assign((size_t)n + 1, 0);
+  m.col.resize((size_t)nnz);
+  m.colLocal.resize((size_t)nnz);
+  m.offDiag.resize((size_t)nnz);
+  m.diag.resize((size_t)n);
+  std::vector<int64_t> haloLocal;
+  if (T > 1) haloLocal.assign((size_t)n, -1);
+  struct E {
+    int64_t local;
+    int32_t pos;
+    float v;
+  };
+  std::vector<E> es;
+  int64_t c = 0;
+  for (int32_t t = 0; t < T; t++) {
+    const int64_t t0 = m.tileStart[(size_t)t];
+    const int64_t nt = m.tileStart[(size_t)t + 1] - t0;
+    if (T > 1)
+      for (int64_t q = m.haloStart[(size_t)t]; q < m.haloStart[(size_t)t + 1]; q++)
+        haloLocal[(size_t)m.haloGlobal[(size_t)q]] = nt + (q - m.haloStart[(size_t)t]);
+    for (int64_t lr = 0; lr < nt; lr++) {
+      const int64_t p = t0 + lr;
+      const int64_t gr = m.newToOld[(size_t)p];
+      m.diag[(size_t)p] = g.diag[(size_t)gr];
+      es.clear();
+      for (int64_t a = g.rowPtr[(size_t)gr]; a < g.rowPtr[(size_t)gr + 1]; a++) {
+        int64_t gc = g.col[(size_t)a];
+        int64_t local;
+        if (T == 1 || m.rowToTile[(size_t)gc] == t)
+          local = m.oldToNew[(size_t)gc] - t0;
+        else
+          local = haloLocal[(size_t)gc];
+        es.push_back(E{local, (int32_t)m.oldToNew[(size_t)gc], g.off[(size_t)a]});
+      }
+      if (T > 1)
+        std::sort(es.begin(), es.end(),
+                  [](const E& x, const E& y) { return x.local < y.local; });
+      for (const E& e : es) {
+        m.col[(size_t)c] = e.pos;
+        m.colLocal[(size_t)c] = (int32_t)e.local;
+        m.offDiag[(size_t)c] = e.v;
+        c++;
+      }
+      m.rowPtr[(size_t)p + 1] = c;
+    }
+    if (T > 1)
+      for (int64_t q = m.haloStart[(size_t)t]; q < m.haloStart[(size_t)t + 1]; q++)
+        haloLocal[(size_t)m.haloGlobal[(size_t)q]] = -1;
+  }
+}
+
+// Level sets (CRSHostMatrix.cpp:527-675): forward level of a row = 1 + the
+// largest level of its in-tile lower neighbours (0 if none), which is the
+// layer the reference's Kahn sweep assigns. Rows are grouped by level,
+// ascending local id inside a level. Backward levels do the same over the
+// in-tile upper neighbours (rows descending) for the backward sweeps.
+static void computeLevels(HostMatrix& m) {
+  const int64_t n = m.nrows;
+  const int32_t T = m.numTiles;
+  m.level.assign((size_t)n, 0);
+  m.colorSortAddr.assign((size_t)n, 0);
+  m.numColors.assign((size_t)T, 0);
+  m.colorStart.clear();
+  m.colorStartOffset.assign((size_t)T + 1, 0);
+  m.upperLevelRows.assign((size_t)n, 0);
+  m.upperColorStart.clear();
+  m.upperColorStartOffset.assign((size_t)T + 1, 0);
+  m.numUpperColors.assign((size_t)T, 0);
+  std::vector<int32_t> ulev((size_t)n, 0);
+  int64_t parRows = 0;
+  std::vector<int64_t> cnt;
+  for (int32_t t = 0; t < T; t++) {
+    const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
+    int32_t maxL = 0;
+    for (int64_t p = t0; p < t1; p++) {
+      int32_t L = 0;
+      for (int64_t a = m.rowPtr[(size_t)p]; a < m.rowPtr[(size_t)p + 1]; a++) {
+        int64_t j = m.col[(size_t)a];
+        if (j < t0 || j >= t1 || j >= p) break;  // halo ids sort last
+        L = std::max(L, m.level[(size_t)j] + 1);
+      }
+      m.level[(size_t)p] = L;
+      maxL = std::max(maxL, L);
+    }
+    int32_t maxU = 0;
+    for (int64_t p = t1 - 1; p >= t0; p--) {
+      int32_t L = 0;
+      for (int64_t a = m.rowPtr[(size_t)p]; a < m.rowPtr[(size_t)p + 1]; a++) {
+        int64_t j = m.col[(size_t)a];
+        if (j < t0 || j >= t1) break;
+        if (j > p) L = std::max(L, ulev[(size_t)j] + 1);
+      }
+      ulev[(size_t)p] = L;
+      maxU = std::max(maxU, L);
+    }
+    auto group = [&](const std::vector<int32_t>& lv, int32_t maxLev,
+                     std::vector<int32_t>& rows, std::vector<int64_t>& starts,
+                     bool countPar) {
+      cnt.assign((size_t)maxLev + 2, 0);
+      for (int64_t p = t0; p < t1; p++) cnt[(size_t)lv[(size_t)p] + 1]++;
+      for (int32_t L = 0; L <= maxLev; L++) {
+        if (countPar && cnt[(size_t)L + 1] > 12) parRows += cnt[(size_t)L + 1];
+        cnt[(size_t)L + 1] += cnt[(size_t)L];
+      }
+      for (int32_t L = 0; L <= maxLev + 1; L++) starts.push_back(cnt[(size_t)L]);
+      std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+      for (int64_t p = t0; p < t1; p++)
+        rows[(size_t)(t0 + fill[(size_t)lv[(size_t)p]]++)] = (int32_t)(p - t0);
+    };
+    m.colorStartOffset[(size_t)t] = (int64_t)m.colorStart.size();
+    m.numColors[(size_t)t] = maxL + 1;
+    group(m.level, maxL, m.colorSortAddr, m.colorStart, true);
+    m.upperColorStartOffset[(size_t)t] = (int64_t)m.upperColorStart.size();
+    m.numUpperColors[(size_t)t] = maxU + 1;
+    group(ulev, maxU, m.upperLevelRows, m.upperColorStart, false);
+  }
+  m.colorStartOffset[(size_t)T] = (int64_t)m.colorStart.size();
+  m.upperColorStartOffset[(size_t)T] = (int64_t)m.upperColorStart.size();
+  // "multicolor recommended" if > 90% of rows sit in levels with > 12 rows
+  m.multicolorRecommended = (double)parRows > 0.9 * (double)n;
+}
+
+std::shared_ptr<HostMatrix> buildHostMatrix(TripletMatrix&& coo,
+                                            int32_t numTiles,
+                                            const int32_t* rowToTile) {
+  if (numTiles < 1) fail(kInvalid, "numTiles must be >= 1");
+  auto m = std::make_shared<HostMatrix>();
+  m->nrows = coo.nrows;
+  GlobalCRS g = convertToCRS(std::move(coo));
+  const int64_t n = m->nrows;
+  if (numTiles > n) fail(kInvalid, "more tiles than rows");
+  if (numTiles == 1) {
+    m->rowToTile.assign((size_t)n, 0);
+  } else if (rowToTile) {
+    m->rowToTile.assign(rowToTile, rowToTile + n);
+    // compact away empty tiles (CRSHostMatrix.cpp:238-264)
+    std::vector<int64_t> perTile((size_t)numTiles, 0);
+    for (int32_t t : m->rowToTile) {
+      if (t < 0 || t >= numTiles) fail(kInvalid, "rowToTile entry out of range");
+      perTile[(size_t)t]++;
+    }
+    std::vector<int32_t> shift((size_t)numTiles, -1);
+    int32_t k = 0;
+    for (int32_t t = 0; t < numTiles; t++)
+      if (perTile[(size_t)t] > 0) shift[(size_t)t] = k++;
+    for (auto& t : m->rowToTile) t = shift[(size_t)t];
+    numTiles = k;
+  } else {
+    m->rowToTile = partitionContiguous(g.rowPtr, numTiles);
+  }
+  m->numTiles = numTiles;
+  computeLayout(*m, g);
+  computeLevels(*m);
+  return m;
+}
+
+LDUHost buildLDU(const HostMatrix& m) {
+  const int64_t n = m.nrows;
+  LDUHost L;
+  L.ownerStart.assign((size_t)n + 1, 0);
+  L.neighbourStart.assign((size_t)n + 1, 0);
+  // faces: owner ascending, neighbour ascending
+  std::vector<std::pair<int32_t, float>> ups;
+  std::vector<int64_t> rowUpperBegin((size_t)n + 1, 0);
+  for (int64_t l = 0; l < n; l++) {
+    L.ownerStart[(size_t)l] = (int32_t)L.upperAddr.size();
+    ups.clear();
+    for (int64_t a = m.rowPtr[(size_t)l]; a < m.rowPtr[(size_t)l + 1]; a++)
+      if (m.col[(size_t)a] > l) ups.emplace_back(m.col[(size_t)a], m.offDiag[(size_t)a]);
+    std::sort(ups.begin(), ups.end(),
+              [](const auto& x, const auto& y) { return x.first < y.first; });
+    for (auto& u : ups) {
+      L.lowerAddr.push_back((int32_t)l);
+      L.upperAddr.push_back(u.first);
+      L.upper.push_back(u.second);
+    }
+  }
+  L.ownerStart[(size_t)n] = (int32_t)L.upperAddr.size();
+  L.numFaces = (int64_t)L.upperAddr.size();
+  // losort: per neighbour cell, faces ordered by face index == owner order
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t f = 0; f < L.numFaces; f++) cnt[(size_t)L.upperAddr[(size_t)f] + 1]++;
+  for (int64_t c = 0; c < n; c++) cnt[(size_t)c + 1] += cnt[(size_t)c];
+  for (int64_t c = 0; c <= n; c++) L.neighbourStart[(size_t)c] = (int32_t)cnt[(size_t)c];
+  L.neighbourColInd.assign((size_t)L.numFaces, 0);
+  L.lower.assign((size_t)L.numFaces, 0.0f);
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t f = 0; f < L.numFaces; f++)
+    L.neighbourColInd[(size_t)fill[(size_t)L.upperAddr[(size_t)f]]++] = (int32_t)f;
+  // lower coefficient A[u][l] of every face, in losort order
+  int64_t lowerCount = 0;
+  for (int64_t u = 0; u < n; u++) {
+    for (int64_t a = m.rowPtr[(size_t)u]; a < m.rowPtr[(size_t)u + 1]; a++) {
+      int64_t l = m.col[(size_t)a];
+      if (l >= u) continue;
+      lowerCount++;
+      // face (l, u): search l's owned faces
+      int32_t f0 = L.ownerStart[(size_t)l], f1 = L.ownerStart[(size_t)l + 1];
+      auto it = std::lower_bound(L.upperAddr.begin() + f0, L.upperAddr.begin() + f1,
+                                 (int32_t)u);
+      if (it == L.upperAddr.begin() + f1 || *it != (int32_t)u)
+        fail(kInvalid, "LDU format needs a structurally symmetric matrix");
+      int64_t f = it - L.upperAddr.begin();
+      // position of face f inside u's losort segment (faces ascending)
+      int32_t s0 = L.neighbourStart[(size_t)u], s1 = L.neighbourStart[(size_t)u + 1];
+      auto is = std::lower_bound(L.neighbourColInd.begin() + s0,
+                                 L.neighbourColInd.begin() + s1, (int32_t)f);
+      L.lower[(size_t)(is - L.neighbourColInd.begin())] = m.offDiag[(size_t)a];
+    }
+  }
+  if (lowerCount != L.numFaces)
+    fail(kInvalid, "LDU format needs a structurally symmetric matrix");
+  return L;
+}
+
+}  // namespace graphene::matrix::host

@@ -1,0 +1,210 @@
+// Device-side helpers shared by the gfx950 kernels: the solver state block
+// that lives in HBM, deterministic block / grid reductions, and the
+// XCD-aware block remap.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gph {
+
+// Scalar state of one Krylov solve. All scalars are stored as double; a
+// value computed in float working precision is exactly representable, so
+// kernels re-derive the working-precision value by a cast.
+struct DevState {
+  double rho, rhoOld, alpha, omega;
+  double r0n, rF, bNorm, aux;
+  int32_t iter, converged, singular, done;
+  float absTol, relTol, relRes;
+  int32_t minIt, maxIt;
+  int32_t pad[3];
+  uint32_t counter[8];
+};
+
+// What the last block of a fused reduction does with the totals.
+enum FinOp : int32_t {
+  kFinNone = 0,
+  kFinStore = 1,      // aux = sum0 (raw sum)
+  kFinNormAux = 2,    // aux = sqrt(sum0)
+  kFinBNorm = 3,      // bNorm = sqrt(sum0)
+  kFinInit = 4,       // PBiCGStab start: r0n = rF = ||r||, rho = r.r
+  kFinAlpha = 5,      // alpha = rho / sum0
+  kFinOmega = 6,      // omega = sum1 / sum0, singularity check
+  kFinUpdate = 7,     // end of iteration: ||r||, convergence, next rho
+};
+
+// Singularity threshold Traits<float>::vsmall() (lg/common/Traits.hpp:38-46).
+constexpr double kVSmall = 1e-37;
+
+template <typename T>
+__device__ __forceinline__ float toF(T v) {
+  return (float)v;
+}
+template <>
+__device__ __forceinline__ float toF<__half>(__half v) {
+  return __half2float(v);
+}
+template <typename A, typename V>
+__device__ __forceinline__ A cvt(V v) {
+  return (A)v;
+}
+template <>
+__device__ __forceinline__ float cvt<float, __half>(__half v) {
+  return __half2float(v);
+}
+template <>
+__device__ __forceinline__ double cvt<double, __half>(__half v) {
+  return (double)__half2float(v);
+}
+
+// Blocks are dispatched to the 8 XCDs round robin (block b -> XCD b % 8).
+// Remap so that each XCD works on one contiguous range of logical blocks,
+// keeping the gathered x window of neighbouring rows in one L2.
+__device__ __forceinline__ int xcdRemap(int b, int nb) {
+  const int q = nb >> 3, rem = nb & 7;
+  const int x = b & 7, i = b >> 3;
+  return x * q + min(x, rem) + i;
+}
+
+// Deterministic block sum of K values over a 1-D block of NT threads
+// (NT multiple of 64). Result valid in thread 0.
+template <typename R, int K, int NT>
+__device__ __forceinline__ void blockSum(R (&v)[K], R* lds /* K*NT/64 */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    R s = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s = s + __shfl_down(s, off, 64);
+    if (lane == 0) lds[k * (NT / 64) + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      R s = lds[k * (NT / 64)];
+      for (int w = 1; w < NT / 64; w++) s = s + lds[k * (NT / 64) + w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid-wide reduction epilogue: every block publishes its K partial sums;
+// the last block to arrive adds all partials in a fixed order and returns
+// true (in all of its threads) with the totals in `tot`. The arrival
+// counter is reset by the last block, so no memset is needed between uses.
+template <typename R, int K, int NT>
+__device__ bool gridReduce(R (&v)[K], R* partials, uint32_t* counter,
+                           R (&tot)[K], R* lds) {
+  __shared__ int isLast;
+  blockSum<R, K, NT>(v, lds);
+  const int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) partials[k * nb + blockIdx.x] = v[k];
+    __threadfence();
+    uint32_t prev = atomicAdd(counter, 1u);
+    isLast = (prev == (uint32_t)(nb - 1));
+  }
+  __syncthreads();
+  if (!isLast) return false;
+  __threadfence();
+  R acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    R s = 0;
+    for (int b = threadIdx.x; b < nb; b += NT)
+      s = s + partials[k * nb + b];
+    acc[k] = s;
+  }
+  blockSum<R, K, NT>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = acc[k];
+    *counter = 0u;
+  }
+  return true;
+}
+
+// Convergence test, SolverStats::checkConvergence
+// (lg/matrix/solver/SolverStats.cpp:33-56).
+__device__ __forceinline__ bool convergedTest(const DevState* s, double rF) {
+  bool c = rF < (double)s->absTol || rF < s->r0n * (double)s->relRes;
+  if (s->relTol > 0) c = c || rF < s->bNorm * (double)s->relTol;
+  return c;
+}
+
+// Scalar epilogues of the fused kernels. X is the working precision.
+template <typename X>
+__device__ void finalize(DevState* s, int op, double sum0, double sum1) {
+  switch (op) {
+    case kFinStore:
+      s->aux = sum0;
+      break;
+    case kFinNormAux:
+      s->aux = (double)sqrt((X)sum0);
+      break;
+    case kFinBNorm:
+      s->bNorm = (double)sqrt((X)sum0);
+      break;
+    case kFinInit: {
+      // pbicgstab/Solver.cpp:66-90: initial residual, rho of the first
+      // iteration (r0.r == r.r), loop-entry condition.
+      double n0 = (double)sqrt((X)sum0);
+      s->r0n = n0;
+      s->rF = n0;
+      s->iter = 0;
+      s->converged = 0;
+      s->singular = 0;
+      s->done = (s->maxIt <= 0) ? 1 : 0;
+      if (!s->done) {
+        s->rho = sum0;
+        if (fabs(sum0) < kVSmall) {
+          s->singular = 1;
+          s->done = 1;
+        }
+      }
+      break;
+    }
+    case kFinAlpha:
+      // pbicgstab/Solver.cpp:121-122
+      s->alpha = (double)((X)s->rho / (X)sum0);
+      break;
+    case kFinOmega: {
+      // pbicgstab/Solver.cpp:151-159
+      X om = (X)sum1 / (X)sum0;
+      s->omega = (double)om;
+      if (fabs((double)om) < kVSmall) {
+        s->singular = 1;
+        s->done = 1;
+      }
+      break;
+    }
+    case kFinUpdate: {
+      // pbicgstab/Solver.cpp:170-180 and the loop condition :86-90; the
+      // next iteration's rho (:91-95) is the second sum.
+      double rF = (double)sqrt((X)sum0);
+      s->rF = rF;
+      s->converged = convergedTest(s, rF) ? 1 : 0;
+      s->rhoOld = s->rho;
+      s->iter = s->iter + 1;
+      bool term = (s->converged && s->iter >= s->minIt) || s->iter >= s->maxIt;
+      if (!term) {
+        s->rho = sum1;
+        if (fabs(sum1) < kVSmall) {
+          s->singular = 1;
+          term = true;
+        }
+      }
+      s->done = term ? 1 : 0;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+}  // namespace gph

@@ -1,0 +1,322 @@
+// Tile-local preconditioners for gfx950: ILU(0) factorization and
+// substitution, Gauss-Seidel sweeps.
+//
+// The reference runs these per IPU tile on the tile's non-halo block
+// (block-Jacobi across tiles): ILU factorization lg/matrix/solver/ilu/Solver.cpp:134-292,
+// substitution :367-550, Gauss-Seidel lg/matrix/solver/gauss-seidel/Solver.cpp:45-86
+// (arithmetic of codelet lg/codelet/ipu/matrix/solvers/gauss-seidel.cpp:41-61).
+// Here one workgroup owns one tile and walks the tile's level sets (rows of
+// a level are independent), with a barrier between levels; the tile's
+// solution vector sits in LDS when it fits. Per row the arithmetic is the
+// reference's sequential row body, so results equal the sequential
+// substitution bit for bit.
+#include "kernels.hpp"
+
+namespace gph {
+
+size_t tileSharedBytes(const TileArgs& t, DType xt) {
+  return (size_t)t.maxTileRows * dtypeSize(xt);
+}
+
+static size_t maxDynamicShared() { return 160 * 1024 - 4096; }
+
+// ---------------------------------------------------------------------------
+// ILU(0) factorization (Saad Alg. 10.4, IKJ), per tile, level by level.
+// Row i: for each k = col(ik) < i in-tile: a_ik /= a_kk; a_ij -= a_ik*a_kj
+// for the later in-tile entries j of row i (a_kj = 0 if absent); then
+// a_ii -= a_ik*a_ki. ilu/Solver.cpp:167-218. Afterwards diag = 1/diag
+// (:290-291).
+// ---------------------------------------------------------------------------
+template <typename FT>
+__global__ __launch_bounds__(kTileThreads) void iluFactorKernel(TileArgs t,
+                                                                FT* off,
+                                                                FT* diag) {
+  const int tile = blockIdx.x;
+  const int t0 = t.tileStart[tile], t1 = t.tileStart[tile + 1];
+  const int* lp = t.lvlPtr + t.lvlPtrOff[tile];
+  const int nL = t.lvlPtrOff[tile + 1] - t.lvlPtrOff[tile] - 1;
+  const int32_t* __restrict__ rowPtr = t.rowPtr;
+  const int32_t* __restrict__ col = t.col;
+  for (int L = 0; L < nL; L++) {
+    for (int q = lp[L] + threadIdx.x; q < lp[L + 1]; q += kTileThreads) {
+      const int i = t0 + t.lvlRows[t0 + q];
+      const int iEnd = rowPtr[i + 1];
+      for (int ik = rowPtr[i]; ik < iEnd; ik++) {
+        const int k = col[ik];
+        if (k >= i || k < t0) break;  // upper part or halo reached
+        const FT a_kk = diag[k];
+        const FT a_ik = off[ik] / a_kk;
+        off[ik] = a_ik;
+        int qk = rowPtr[k];
+        const int kEnd = rowPtr[k + 1];
+        for (int ij = ik + 1; ij < iEnd; ij++) {
+          const int j = col[ij];
+          if (j < t0 || j >= t1) break;  // stop at halo coefficients
+          while (qk < kEnd) {
+            const int c = col[qk];
+            if (c < t0 || c >= t1 || c >= j) break;
+            qk++;
+          }
+          const FT a_kj = (qk < kEnd && col[qk] == j) ? off[qk] : (FT)0;
+          off[ij] = off[ij] - a_ik * a_kj;
+        }
+        FT a_ki = 0;
+        for (int qq = rowPtr[k]; qq < kEnd; qq++) {
+          const int c = col[qq];
+          if (c < t0 || c >= t1 || c > i) break;
+          if (c == i) a_ki = off[qq];
+        }
+        diag[i] = diag[i] - a_ik * a_ki;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = t0 + threadIdx.x; i < t1; i += kTileThreads)
+    diag[i] = (FT)1 / diag[i];
+}
+
+void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
+                     hipStream_t s) {
+  if (t.T <= 0) return;
+  if (ft == DType::F32)
+    hipLaunchKernelGGL(iluFactorKernel<float>, dim3(t.T), dim3(kTileThreads),
+                       0, s, t, static_cast<float*>(off),
+                       static_cast<float*>(diag));
+  else if (ft == DType::F64)
+    hipLaunchKernelGGL(iluFactorKernel<double>, dim3(t.T), dim3(kTileThreads),
+                       0, s, t, static_cast<double*>(off),
+                       static_cast<double*>(diag));
+  else
+    fail(kUnsupported, "ILU factorization type must be float32 or float64");
+  GPH_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// ILU(0) substitution per tile (ilu/Solver.cpp:380-442, :526-548):
+// forward  t = b_i; t -= L_a*x_j for in-tile j < i in stored order; x_i = t
+// backward t = 0;  t -= U_a*x_j for in-tile j > i, entries descending;
+//          x_i = (x_i + t) * invD_i
+// Forward levels ascending, then backward levels (levels of the upper
+// triangle) ascending. The right-hand side can be formed on the fly:
+// kPreP: p = r + beta*(p - omega*v) (iter > 0), kPreS: s = r - alpha*v.
+// ---------------------------------------------------------------------------
+template <typename FT, typename X, bool USE_LDS>
+__global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
+    TileArgs t, const FT* __restrict__ lu, const FT* __restrict__ invD,
+    const X* __restrict__ rhs, X* __restrict__ out, PreOp pre, DevState* st,
+    int checkDone) {
+  extern __shared__ unsigned char smemRaw[];
+  if (checkDone && st->done) return;
+  const int tile = blockIdx.x;
+  const int t0 = t.tileStart[tile], t1 = t.tileStart[tile + 1];
+  X* xs = USE_LDS ? reinterpret_cast<X*>(smemRaw) : out + t0;
+  const int32_t* __restrict__ rowPtr = t.rowPtr;
+  const int32_t* __restrict__ col = t.col;
+
+  X beta = 0, omega = 0, alpha = 0;
+  bool pupd = false;
+  if (pre.kind == kPreP && st) {
+    pupd = st->iter != 0;
+    beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+    omega = (X)st->omega;
+  } else if (pre.kind == kPreS && st) {
+    alpha = (X)st->alpha;
+  }
+  const X* r = static_cast<const X*>(pre.r);
+  X* pv = static_cast<X*>(pre.p);
+  const X* vv = static_cast<const X*>(pre.v);
+
+  const int* lp = t.lvlPtr + t.lvlPtrOff[tile];
+  const int nL = t.lvlPtrOff[tile + 1] - t.lvlPtrOff[tile] - 1;
+  for (int L = 0; L < nL; L++) {
+    for (int q = lp[L] + threadIdx.x; q < lp[L + 1]; q += kTileThreads) {
+      const int li = t.lvlRows[t0 + q];
+      const int i = t0 + li;
+      X s;
+      if (pre.kind == kPreP) {
+        X pi = pv[i];
+        if (pupd) {
+          pi = r[i] + beta * (pi - omega * vv[i]);
+          pv[i] = pi;
+        }
+        s = pi;
+      } else if (pre.kind == kPreS) {
+        s = r[i] - alpha * vv[i];
+        pv[i] = s;
+      } else {
+        s = rhs[i];
+      }
+      const int e = rowPtr[i + 1];
+      for (int a = rowPtr[i]; a < e; a++) {
+        const int j = col[a];
+        if (i < j) break;
+        if (j < t0 || j >= t1) break;
+        s = s - cvt<X>(lu[a]) * xs[j - t0];
+      }
+      xs[li] = s;
+    }
+    __syncthreads();
+  }
+  const int* up = t.ulvlPtr + t.ulvlPtrOff[tile];
+  const int nU = t.ulvlPtrOff[tile + 1] - t.ulvlPtrOff[tile] - 1;
+  for (int L = 0; L < nU; L++) {
+    for (int q = up[L] + threadIdx.x; q < up[L + 1]; q += kTileThreads) {
+      const int li = t.ulvlRows[t0 + q];
+      const int i = t0 + li;
+      X s = 0;
+      const int b = rowPtr[i];
+      for (int a = rowPtr[i + 1] - 1; a >= b; a--) {
+        const int j = col[a];
+        if (j < t0 || j >= t1) continue;
+        if (j < i) break;
+        s = s - cvt<X>(lu[a]) * xs[j - t0];
+      }
+      xs[li] = (xs[li] + s) * cvt<X>(invD[i]);
+    }
+    __syncthreads();
+  }
+  if (USE_LDS)
+    for (int q = threadIdx.x; q < t1 - t0; q += kTileThreads)
+      out[t0 + q] = xs[q];
+}
+
+template <typename FT, typename X>
+static void iluApplyTyped(const TileArgs& t, const void* lu, const void* invD,
+                          const void* rhs, void* out, const PreOp& pre,
+                          DevState* st, int checkDone, hipStream_t s) {
+  const size_t shm = (size_t)t.maxTileRows * sizeof(X);
+  if (shm <= maxDynamicShared()) {
+    auto k = iluApplyKernel<FT, X, true>;
+    if (shm > 64 * 1024)
+      GPH_HIP(hipFuncSetAttribute((const void*)k,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm));
+    hipLaunchKernelGGL(k, dim3(t.T), dim3(kTileThreads), shm, s, t,
+                       static_cast<const FT*>(lu),
+                       static_cast<const FT*>(invD),
+                       static_cast<const X*>(rhs), static_cast<X*>(out), pre,
+                       st, checkDone);
+  } else {
+    hipLaunchKernelGGL((iluApplyKernel<FT, X, false>), dim3(t.T),
+                       dim3(kTileThreads), 0, s, t,
+                       static_cast<const FT*>(lu),
+                       static_cast<const FT*>(invD),
+                       static_cast<const X*>(rhs), static_cast<X*>(out), pre,
+                       st, checkDone);
+  }
+  GPH_HIP(hipGetLastError());
+}
+
+void launchIluApply(const TileArgs& t, const void* lu, const void* invD,
+                    DType ft, const void* rhs, void* out, DType xt,
+                    const PreOp& pre, DevState* st, int checkDone,
+                    hipStream_t s) {
+  if (t.T <= 0) return;
+  if (ft == DType::F32 && xt == DType::F32)
+    iluApplyTyped<float, float>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F16 && xt == DType::F32)
+    iluApplyTyped<__half, float>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F64 && xt == DType::F64)
+    iluApplyTyped<double, double>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F32 && xt == DType::F64)
+    iluApplyTyped<float, double>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+  else
+    fail(kUnsupported, std::string("unsupported ILU apply types: factors ") +
+                           dtypeName(ft) + ", vectors " + dtypeName(xt));
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel sweep per tile. Forward: rows in local order,
+// x_i = (b_i - sum_{j != i} a_ij x_j) / a_ii with the new value for in-tile
+// j < i and the previous value (xin) otherwise (halo included). Backward:
+// reverse local order, new values for in-tile j > i.
+// ---------------------------------------------------------------------------
+template <typename VT, typename X, bool BACKWARD, bool USE_LDS>
+__global__ __launch_bounds__(kTileThreads) void gsSweepKernel(
+    TileArgs t, const VT* __restrict__ off, const VT* __restrict__ diag,
+    const X* __restrict__ b, const X* __restrict__ xin, X* __restrict__ xout,
+    DevState* st, int checkDone) {
+  extern __shared__ unsigned char smemRaw[];
+  if (checkDone && st->done) return;
+  const int tile = blockIdx.x;
+  const int t0 = t.tileStart[tile], t1 = t.tileStart[tile + 1];
+  X* xs = USE_LDS ? reinterpret_cast<X*>(smemRaw) : xout + t0;
+  const int32_t* __restrict__ rowPtr = t.rowPtr;
+  const int32_t* __restrict__ col = t.col;
+  const int* lp = BACKWARD ? t.ulvlPtr + t.ulvlPtrOff[tile]
+                           : t.lvlPtr + t.lvlPtrOff[tile];
+  const int32_t* rows = BACKWARD ? t.ulvlRows : t.lvlRows;
+  const int nL = BACKWARD
+                     ? t.ulvlPtrOff[tile + 1] - t.ulvlPtrOff[tile] - 1
+                     : t.lvlPtrOff[tile + 1] - t.lvlPtrOff[tile] - 1;
+  for (int L = 0; L < nL; L++) {
+    for (int q = lp[L] + threadIdx.x; q < lp[L + 1]; q += kTileThreads) {
+      const int li = rows[t0 + q];
+      const int i = t0 + li;
+      X s = b[i];
+      const int e = rowPtr[i + 1];
+      for (int a = rowPtr[i]; a < e; a++) {
+        const int j = col[a];
+        const bool inTile = j >= t0 && j < t1;
+        const bool fresh = inTile && (BACKWARD ? (j > i) : (j < i));
+        const X xj = fresh ? xs[j - t0] : xin[j];
+        s = s - cvt<X>(off[a]) * xj;
+      }
+      xs[li] = s / cvt<X>(diag[i]);
+    }
+    __syncthreads();
+  }
+  if (USE_LDS)
+    for (int q = threadIdx.x; q < t1 - t0; q += kTileThreads)
+      xout[t0 + q] = xs[q];
+}
+
+template <typename VT, typename X>
+static void gsTyped(const TileArgs& t, const void* off, const void* diag,
+                    const void* b, const void* xin, void* xout, bool backward,
+                    DevState* st, int checkDone, hipStream_t s) {
+  const size_t shm = (size_t)t.maxTileRows * sizeof(X);
+  const bool lds = shm <= maxDynamicShared();
+  auto launch = [&](auto kern, size_t bytes) {
+    if (bytes > 64 * 1024)
+      GPH_HIP(hipFuncSetAttribute((const void*)kern,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes));
+    hipLaunchKernelGGL(kern, dim3(t.T), dim3(kTileThreads), bytes, s, t,
+                       static_cast<const VT*>(off),
+                       static_cast<const VT*>(diag), static_cast<const X*>(b),
+                       static_cast<const X*>(xin), static_cast<X*>(xout), st,
+                       checkDone);
+  };
+  if (backward) {
+    if (lds)
+      launch(gsSweepKernel<VT, X, true, true>, shm);
+    else
+      launch(gsSweepKernel<VT, X, true, false>, 0);
+  } else {
+    if (lds)
+      launch(gsSweepKernel<VT, X, false, true>, shm);
+    else
+      launch(gsSweepKernel<VT, X, false, false>, 0);
+  }
+  GPH_HIP(hipGetLastError());
+}
+
+void launchGsSweep(const TileArgs& t, const void* off, const void* diag,
+                   DType vt, const void* b, const void* xin, void* xout,
+                   DType xt, bool backward, DevState* st, int checkDone,
+                   hipStream_t s) {
+  if (t.T <= 0) return;
+  if (vt == DType::F32 && xt == DType::F32)
+    gsTyped<float, float>(t, off, diag, b, xin, xout, backward, st, checkDone, s);
+  else if (vt == DType::F32 && xt == DType::F64)
+    gsTyped<float, double>(t, off, diag, b, xin, xout, backward, st, checkDone, s);
+  else if (vt == DType::F64 && xt == DType::F64)
+    gsTyped<double, double>(t, off, diag, b, xin, xout, backward, st, checkDone, s);
+  else if (vt == DType::F16 && xt == DType::F32)
+    gsTyped<__half, float>(t, off, diag, b, xin, xout, backward, st, checkDone, s);
+  else
+    fail(kUnsupported, "unsupported Gauss-Seidel types");
+}
+
+}  // namespace gph

@@ -1,0 +1,188 @@
+// Row-gather kernels for gfx950: CRS and LDU SpMV and residual.
+//
+// Reference kernels: CRSMatrix::spmv / residual (lg/matrix/details/crs/CRSMatrix.cpp:62-80,
+// :107-125) and LDUMatrix::spmv / residual (lg/matrix/details/ldu/LDUMatrix.cpp:62-99,
+// :126-165). Per row: sum = diag*x, then every stored entry in order
+// (CRS: local column order; LDU: owned faces, then losort faces), no FMA
+// contraction (the library is compiled with -ffp-contract=off), so results
+// match a sequential evaluation bit for bit.
+//
+// Layout per block (256 threads, one row per thread, <= 256 rows): the
+// block's contiguous range of stream entries is processed in chunks of
+// kSpmvCap; threads stream column indices and values with coalesced loads,
+// gather x, and stage the products in LDS; each thread then adds the
+// products of its own row in stored order. Blocks are remapped so that each
+// XCD sweeps one contiguous row range (x window stays in that XCD's L2).
+// Optional fused epilogues: residual, extra output copies, dot products
+// with a last-block grid reduction that updates the solver state.
+#include "kernels.hpp"
+
+namespace gph {
+
+template <int S, typename VT, typename XT, typename BT, typename OT,
+          typename AT, int MODE, int RED>
+__global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
+  __shared__ AT prod[kSpmvCap];
+  __shared__ AT redLds[2 * kSpmvThreads / 64];
+  if (a.checkDone && a.state->done) return;
+  const int lb = a.remap ? xcdRemap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int r0 = a.blockRow[lb], r1 = a.blockRow[lb + 1];
+  const int tid = threadIdx.x;
+  const int row = r0 + tid;
+  const bool has = row < r1;
+  const XT* __restrict__ x = static_cast<const XT*>(a.x);
+
+  AT sum = 0;
+  if (has) sum = cvt<AT>(static_cast<const VT*>(a.diag)[row]) * (AT)x[row];
+
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    const int32_t* __restrict__ ptr = a.ptr[s];
+    const int32_t* __restrict__ col = a.col[s];
+    const VT* __restrict__ val = static_cast<const VT*>(a.val[s]);
+    const int e0 = ptr[r0], e1 = ptr[r1];
+    const int mb = has ? ptr[row] : 0;
+    const int me = has ? ptr[row + 1] : 0;
+    for (int c0 = e0; c0 < e1; c0 += kSpmvCap) {
+      const int cn = min(kSpmvCap, e1 - c0);
+      constexpr int U = kSpmvCap / kSpmvThreads;
+      int ci[U];
+      VT vv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = tid + u * kSpmvThreads;
+        if (k < cn) {
+          ci[u] = col[c0 + k];
+          vv[u] = val[c0 + k];
+        }
+      }
+      XT xv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = tid + u * kSpmvThreads;
+        if (k < cn) xv[u] = x[ci[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int k = tid + u * kSpmvThreads;
+        if (k < cn) prod[k] = cvt<AT>(vv[u]) * (AT)xv[u];
+      }
+      __syncthreads();
+      const int lo = max(mb, c0), hi = min(me, c0 + cn);
+      for (int k = lo; k < hi; k++) sum = sum + prod[k - c0];
+      __syncthreads();
+    }
+  }
+
+  AT o = sum;
+  if constexpr (MODE == kModeResidual) {
+    if (has) o = (AT)static_cast<const BT*>(a.b)[row] - sum;
+  }
+  if (has) {
+    const OT ov = (OT)o;
+    static_cast<OT*>(a.out)[row] = ov;
+    if (a.out2) static_cast<OT*>(a.out2)[row] = ov;
+    if (a.out3) static_cast<OT*>(a.out3)[row] = ov;
+    if (a.out64) a.out64[row] = (double)o;
+  }
+  if constexpr (RED != kRedNone) {
+    constexpr int K = (RED == kRedSelfW) ? 2 : 1;
+    AT v[K];
+    AT wv = 0;
+    if constexpr (RED == kRedW || RED == kRedSelfW) {
+      if (has) wv = (AT) static_cast<const XT*>(a.w)[row];
+    }
+    if constexpr (RED == kRedSelf) v[0] = has ? o * o : (AT)0;
+    if constexpr (RED == kRedW) v[0] = has ? o * wv : (AT)0;
+    if constexpr (RED == kRedSelfW) {
+      v[0] = has ? o * o : (AT)0;
+      v[1] = has ? o * wv : (AT)0;
+    }
+    AT tot[K];
+    if (gridReduce<AT, K, kSpmvThreads>(v, static_cast<AT*>(a.partials),
+                                        &a.state->counter[0], tot, redLds)) {
+      if (tid == 0)
+        finalize<AT>(a.state, a.fin, (double)tot[0],
+                     K > 1 ? (double)tot[1] : 0.0);
+    }
+  }
+}
+
+template <int S, typename VT, typename XT, typename BT, typename OT,
+          typename AT, int MODE, int RED>
+static void launchOne(const SpmvArgs& a, hipStream_t s) {
+  if (a.numBlocks <= 0) return;
+  hipLaunchKernelGGL((spmvKernel<S, VT, XT, BT, OT, AT, MODE, RED>),
+                     dim3(a.numBlocks), dim3(kSpmvThreads), 0, s, a);
+  GPH_HIP(hipGetLastError());
+}
+
+template <typename VT, typename XT, typename BT, typename OT, typename AT,
+          int MODE>
+static bool dispatchRed(const SpmvArgs& a, int red, hipStream_t s) {
+  auto go = [&](auto sTag) {
+    constexpr int S = decltype(sTag)::value;
+    switch (red) {
+      case kRedNone:
+        launchOne<S, VT, XT, BT, OT, AT, MODE, kRedNone>(a, s);
+        return true;
+      case kRedSelf:
+        launchOne<S, VT, XT, BT, OT, AT, MODE, kRedSelf>(a, s);
+        return true;
+      case kRedW:
+        launchOne<S, VT, XT, BT, OT, AT, MODE, kRedW>(a, s);
+        return true;
+      case kRedSelfW:
+        launchOne<S, VT, XT, BT, OT, AT, MODE, kRedSelfW>(a, s);
+        return true;
+    }
+    return false;
+  };
+  if (a.nstreams == 1) return go(std::integral_constant<int, 1>{});
+  if (a.nstreams == 2) return go(std::integral_constant<int, 2>{});
+  return false;
+}
+
+// Supported type combinations (value, x, b, out, accumulate).
+void launchSpmv(const SpmvArgs& a, DType vt, DType xt, DType bt, DType ot,
+                DType at, int mode, int red, hipStream_t s) {
+  using F16 = __half;
+  const DType H = DType::F16, F = DType::F32, D = DType::F64;
+  bool ok = false;
+  if (mode == kModeSpmv) {
+    if (vt == F && xt == F && ot == F && at == F)
+      ok = dispatchRed<float, float, float, float, float, kModeSpmv>(a, red, s);
+    else if (vt == H && xt == F && ot == F && at == F)
+      ok = dispatchRed<F16, float, float, float, float, kModeSpmv>(a, red, s);
+    else if (vt == D && xt == D && ot == D && at == D)
+      ok = dispatchRed<double, double, double, double, double, kModeSpmv>(a, red, s);
+    else if (vt == F && xt == D && ot == D && at == D)
+      ok = dispatchRed<float, double, double, double, double, kModeSpmv>(a, red, s);
+    else if (vt == F && xt == F && ot == F && at == D)
+      ok = dispatchRed<float, float, float, float, double, kModeSpmv>(a, red, s);
+  } else if (mode == kModeResidual) {
+    if (vt == F && xt == F && bt == F && ot == F && at == F)
+      ok = dispatchRed<float, float, float, float, float, kModeResidual>(a, red, s);
+    else if (vt == H && xt == F && bt == F && ot == F && at == F)
+      ok = dispatchRed<F16, float, float, float, float, kModeResidual>(a, red, s);
+    else if (vt == D && xt == D && bt == D && ot == D && at == D)
+      ok = dispatchRed<double, double, double, double, double, kModeResidual>(a, red, s);
+    else if (vt == F && xt == D && bt == D && ot == D && at == D)
+      ok = dispatchRed<float, double, double, double, double, kModeResidual>(a, red, s);
+    else if (vt == F && xt == D && bt == F && ot == F && at == D)
+      ok = dispatchRed<float, double, float, float, double, kModeResidual>(a, red, s);
+    else if (vt == F && xt == D && bt == D && ot == F && at == D)
+      ok = dispatchRed<float, double, double, float, double, kModeResidual>(a, red, s);
+    else if (vt == F && xt == D && bt == F && ot == D && at == D)
+      ok = dispatchRed<float, double, float, double, double, kModeResidual>(a, red, s);
+    else if (vt == F && xt == F && bt == F && ot == F && at == D)
+      ok = dispatchRed<float, float, float, float, double, kModeResidual>(a, red, s);
+  }
+  if (!ok)
+    fail(kUnsupported, std::string("unsupported SpMV type combination: value ") +
+                           dtypeName(vt) + ", x " + dtypeName(xt) + ", b " +
+                           dtypeName(bt) + ", out " + dtypeName(ot) +
+                           ", accumulate " + dtypeName(at));
+}
+
+}  // namespace gph

@@ -1,0 +1,126 @@
+// Host-side launch interface of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gph_common.hpp"
+#include "k_common.hpp"
+
+namespace gph {
+
+constexpr int kSpmvThreads = 256;   // one row per thread, <= 256 rows/block
+constexpr int kSpmvCap = 2048;      // staged products per chunk
+constexpr int kVecThreads = 256;
+constexpr int kTileThreads = 256;
+
+// Row-gather operator in "streams" form: CRS has one stream
+// (rowPtr, col, offDiag); LDU has two (owner faces: ownerStart, upperAddr,
+// upper; losort faces: neighbourStart, resolved lower column, lower). Rows
+// sum diag*x first, then each stream in stored order.
+struct SpmvArgs {
+  int32_t n = 0;
+  int32_t numBlocks = 0;
+  const int32_t* blockRow = nullptr;  // numBlocks+1 row boundaries
+  int32_t nstreams = 1;
+  const int32_t* ptr[2] = {nullptr, nullptr};
+  const int32_t* col[2] = {nullptr, nullptr};
+  const void* val[2] = {nullptr, nullptr};
+  const void* diag = nullptr;
+  const void* x = nullptr;
+  const void* b = nullptr;      // residual right-hand side
+  void* out = nullptr;
+  void* out2 = nullptr;         // optional copies of out
+  void* out3 = nullptr;
+  double* out64 = nullptr;      // optional unrounded FP64 result
+  const void* w = nullptr;      // dot partner
+  DevState* state = nullptr;
+  void* partials = nullptr;
+  int32_t fin = kFinNone;
+  int32_t checkDone = 0;
+  int32_t remap = 1;
+};
+
+enum SpmvMode : int32_t { kModeSpmv = 0, kModeResidual = 1 };
+enum RedKind : int32_t { kRedNone = 0, kRedSelf = 1, kRedW = 2, kRedSelfW = 3 };
+
+// Launches the row-gather kernel. vt: value type; xt: x type; bt: b type
+// (residual); ot: output type; at: accumulation type.
+void launchSpmv(const SpmvArgs& a, DType vt, DType xt, DType bt, DType ot,
+                DType at, int mode, int red, hipStream_t s);
+
+// --- vector kernels -------------------------------------------------------
+struct VecArgs {
+  int32_t n = 0;
+  const void* a = nullptr;
+  const void* b = nullptr;
+  const void* c = nullptr;
+  const void* d = nullptr;
+  void* o1 = nullptr;
+  void* o2 = nullptr;
+  DevState* state = nullptr;
+  void* partials = nullptr;
+  int32_t fin = kFinNone;
+  int32_t checkDone = 0;
+};
+int vecGrid(int64_t n);
+// x += alpha*y + omega*z ; r = s - omega*t ; sums r.r and r0.r  (kFinUpdate)
+//   a=y, b=z, c=s, d=t, o1=x, o2=r, w=r0 passed in `extra`
+void launchUpdateXR(const VecArgs& v, const void* r0, DType x, hipStream_t s);
+// p = r + beta*(p - omega*v) when iter > 0; o2 (optional) = p
+//   a=r, c=v, o1=p
+void launchUpdateP(const VecArgs& v, DType x, hipStream_t s);
+// s = r - alpha*v ; o2 (optional) = s   (a=r, c=v, o1=s)
+void launchUpdateS(const VecArgs& v, DType x, hipStream_t s);
+// sum of squares of a (type t), finalised with v.fin into the state
+void launchNorm2(const VecArgs& v, DType t, hipStream_t s);
+void launchFill(void* x, DType t, int64_t n, double value, hipStream_t s);
+void launchCast(const void* in, DType it, void* out, DType ot, int64_t n,
+                hipStream_t s);
+// phi(f64) += d (dt)
+void launchAccumulate(double* phi, const void* d, DType dt, int64_t n,
+                      hipStream_t s);
+
+// --- tile-local preconditioners -------------------------------------------
+struct TileArgs {
+  int32_t T = 0;
+  const int32_t* tileStart = nullptr;   // T+1
+  const int32_t* lvlRows = nullptr;     // local ids grouped by forward level
+  const int32_t* lvlPtr = nullptr;      // per tile numColors+1, relative
+  const int32_t* lvlPtrOff = nullptr;   // T+1
+  const int32_t* ulvlRows = nullptr;    // backward levels
+  const int32_t* ulvlPtr = nullptr;
+  const int32_t* ulvlPtrOff = nullptr;
+  const int32_t* rowPtr = nullptr;
+  const int32_t* col = nullptr;
+  int32_t maxTileRows = 0;
+};
+
+// Right-hand side pre-operation fused into the preconditioner sweep.
+enum PreOpKind : int32_t { kPreNone = 0, kPreP = 1, kPreS = 2 };
+struct PreOp {
+  int32_t kind = kPreNone;
+  const void* r = nullptr;
+  void* p = nullptr;   // p (kPreP, in/out) or s (kPreS, out)
+  const void* v = nullptr;
+};
+
+// In-place ILU(0) factorization of copies of off/diag (ft = f32 or f64),
+// diagonal inverted at the end.
+void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
+                     hipStream_t s);
+// out = (LU)^-1 rhs per tile; ft factor type, xt vector type.
+void launchIluApply(const TileArgs& t, const void* lu, const void* invD,
+                    DType ft, const void* rhs, void* out, DType xt,
+                    const PreOp& pre, DevState* st, int checkDone,
+                    hipStream_t s);
+// One Gauss-Seidel sweep per tile, xout != xin.
+void launchGsSweep(const TileArgs& t, const void* off, const void* diag,
+                   DType vt, const void* b, const void* xin, void* xout,
+                   DType xt, bool backward, DevState* st, int checkDone,
+                   hipStream_t s);
+
+size_t tileSharedBytes(const TileArgs& t, DType xt);
+
+}  // namespace gph

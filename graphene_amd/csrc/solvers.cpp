@@ -1,0 +1,628 @@
+// Solver tree implementation (see solvers.hpp).
+#include "solvers.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cmath>
+
+namespace graphene::matrix::solver {
+
+using gph::DevBuf;
+using gph::DevState;
+using gph::DType;
+using gph::fail;
+using gph::kInvalid;
+using gph::kUnsupported;
+
+// ---------------------------------------------------------------------------
+void SolverStats::print() const {
+  std::fprintf(stderr,
+               "[graphene_amd] %s: Solved on %d tiles, %s norm, Initial residual "
+               "= %.9g, Final residual = %.9g, Iterations = %lld, Converged = %s, "
+               "Singular = %s, Current Execution Time = %.6f s\n",
+               solverName.c_str(), numTiles, normToString(norm).c_str(),
+               initialResidual, finalResidual, (long long)iterations,
+               converged ? "true" : "false", singular ? "true" : "false",
+               seconds);
+}
+
+void Solver::applyDevice(const Vec&, const Vec&, const gph::PreOp&, DevState*,
+                         hipStream_t) {
+  fail(kUnsupported, name() + " cannot be used as a preconditioner");
+}
+
+Vec Workspace::get(int slot, DType t, int64_t n) {
+  if ((int)e_.size() <= slot) e_.resize((size_t)slot + 1);
+  Entry& e = e_[(size_t)slot];
+  if (!e.buf || e.type != t || e.n < n) {
+    e.buf.alloc((size_t)std::max<int64_t>(n, 1) * gph::dtypeSize(t));
+    e.type = t;
+    e.n = n;
+  }
+  return Vec{e.buf.get(), t};
+}
+size_t Workspace::bytes() const {
+  size_t b = 0;
+  for (auto& e : e_) b += e.buf.bytes();
+  return b;
+}
+
+static double nowSeconds() {
+  return std::chrono::duration<double>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// Host-pinned copy of a DevState for asynchronous read-back.
+class PinnedState {
+ public:
+  PinnedState() { GPH_HIP(hipHostMalloc((void**)&p_, 2 * sizeof(DevState), 0)); }
+  ~PinnedState() {
+    if (p_) (void)hipHostFree(p_);
+  }
+  DevState* operator[](int i) { return p_ + i; }
+
+ private:
+  DevState* p_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// ILU(0): factorization in the constructor (ilu/Solver.cpp:44-51), apply per
+// tile. DILU (diagonalBased) is not part of this build.
+// ---------------------------------------------------------------------------
+class IluSolver : public Solver {
+ public:
+  IluSolver(const DeviceMatrix& A, std::shared_ptr<ilu::Configuration> c)
+      : Solver(A), cfg_(std::move(c)) {
+    if (cfg_->diagonalBased)
+      fail(kUnsupported, "DILU (diagonalBased) is not supported by this build");
+    ft_ = cfg_->factorType.value_or(DType::F32);
+    factorize();
+  }
+  std::string name() const override { return cfg_->solverName(); }
+  bool usesInitialGuess() const override { return false; }
+  bool supportsDeviceApply() const override { return true; }
+
+  SolverStats solve(const Vec& x, const Vec& b) override {
+    double t0 = nowSeconds();
+    hipStream_t s = A_.stream();
+    applyDevice(b, x, gph::PreOp{}, nullptr, s);
+    GPH_HIP(hipStreamSynchronize(s));
+    SolverStats st;
+    st.solverName = name();
+    st.norm = VectorNorm::None;
+    st.numTiles = A_.numTiles();
+    st.iterations = 1;  // ilu/Solver.cpp:570-573
+    st.seconds = nowSeconds() - t0;
+    return st;
+  }
+
+  void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,
+                   DevState* st, hipStream_t s) override {
+    const void* lu = lu_.get();
+    const void* inv = invD_.get();
+    DType ft = ft_;
+    if (ft_ == DType::F16) {
+      lu = lu16_.get();
+      inv = invD16_.get();
+    }
+    gph::launchIluApply(A_.tileArgs(), lu, inv, ft, rhs.ptr, out.ptr, out.type,
+                        pre, st, st ? 1 : 0, s);
+  }
+
+  size_t factorBytes() const {
+    return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes();
+  }
+
+ private:
+  void factorize() {
+    hipStream_t s = A_.stream();
+    const int64_t nnz = A_.nnzOff(), n = A_.n();
+    DType work = ft_ == DType::F64 ? DType::F64 : DType::F32;
+    lu_.alloc((size_t)std::max<int64_t>(nnz, 1) * gph::dtypeSize(work));
+    invD_.alloc((size_t)n * gph::dtypeSize(work));
+    gph::launchCast(A_.crsOff(DType::F32), DType::F32, lu_.get(), work, nnz, s);
+    gph::launchCast(A_.crsDiag(DType::F32), DType::F32, invD_.get(), work, n, s);
+    gph::launchIluFactor(A_.tileArgs(), lu_.get(), invD_.get(), work, s);
+    if (ft_ == DType::F16) {
+      lu16_.alloc((size_t)std::max<int64_t>(nnz, 1) * 2);
+      invD16_.alloc((size_t)n * 2);
+      gph::launchCast(lu_.get(), work, lu16_.get(), DType::F16, nnz, s);
+      gph::launchCast(invD_.get(), work, invD16_.get(), DType::F16, n, s);
+    }
+    GPH_HIP(hipStreamSynchronize(s));
+  }
+
+  std::shared_ptr<ilu::Configuration> cfg_;
+  DType ft_;
+  DevBuf lu_, invD_, lu16_, invD16_;
+};
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel (gauss-seidel/Solver.cpp:45-160) with the codelet's use of b.
+// ---------------------------------------------------------------------------
+class GaussSeidelSolver : public Solver {
+ public:
+  GaussSeidelSolver(const DeviceMatrix& A,
+                    std::shared_ptr<gaussseidel::Configuration> c)
+      : Solver(A), cfg_(std::move(c)) {
+    if (cfg_->norm != VectorNorm::L2)
+      fail(kUnsupported, "only the L2 norm is supported by this build");
+  }
+  std::string name() const override { return cfg_->solverName(); }
+  bool usesInitialGuess() const override { return true; }
+  bool supportsDeviceApply() const override { return true; }
+
+  // One "iteration" = forward sweep (+ backward sweep when symmetric).
+  void sweeps(const Vec& x, const Vec& b, int count, DevState* st,
+              hipStream_t s) {
+    Vec tmp = ws_.get(0, x.type, A_.n());
+    auto tiles = A_.tileArgs();
+    const void* off = A_.crsOff(DType::F32);
+    const void* dg = A_.crsDiag(DType::F32);
+    const int chk = st ? 1 : 0;
+    for (int k = 0; k < count; k++) {
+      gph::launchGsSweep(tiles, off, dg, DType::F32, b.ptr, x.ptr, tmp.ptr,
+                         x.type, false, st, chk, s);
+      if (cfg_->symmetric) {
+        gph::launchGsSweep(tiles, off, dg, DType::F32, b.ptr, tmp.ptr, x.ptr,
+                           x.type, true, st, chk, s);
+      } else {
+        gph::launchCast(tmp.ptr, x.type, x.ptr, x.type, A_.n(), s);
+      }
+    }
+  }
+
+  void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,
+                   DevState* st, hipStream_t s) override {
+    Vec b = rhs;
+    if (pre.kind != gph::kPreNone) {
+      gph::VecArgs v;
+      v.n = A_.n();
+      v.a = pre.r;
+      v.c = pre.v;
+      v.o1 = pre.p;
+      v.state = st;
+      if (pre.kind == gph::kPreP)
+        gph::launchUpdateP(v, out.type, s);
+      else
+        gph::launchUpdateS(v, out.type, s);
+      b = Vec{pre.p, out.type};
+    }
+    gph::launchFill(out.ptr, out.type, A_.n(), 0.0, s);  // usesInitialGuess
+    int n = cfg_->numFixedIterations > 0 ? cfg_->numFixedIterations
+                                         : std::max(1, cfg_->numSweeps);
+    sweeps(out, b, n, st, s);
+  }
+
+  SolverStats solve(const Vec& x, const Vec& b) override {
+    double t0 = nowSeconds();
+    hipStream_t s = A_.stream();
+    SolverStats st;
+    st.solverName = name();
+    st.norm = cfg_->norm;
+    st.numTiles = A_.numTiles();
+    if (cfg_->numFixedIterations > 0) {
+      sweeps(x, b, cfg_->numFixedIterations, nullptr, s);
+      GPH_HIP(hipStreamSynchronize(s));
+      st.iterations = cfg_->numFixedIterations;
+      st.seconds = nowSeconds() - t0;
+      return st;
+    }
+    double bNorm = 0;
+    if (cfg_->relTolerance > 0) bNorm = A_.norm2(b, s);
+    Vec r = ws_.get(1, x.type, A_.n());
+    auto resNorm = [&]() {
+      A_.residual(x, b, r, x.type, DType::F32, s);
+      return A_.norm2(r, s);
+    };
+    st.initialResidual = resNorm();
+    st.finalResidual = st.initialResidual;
+    while (!((st.converged && st.iterations >= cfg_->minIterations) ||
+             st.iterations >= cfg_->maxIterations)) {
+      sweeps(x, b, std::max(1, cfg_->numSweeps), nullptr, s);
+      st.iterations += std::max(1, cfg_->numSweeps);
+      st.finalResidual = resNorm();
+      st.converged = st.finalResidual < cfg_->absTolerance ||
+                     st.finalResidual < st.initialResidual * cfg_->relResidual ||
+                     (cfg_->relTolerance > 0 &&
+                      st.finalResidual < bNorm * cfg_->relTolerance);
+      if (cfg_->printPerformanceEachIteration) st.print();
+    }
+    st.seconds = nowSeconds() - t0;
+    if (cfg_->printPerformanceAfterSolve) st.print();
+    return st;
+  }
+
+ private:
+  std::shared_ptr<gaussseidel::Configuration> cfg_;
+  Workspace ws_;
+};
+
+// ---------------------------------------------------------------------------
+// PBiCGStab (pbicgstab/Solver.cpp:48-194). Per iteration five kernels:
+//   K1 p-update fused into y = M^-1 p
+//   K2 v = A y with r0.v            -> alpha
+//   K3 s-update fused into z = M^-1 s
+//   K4 t = A z with t.t and t.s     -> omega
+//   K5 x, r update with r.r, r0.r   -> convergence, next rho
+// ---------------------------------------------------------------------------
+class PBiCGStabSolver : public Solver {
+ public:
+  PBiCGStabSolver(const DeviceMatrix& A,
+                  std::shared_ptr<pbicgstab::Configuration> c)
+      : Solver(A), cfg_(std::move(c)) {
+    if (cfg_->norm != VectorNorm::L2)
+      fail(kUnsupported, "only the L2 norm is supported by this build");
+    if (cfg_->preconditioner) {
+      pre_ = Solver::createSolver(A, cfg_->preconditioner);
+      if (!pre_->supportsDeviceApply())
+        fail(kUnsupported, pre_->name() + " cannot precondition PBiCGStab");
+    }
+    state_.alloc(sizeof(DevState));
+    GPH_HIP(hipMemset(state_.get(), 0, sizeof(DevState)));
+    partials_.alloc(A.partialsBytes());
+  }
+  std::string name() const override { return cfg_->solverName(); }
+  bool usesInitialGuess() const override { return true; }
+
+  SolverStats solve(const Vec& xIn, const Vec& bIn) override {
+    const double t0 = nowSeconds();
+    hipStream_t s = A_.stream();
+    const int64_t n = A_.n();
+    const DType X = cfg_->workingType.value_or(xIn.type);
+    if (X != DType::F32 && X != DType::F64)
+      fail(kUnsupported, "PBiCGStab working type must be float32 or float64");
+    V_ = cfg_->matrixValueType.value_or(DType::F32);
+
+    // bring x and b to the working type
+    Vec x = xIn, b = bIn;
+    if (x.type != X) {
+      x = ws_.get(8, X, n);
+      gph::launchCast(xIn.ptr, xIn.type, x.ptr, X, n, s);
+    }
+    if (b.type != X) {
+      b = ws_.get(9, X, n);
+      gph::launchCast(bIn.ptr, bIn.type, b.ptr, X, n, s);
+    }
+    r_ = ws_.get(0, X, n);
+    r0_ = ws_.get(1, X, n);
+    p_ = ws_.get(2, X, n);
+    v_ = ws_.get(3, X, n);
+    y_ = ws_.get(4, X, n);
+    s_ = ws_.get(5, X, n);
+    z_ = ws_.get(6, X, n);
+    t_ = ws_.get(7, X, n);
+    X_ = X;
+    x_ = x;
+
+    DevState init{};
+    init.absTol = cfg_->absTolerance;
+    init.relTol = cfg_->relTolerance;
+    init.relRes = cfg_->relResidual;
+    init.minIt = cfg_->minIterations;
+    init.maxIt = cfg_->maxIterations;
+    DevState* h = pinned_[0];
+    *h = init;
+    GPH_HIP(hipMemcpyAsync(state_.get(), h, sizeof(DevState),
+                           hipMemcpyHostToDevice, s));
+    DevState* st = state_.as<DevState>();
+
+    // ||x|| (reference quirk, pbicgstab/Solver.cpp:69-70) or ||b||
+    if (cfg_->relTolerance > 0) {
+      gph::VecArgs v;
+      v.n = (int32_t)n;
+      v.a = cfg_->relToleranceFromB ? b.ptr : x.ptr;
+      v.state = st;
+      v.partials = partials_.get();
+      v.fin = gph::kFinBNorm;
+      gph::launchNorm2(v, X, s);
+    }
+    // r = b - A x ; r0 = p = r ; ||r|| and rho (pbicgstab/Solver.cpp:66-83)
+    {
+      gph::SpmvArgs a = A_.spmvArgs(V_);
+      a.x = x.ptr;
+      a.b = b.ptr;
+      a.out = r_.ptr;
+      a.out2 = r0_.ptr;
+      a.out3 = p_.ptr;
+      a.state = st;
+      a.partials = partials_.get();
+      a.fin = gph::kFinInit;
+      gph::launchSpmv(a, V_, X, X, X, X, gph::kModeResidual, gph::kRedSelf, s);
+    }
+
+    int chunk = cfg_->iterationsPerSync > 0 ? cfg_->iterationsPerSync : 4;
+    if (cfg_->printPerformanceEachIteration) chunk = 1;
+    const bool pipelined = !cfg_->printPerformanceEachIteration;
+    // pipelined polling: keep one chunk in flight while checking the previous
+    hipEvent_t ev[2];
+    GPH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    GPH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int cur = 0;
+    int64_t launched = 0;
+    auto enqueueChunk = [&](int slot) {
+      for (int k = 0; k < chunk; k++) enqueueIteration(st, s);
+      launched += chunk;
+      GPH_HIP(hipMemcpyAsync(pinned_[slot], st, sizeof(DevState),
+                             hipMemcpyDeviceToHost, s));
+      GPH_HIP(hipEventRecord(ev[slot], s));
+    };
+    enqueueChunk(cur);
+    SolverStats stats;
+    stats.solverName = name();
+    stats.norm = cfg_->norm;
+    stats.numTiles = A_.numTiles();
+    for (;;) {
+      int next = cur ^ 1;
+      if (pipelined) enqueueChunk(next);
+      GPH_HIP(hipEventSynchronize(ev[cur]));
+      DevState hs = *pinned_[cur];
+      if (cfg_->printPerformanceEachIteration) {
+        stats.initialResidual = hs.r0n;
+        stats.finalResidual = hs.rF;
+        stats.iterations = hs.iter;
+        stats.converged = hs.converged;
+        stats.singular = hs.singular;
+        stats.seconds = nowSeconds() - t0;
+        stats.print();
+      }
+      if (hs.done) break;
+      if (pipelined)
+        cur = next;
+      else
+        enqueueChunk(cur);
+    }
+    GPH_HIP(hipStreamSynchronize(s));
+    GPH_HIP(hipMemcpy(pinned_[0], st, sizeof(DevState), hipMemcpyDeviceToHost));
+    DevState hs = *pinned_[0];
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (x.ptr != xIn.ptr) {
+      gph::launchCast(x.ptr, X, xIn.ptr, xIn.type, n, s);
+      GPH_HIP(hipStreamSynchronize(s));
+    }
+    stats.initialResidual = hs.r0n;
+    stats.finalResidual = hs.rF;
+    stats.iterations = hs.iter;
+    stats.converged = hs.converged != 0;
+    stats.singular = hs.singular != 0;
+    stats.seconds = nowSeconds() - t0;
+    lastLaunched_ = launched;
+    if (cfg_->printPerformanceAfterSolve) stats.print();
+    return stats;
+  }
+
+  int64_t lastLaunchedIterations() const { return lastLaunched_; }
+
+ private:
+  void enqueueIteration(DevState* st, hipStream_t s) {
+    const int32_t n = A_.n();
+    // K1: p = r + beta (p - omega v) ; y = M^-1 p
+    gph::PreOp pp;
+    pp.kind = gph::kPreP;
+    pp.r = r_.ptr;
+    pp.p = p_.ptr;
+    pp.v = v_.ptr;
+    if (pre_) {
+      pre_->applyDevice(p_, y_, pp, st, s);
+    } else {
+      gph::VecArgs v;
+      v.n = n;
+      v.a = r_.ptr;
+      v.c = v_.ptr;
+      v.o1 = p_.ptr;
+      v.o2 = y_.ptr;
+      v.state = st;
+      gph::launchUpdateP(v, X_, s);
+    }
+    // K2: v = A y ; alpha = rho / (r0 . v)
+    {
+      gph::SpmvArgs a = A_.spmvArgs(V_);
+      a.x = y_.ptr;
+      a.out = v_.ptr;
+      a.w = r0_.ptr;
+      a.state = st;
+      a.partials = partials_.get();
+      a.fin = gph::kFinAlpha;
+      a.checkDone = 1;
+      gph::launchSpmv(a, V_, X_, X_, X_, X_, gph::kModeSpmv, gph::kRedW, s);
+    }
+    // K3: s = r - alpha v ; z = M^-1 s
+    gph::PreOp ps;
+    ps.kind = gph::kPreS;
+    ps.r = r_.ptr;
+    ps.p = s_.ptr;
+    ps.v = v_.ptr;
+    if (pre_) {
+      pre_->applyDevice(s_, z_, ps, st, s);
+    } else {
+      gph::VecArgs v;
+      v.n = n;
+      v.a = r_.ptr;
+      v.c = v_.ptr;
+      v.o1 = s_.ptr;
+      v.o2 = z_.ptr;
+      v.state = st;
+      gph::launchUpdateS(v, X_, s);
+    }
+    // K4: t = A z ; omega = (t . s) / (t . t)
+    {
+      gph::SpmvArgs a = A_.spmvArgs(V_);
+      a.x = z_.ptr;
+      a.out = t_.ptr;
+      a.w = s_.ptr;
+      a.state = st;
+      a.partials = partials_.get();
+      a.fin = gph::kFinOmega;
+      a.checkDone = 1;
+      gph::launchSpmv(a, V_, X_, X_, X_, X_, gph::kModeSpmv, gph::kRedSelfW, s);
+    }
+    // K5: x += alpha y + omega z ; r = s - omega t ; ||r||, next rho
+    {
+      gph::VecArgs v;
+      v.n = n;
+      v.a = y_.ptr;
+      v.b = z_.ptr;
+      v.c = s_.ptr;
+      v.d = t_.ptr;
+      v.o1 = x_.ptr;
+      v.o2 = r_.ptr;
+      v.state = st;
+      v.partials = partials_.get();
+      gph::launchUpdateXR(v, r0_.ptr, X_, s);
+    }
+  }
+
+  std::shared_ptr<pbicgstab::Configuration> cfg_;
+  std::unique_ptr<Solver> pre_;
+  Workspace ws_;
+  DevBuf state_, partials_;
+  PinnedState pinned_;
+  DType X_ = DType::F32, V_ = DType::F32;
+  Vec r_, r0_, p_, v_, y_, s_, z_, t_, x_;
+  int64_t lastLaunched_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Mixed-precision iterative refinement (iterative-refinement/Solver.cpp:48-123).
+// The residual norm is the FP64 norm of the unrounded residual (the
+// reference takes the norm of the residual after rounding to the working
+// type, :80-81 and :110).
+// ---------------------------------------------------------------------------
+class IterativeRefinementSolver : public Solver {
+ public:
+  IterativeRefinementSolver(const DeviceMatrix& A,
+                            std::shared_ptr<iterativerefinement::Configuration> c)
+      : Solver(A), cfg_(std::move(c)) {
+    if (cfg_->norm != VectorNorm::L2)
+      fail(kUnsupported, "only the L2 norm is supported by this build");
+    inner_ = Solver::createSolver(A, cfg_->innerSolver);
+    state_.alloc(sizeof(DevState));
+    GPH_HIP(hipMemset(state_.get(), 0, sizeof(DevState)));
+    partials_.alloc(A.partialsBytes());
+  }
+  std::string name() const override { return cfg_->solverName(); }
+  bool usesInitialGuess() const override { return true; }
+
+  SolverStats solve(const Vec& x, const Vec& b) override {
+    const double t0 = nowSeconds();
+    hipStream_t s = A_.stream();
+    const int64_t n = A_.n();
+    const DType E = cfg_->extendedPrecisionType.value_or(x.type);
+    const DType W = cfg_->workingType.value_or(x.type);
+    const DType V = cfg_->matrixValueType.value_or(DType::F32);
+    Vec phi = x;
+    if (x.type != E) {
+      phi = ws_.get(0, E, n);
+      gph::launchCast(x.ptr, x.type, phi.ptr, E, n, s);
+    }
+    Vec r = ws_.get(1, W, n);
+    Vec d = ws_.get(2, W, n);
+    DevState* st = state_.as<DevState>();
+
+    SolverStats stats;
+    stats.solverName = name();
+    stats.norm = cfg_->norm;
+    stats.numTiles = A_.numTiles();
+    double bNorm = 0;
+    if (cfg_->relTolerance > 0) bNorm = A_.norm2(b, s);
+
+    // r = fl_W(b - A phi) accumulated in E, norm of the unrounded residual
+    auto residual = [&]() {
+      gph::SpmvArgs a = A_.spmvArgs(V);
+      a.x = phi.ptr;
+      a.b = b.ptr;
+      a.out = r.ptr;
+      a.state = st;
+      a.partials = partials_.get();
+      a.fin = gph::kFinNormAux;
+      gph::launchSpmv(a, V, E, b.type, W, E, gph::kModeResidual, gph::kRedSelf, s);
+      double out = 0;
+      GPH_HIP(hipMemcpyAsync(&out, &st->aux, sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+      GPH_HIP(hipStreamSynchronize(s));
+      return out;
+    };
+    stats.initialResidual = residual();
+    stats.finalResidual = stats.initialResidual;
+    while (!((stats.converged && stats.iterations >= cfg_->minIterations) ||
+             stats.iterations >= cfg_->maxIterations)) {
+      stats.iterations++;
+      if (inner_->usesInitialGuess()) gph::launchFill(d.ptr, W, n, 0.0, s);
+      SolverStats in = inner_->solve(d, r);
+      stats.innerIterations += in.iterations;
+      if (E == DType::F64) {
+        gph::launchAccumulate(static_cast<double*>(phi.ptr), d.ptr, W, n, s);
+      } else {
+        fail(kUnsupported, "IterativeRefinement needs float64 extended precision");
+      }
+      stats.finalResidual = residual();
+      stats.converged =
+          stats.finalResidual < (double)cfg_->absTolerance ||
+          stats.finalResidual < stats.initialResidual * (double)cfg_->relResidual ||
+          (cfg_->relTolerance > 0 &&
+           stats.finalResidual < bNorm * (double)cfg_->relTolerance);
+      if (cfg_->printPerformanceEachIteration) {
+        stats.seconds = nowSeconds() - t0;
+        stats.print();
+      }
+    }
+    if (phi.ptr != x.ptr) gph::launchCast(phi.ptr, E, x.ptr, x.type, n, s);
+    GPH_HIP(hipStreamSynchronize(s));
+    stats.seconds = nowSeconds() - t0;
+    if (cfg_->printPerformanceAfterSolve) stats.print();
+    return stats;
+  }
+
+ private:
+  std::shared_ptr<iterativerefinement::Configuration> cfg_;
+  std::unique_ptr<Solver> inner_;
+  Workspace ws_;
+  DevBuf state_, partials_;
+};
+
+// restarter/Solver.cpp:40-56
+class RestarterSolver : public Solver {
+ public:
+  RestarterSolver(const DeviceMatrix& A, std::shared_ptr<restarter::Configuration> c)
+      : Solver(A), cfg_(std::move(c)) {
+    inner_ = Solver::createSolver(A, cfg_->innerSolver);
+  }
+  std::string name() const override { return cfg_->solverName(); }
+  bool usesInitialGuess() const override { return inner_->usesInitialGuess(); }
+  SolverStats solve(const Vec& x, const Vec& b) override {
+    SolverStats st = inner_->solve(x, b);
+    int restarts = 0;
+    int64_t total = st.iterations;
+    while (restarts < cfg_->maxRestarts && !st.converged) {
+      restarts++;
+      st = inner_->solve(x, b);
+      total += st.iterations;
+    }
+    st.innerIterations = total;
+    return st;
+  }
+
+ private:
+  std::shared_ptr<restarter::Configuration> cfg_;
+  std::unique_ptr<Solver> inner_;
+};
+
+// Solver.cpp:32-56
+std::unique_ptr<Solver> Solver::createSolver(const DeviceMatrix& A,
+                                             std::shared_ptr<Configuration> c) {
+  if (auto g = std::dynamic_pointer_cast<gaussseidel::Configuration>(c))
+    return std::make_unique<GaussSeidelSolver>(A, g);
+  if (auto ir = std::dynamic_pointer_cast<iterativerefinement::Configuration>(c))
+    return std::make_unique<IterativeRefinementSolver>(A, ir);
+  if (auto il = std::dynamic_pointer_cast<ilu::Configuration>(c))
+    return std::make_unique<IluSolver>(A, il);
+  if (auto pb = std::dynamic_pointer_cast<pbicgstab::Configuration>(c))
+    return std::make_unique<PBiCGStabSolver>(A, pb);
+  if (auto rs = std::dynamic_pointer_cast<restarter::Configuration>(c))
+    return std::make_unique<RestarterSolver>(A, rs);
+  fail(kInvalid, "Unknown solver: " + (c ? c->solverName() : std::string("null")));
+}
+
+}  // namespace graphene::matrix::solver

@@ -1,0 +1,76 @@
+// Solver tree on one GPU: PBiCGStab, mixed-precision iterative refinement,
+// ILU(0), Gauss-Seidel, restarter.
+//
+// Interface of lg/matrix/solver/Solver.hpp:41-44 (solve(x, b) -> SolverStats,
+// name(), usesInitialGuess()) and factory Solver::createSolver
+// (lg/matrix/solver/Solver.cpp:32-56). The reference builds a device-side
+// program with cf::While loops; here the host enqueues fused kernels on a
+// HIP stream and the loop state (rho, alpha, omega, iteration, convergence)
+// stays in HBM, checked by the host every few iterations.
+#pragma once
+
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "device_matrix.hpp"
+
+namespace graphene::matrix::solver {
+
+// lg/matrix/solver/SolverStats.hpp:29-69
+struct SolverStats {
+  std::string solverName;
+  VectorNorm norm = VectorNorm::L2;
+  int32_t numTiles = 1;
+  double initialResidual = std::numeric_limits<double>::infinity();
+  double finalResidual = std::numeric_limits<double>::infinity();
+  int64_t iterations = 0;
+  bool converged = false;
+  bool singular = false;
+  int64_t innerIterations = 0;
+  double seconds = 0;
+  void print() const;  // SolverStats.cpp:65-117
+};
+
+class Solver {
+ public:
+  explicit Solver(const DeviceMatrix& A) : A_(A) {}
+  virtual ~Solver() = default;
+  virtual SolverStats solve(const Vec& x, const Vec& b) = 0;
+  virtual std::string name() const = 0;
+  virtual bool usesInitialGuess() const = 0;
+
+  // Preconditioner use inside a Krylov iteration: out = M^-1 rhs, where the
+  // right-hand side is formed on the fly by `pre` (p- or s-update of
+  // PBiCGStab). Kernels skip their work once st->done is set.
+  virtual bool supportsDeviceApply() const { return false; }
+  virtual void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,
+                           gph::DevState* st, hipStream_t s);
+
+  static std::unique_ptr<Solver> createSolver(
+      const DeviceMatrix& A, std::shared_ptr<Configuration> config);
+
+  const DeviceMatrix& matrix() const { return A_; }
+
+ protected:
+  const DeviceMatrix& A_;
+};
+
+// Workspace of device vectors keyed by slot.
+class Workspace {
+ public:
+  Vec get(int slot, gph::DType t, int64_t n);
+  size_t bytes() const;
+
+ private:
+  struct Entry {
+    gph::DevBuf buf;
+    gph::DType type = gph::DType::F32;
+    int64_t n = 0;
+  };
+  std::vector<Entry> e_;
+};
+
+}  // namespace graphene::matrix::solver

@@ -1,0 +1,221 @@
+/*
+ * graphene_amd: C-ABI of the MI355X sparse-solver hot path (CRS/LDU SpMV,
+ * PBiCGStab, ILU(0) and Gauss-Seidel preconditioning, mixed-precision
+ * iterative refinement).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the esa-tu-darmstadt/graphene checkout; lg/ = libgraphene/libgraphene/).
+ * The reference is a C++ API with no FFI of its own; these functions are what
+ * an FFI for that API binds: opaque handles, plain pointers and sizes, int
+ * status (0 = OK), gph_last_error() for the message. No exception crosses the
+ * boundary.
+ *
+ * Vectors are device pointers in the matrix's tile-major row order (the
+ * reference's distributed vector layout, DistributedTileLayout.cpp:132-154):
+ * use gph_host_matrix_to_local / gph_host_matrix_to_global to convert from
+ * and to the natural row order. Vectors "with halo" and "without halo" are
+ * the same vector here: halo cells are read from their owner's slot.
+ */
+#ifndef GRAPHENE_AMD_H
+#define GRAPHENE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GPH_OK = 0,
+  GPH_ERR_INVALID = 1,
+  GPH_ERR_RUNTIME = 2,
+  GPH_ERR_HIP = 3,
+  GPH_ERR_UNSUPPORTED = 4
+} gph_status;
+
+/* Element types, names of lg/common/Type.cpp:69-88. */
+typedef enum { GPH_FLOAT16 = 1, GPH_FLOAT32 = 2, GPH_FLOAT64 = 3 } gph_type;
+
+/* lg/matrix/MatrixFormat.hpp */
+typedef enum { GPH_CRS = 0, GPH_LDU = 1 } gph_format;
+
+typedef struct gph_host_matrix gph_host_matrix;
+typedef struct gph_matrix gph_matrix;
+typedef struct gph_solver gph_solver;
+
+/* lg/matrix/solver/SolverStats.hpp:29-69 (+ total inner iterations and the
+ * wall time of the solve). */
+typedef struct {
+  double initial_residual;
+  double final_residual;
+  int64_t iterations;
+  int64_t inner_iterations;
+  int32_t converged;
+  int32_t singular;
+  double seconds;
+  char name[96];
+} gph_solver_stats;
+
+typedef struct {
+  int64_t nrows;
+  int64_t nnz_off;
+  int64_t num_halo;
+  int32_t num_tiles;
+  int32_t max_colors;
+  int64_t num_color_entries;
+  int64_t num_upper_color_entries;
+  int32_t multicolor_recommended;
+  int32_t max_tile_rows;
+} gph_host_matrix_info;
+
+/* ---- errors --------------------------------------------------------------- */
+/* Message of the last failing call on this thread ("" if none). */
+const char* gph_last_error(void);
+int32_t gph_version(void);
+
+/* ---- triplet generators (COO) --------------------------------------------- *
+ * Query the count with rows == NULL, then call again with buffers of that
+ * size. Values are float32 (the reference generates Type::FLOAT32).         */
+/* generate3DPoissonTripletMatrix, lg/matrix/host/details/Poisson.cpp:26-69 */
+int gph_generate_poisson3d(int64_t nx, int64_t ny, int64_t nz, int64_t* count,
+                           int64_t* rows, int64_t* cols, float* vals);
+/* No reference counterpart: 27-point convection-diffusion (upwind, cell
+ * Peclet `peclet`, seeded direction). */
+int gph_generate_convdiff3d27(int64_t nx, int64_t ny, int64_t nz,
+                              double peclet, uint64_t seed, int64_t* count,
+                              int64_t* rows, int64_t* cols, float* vals);
+/* No reference counterpart: 2D 9-point convection-diffusion. */
+int gph_generate_convdiff2d9(int64_t nx, int64_t ny, double diffusion,
+                             double peclet, uint64_t seed, int64_t* count,
+                             int64_t* rows, int64_t* cols, float* vals);
+/* No reference counterpart: unstructured power-law matrix. */
+int gph_generate_powerlaw(int64_t n, int32_t min_len, int32_t max_len,
+                          double exponent, int32_t band, double local_fraction,
+                          uint64_t seed, int64_t* count, int64_t* rows,
+                          int64_t* cols, float* vals);
+/* Portable RNG: U[-1,1) floats from a seed. */
+int gph_uniform_pm1(uint64_t seed, int64_t n, float* out);
+
+/* ---- partitions (stand-ins for the METIS call, CRSHostMatrix.cpp:139-271) */
+int gph_partition_grid3d(int64_t nx, int64_t ny, int64_t nz, int64_t bx,
+                         int64_t by, int64_t bz, int32_t* row_to_tile,
+                         int32_t* num_tiles);
+
+/* ---- host matrices -------------------------------------------------------- */
+/* HostMatrix(CRS, triplets, numTiles), lg/matrix/host/HostMatrix.cpp:31-44 and
+ * CRSHostMatrix::CRSHostMatrix, CRSHostMatrix.cpp:63-85. row_to_tile may be
+ * NULL (one tile, or contiguous row ranges when num_tiles > 1). */
+int gph_host_matrix_from_coo(int64_t nrows, int64_t nnz, const int64_t* rows,
+                             const int64_t* cols, const float* vals,
+                             int32_t num_tiles, const int32_t* row_to_tile,
+                             gph_host_matrix** out);
+/* host::generate3DPoissonMatrix, lg/matrix/host/HostMatrix.hpp:92-96 */
+int gph_host_matrix_poisson3d(int64_t nx, int64_t ny, int64_t nz,
+                              int32_t num_tiles, const int32_t* row_to_tile,
+                              gph_host_matrix** out);
+/* No reference counterpart: 27-point convection-diffusion host matrix. */
+int gph_host_matrix_convdiff3d27(int64_t nx, int64_t ny, int64_t nz,
+                                 double peclet, uint64_t seed,
+                                 int32_t num_tiles, const int32_t* row_to_tile,
+                                 gph_host_matrix** out);
+/* No reference counterpart: power-law host matrix. */
+int gph_host_matrix_powerlaw(int64_t n, int32_t min_len, int32_t max_len,
+                             double exponent, int32_t band,
+                             double local_fraction, uint64_t seed,
+                             int32_t num_tiles, const int32_t* row_to_tile,
+                             gph_host_matrix** out);
+/* host::loadMatrixFromFile, lg/matrix/host/HostMatrix.hpp:88-91 */
+int gph_host_matrix_load_mtx(const char* path, int32_t num_tiles,
+                             const int32_t* row_to_tile,
+                             gph_host_matrix** out);
+void gph_host_matrix_destroy(gph_host_matrix* h);
+int gph_host_matrix_get_info(const gph_host_matrix* h,
+                             gph_host_matrix_info* info);
+/* Structure export (any pointer may be NULL). Sizes: tile_start T+1,
+ * new_to_old n, num_interior T, num_separator_regions T, row_ptr n+1,
+ * col/col_local/off nnz_off, diag n, halo_start T+1, halo_global num_halo. */
+int gph_host_matrix_export(const gph_host_matrix* h, int64_t* tile_start,
+                           int64_t* new_to_old, int32_t* num_interior,
+                           int32_t* num_separator_regions, int64_t* row_ptr,
+                           int32_t* col, int32_t* col_local, float* off,
+                           float* diag, int64_t* halo_start,
+                           int64_t* halo_global);
+/* Level sets (calculateRowColors / calculateColorAddressings,
+ * CRSHostMatrix.cpp:527-675): level n, color_sort_addr n, color_start
+ * num_color_entries, num_colors T. */
+int gph_host_matrix_export_levels(const gph_host_matrix* h, int32_t* level,
+                                  int32_t* color_sort_addr,
+                                  int64_t* color_start, int32_t* num_colors);
+/* LDU addressing (LDUAddressing.hpp:30-63), sizes: faces = nnz_off / 2.
+ * lower_addr/upper_addr/upper faces, owner_start n+1, neighbour_start n+1,
+ * neighbour_col_ind/lower faces. */
+int gph_host_matrix_export_ldu(const gph_host_matrix* h, int64_t* num_faces,
+                               int32_t* lower_addr, int32_t* upper_addr,
+                               int32_t* owner_start, int32_t* neighbour_start,
+                               int32_t* neighbour_col_ind, float* upper,
+                               float* lower);
+/* Permute a vector between natural row order and tile-major order
+ * (DistributedTileLayout::decomposeVector, DistributedTileLayout.cpp:157-183). */
+int gph_host_matrix_to_local(const gph_host_matrix* h, gph_type t,
+                             const void* global_in, void* local_out);
+int gph_host_matrix_to_global(const gph_host_matrix* h, gph_type t,
+                              const void* local_in, void* global_out);
+
+/* ---- device memory -------------------------------------------------------- */
+int gph_device_count(int32_t* count);
+int gph_malloc(size_t bytes, void** ptr);
+int gph_free(void* ptr);
+int gph_memcpy_htod(void* dst, const void* src, size_t bytes);
+int gph_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int gph_memset(void* dst, int value, size_t bytes);
+int gph_synchronize(void);
+
+/* ---- device matrices ------------------------------------------------------ */
+/* HostMatrix::copyToTile, lg/matrix/host/HostMatrix.hpp:53 (CRS) and the
+ * LDUMatrix constructor, lg/matrix/details/ldu/LDUMatrix.hpp:49-57 (LDU).
+ * rows_per_block: SpMV rows per workgroup (0 = automatic). */
+int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
+                      int32_t rows_per_block, gph_matrix** out);
+void gph_matrix_destroy(gph_matrix* A);
+int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes);
+
+/* y = A*x (Matrix::operator*, lg/matrix/Matrix.hpp:95-97; MatrixBase::spmv,
+ * MatrixBase.hpp:66-74). Matrix values used in `value_type`. */
+int gph_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
+             const void* x, gph_type y_type, void* y, gph_type acc_type);
+/* r = b - A*x (Matrix::residual, lg/matrix/Matrix.hpp:85-93). r64 (optional)
+ * receives the unrounded float64 residual when acc_type is float64. */
+int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
+                 const void* x, gph_type b_type, const void* b,
+                 gph_type r_type, void* r, gph_type acc_type, double* r64);
+/* ||x||_2 (Matrix::vectorNorm, lg/matrix/Matrix.hpp:115-117). */
+int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
+                    double* out);
+/* Mean device time (ms) of `iters` back-to-back SpMV launches, HIP events on
+ * the launch stream. */
+int gph_bench_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
+                   const void* x, void* y, int32_t iters, double* avg_ms);
+
+/* ---- solvers -------------------------------------------------------------- */
+/* solver::Configuration::fromJSON (lg/matrix/solver/Configuration.cpp:126-136)
+ * + Solver::createSolver (lg/matrix/solver/Solver.cpp:32-56). ILU factorizes
+ * here, as the reference does in its constructor. `json` is JSON with
+ * comments. */
+int gph_solver_create(const gph_matrix* A, const char* json, gph_solver** out);
+void gph_solver_destroy(gph_solver* S);
+/* Solver::solve(x, b) (lg/matrix/solver/Solver.hpp:41); Matrix::solve
+ * (lg/matrix/Matrix.cpp:26-30) is create + solve. */
+int gph_solve(gph_solver* S, gph_type x_type, void* x, gph_type b_type,
+              const void* b, gph_solver_stats* stats);
+/* Solver::name(), Solver::usesInitialGuess() */
+int gph_solver_name(const gph_solver* S, char* buf, size_t len);
+int gph_solver_uses_initial_guess(const gph_solver* S, int32_t* out);
+/* Validates a configuration without a matrix (CPU-only). */
+int gph_config_check(const char* json, char* name_buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHENE_AMD_H */
