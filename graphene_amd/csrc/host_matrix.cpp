@@ -452,6 +452,26 @@ static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
     std::vector<std::vector<int64_t>> interior((size_t)T);
     std::vector<std::vector<Region>> seps((size_t)T);
     std::vector<std::map<std::vector<int32_t>, int32_t>> lookup((size_t)T);
+    // Destination tiles of a row: tiles owning its columns
+    // (CRSHostMatrix.cpp:289-299) united with tiles whose rows reference it;
+    // identical for structurally symmetric patterns, and well defined for
+    // nonsymmetric ones (where the reference's local addressing throws).
+    std::vector<int64_t> refPtr((size_t)n + 1, 0);
+    for (int64_t r = 0; r < n; r++)
+      for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++)
+        if (m.rowToTile[(size_t)g.col[(size_t)a]] != m.rowToTile[(size_t)r])
+          refPtr[(size_t)g.col[(size_t)a] + 1]++;
+    for (int64_t r = 0; r < n; r++) refPtr[(size_t)r + 1] += refPtr[(size_t)r];
+    std::vector<int32_t> refTile((size_t)refPtr[(size_t)n]);
+    {
+      std::vector<int64_t> fill(refPtr.begin(), refPtr.end() - 1);
+      for (int64_t r = 0; r < n; r++)
+        for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
+          int64_t c = g.col[(size_t)a];
+          if (m.rowToTile[(size_t)c] != m.rowToTile[(size_t)r])
+            refTile[(size_t)fill[(size_t)c]++] = m.rowToTile[(size_t)r];
+        }
+    }
     std::vector<int32_t> key;
     for (int64_t r = 0; r < n; r++) {
       int32_t t = m.rowToTile[(size_t)r];
@@ -460,6 +480,8 @@ static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
         int32_t ct = m.rowToTile[(size_t)g.col[(size_t)a]];
         if (ct != t) key.push_back(ct);
       }
+      for (int64_t q = refPtr[(size_t)r]; q < refPtr[(size_t)r + 1]; q++)
+        key.push_back(refTile[(size_t)q]);
       if (key.empty()) {
         interior[(size_t)t].push_back(r);
         continue;
@@ -546,6 +568,7 @@ static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
           local = m.oldToNew[(size_t)gc] - t0;
         else
           local = haloLocal[(size_t)gc];
+        if (local < 0) fail(kRuntime, "internal: column missing from the halo");
         es.push_back(E{local, (int32_t)m.oldToNew[(size_t)gc], g.off[(size_t)a]});
       }
       if (T > 1)

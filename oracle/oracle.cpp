@@ -194,10 +194,20 @@ ORC_API int64_t orc_tile_layout(int64_t n, const int64_t* rowPtr, const int32_t*
   };
   std::vector<std::vector<int64_t>> interior(T);
   std::vector<std::vector<Region>> seps(T);
+  for (int64_t r = 0; r < n; r++)
+    if (rowToTile[r] < 0 || rowToTile[r] >= T) return -1;
+  // Destination tiles of a row: tiles owning its columns
+  // (CRSHostMatrix.cpp:289-299) united with tiles whose rows reference it.
+  // The two coincide for structurally symmetric patterns; the union keeps
+  // nonsymmetric patterns well defined (the reference throws there).
+  std::vector<std::set<int32_t>> refBy(n);
+  for (int64_t r = 0; r < n; r++)
+    for (int64_t a = rowPtr[r]; a < rowPtr[r + 1]; a++)
+      if (rowToTile[colInd[a]] != rowToTile[r])
+        refBy[colInd[a]].insert(rowToTile[r]);
   for (int64_t r = 0; r < n; r++) {
     int32_t t = rowToTile[r];
-    if (t < 0 || t >= T) return -1;
-    std::set<int32_t> nb;
+    std::set<int32_t> nb = refBy[r];
     for (int64_t a = rowPtr[r]; a < rowPtr[r + 1]; a++) {
       int32_t ct = rowToTile[colInd[a]];
       if (ct != t) nb.insert(ct);
@@ -271,10 +281,13 @@ ORC_API int64_t orc_tile_layout(int64_t n, const int64_t* rowPtr, const int32_t*
       for (int64_t a = rowPtr[g]; a < rowPtr[g + 1]; a++) {
         int64_t gc = colInd[a];
         int64_t local;
-        if (rowToTile[gc] == t)
+        if (rowToTile[gc] == t) {
           local = oldToNew[gc] - tileStart[t];
-        else
-          local = haloLocal.at(gc);
+        } else {
+          auto it = haloLocal.find(gc);
+          if (it == haloLocal.end()) return -1;
+          local = it->second;
+        }
         es.push_back(E{local, oldToNew[gc], offDiag[a]});
       }
       std::sort(es.begin(), es.end(),

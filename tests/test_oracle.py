@@ -1,0 +1,337 @@
+"""Pins of the CPU oracle (CPU only). Each check is independent of the
+oracle's own formulas: closed forms, scipy/numpy library routines, textbook
+dense algorithms on tiny inputs, and the reference's known-answer values."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as o
+from helpers import oracle_layout
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def to_scipy(L):
+    """Oracle tile-major layout -> scipy CSR incl. diagonal (float64)."""
+    n = len(L["diag"])
+    rp = L["row_ptr"]
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    A = sp.csr_matrix((L["off"].astype(np.float64), (rows, L["col"])), shape=(n, n))
+    return (A + sp.diags(L["diag"].astype(np.float64))).tocsr()
+
+
+def poisson_layout(nx, ny, nz, T=1, r2t=None):
+    n = nx * ny * nz
+    r, c, v = o.poisson7(nx, ny, nz)
+    return oracle_layout(n, r, c, v, T, r2t)
+
+
+# --- Poisson generator: closed form + Kronecker-sum construction ------------
+@pytest.mark.parametrize("dims", [(32, 32, 32), (5, 3, 4), (1, 1, 7)])
+def test_poisson_closed_form(dims):
+    nx, ny, nz = dims
+    r, c, v = o.poisson7(nx, ny, nz)
+    nnz = nx * ny * nz + 2 * ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1))
+    assert len(v) == nnz
+    if dims == (32, 32, 32):
+        assert nnz == 223232
+
+    def lap1(m):
+        return sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+
+    I = sp.identity
+    K = (sp.kron(I(nz), sp.kron(I(ny), lap1(nx))) + sp.kron(I(nz), sp.kron(lap1(ny), I(nx)))
+         + sp.kron(lap1(nz), sp.kron(I(ny), I(nx))))
+    A = sp.csr_matrix((v.astype(np.float64), (r, c)), shape=K.shape)
+    # Dirichlet boundary rows of the reference keep the full diagonal 6
+    Dfix = sp.diags(6.0 - K.diagonal())
+    assert abs(A - (K + Dfix)).max() == 0
+
+
+# --- COO -> CRS vs scipy (duplicates summed) ---------------------------------
+def test_coo_to_crs_vs_scipy():
+    rng = np.random.default_rng(0)
+    n, m = 50, 600
+    rows = rng.integers(0, n, m)
+    cols = rng.integers(0, n, m)
+    vals = rng.standard_normal(m).astype(np.float32)
+    rp, col, off, dg = o.coo_to_crs(n, rows, cols, vals)
+    S = sp.coo_matrix((vals.astype(np.float64), (rows, cols)), shape=(n, n)).tocsr()
+    S.sum_duplicates()
+    S.sort_indices()
+    assert np.allclose(dg, S.diagonal(), atol=1e-5)
+    offd = S - sp.diags(S.diagonal())
+    offd.eliminate_zeros()
+    # structure: every (row, col) off-diagonal pair, sorted by row then col
+    pairs = set(zip(np.repeat(np.arange(n), np.diff(rp)).tolist(), col.tolist()))
+    R, Cc = S.nonzero()
+    want = {(a, b) for a, b in zip(R.tolist(), Cc.tolist()) if a != b}
+    assert pairs == want
+    for i in range(n):
+        assert np.all(np.diff(col[rp[i]:rp[i + 1]]) > 0)
+    dense = np.zeros((n, n))
+    dense[np.repeat(np.arange(n), np.diff(rp)), col] = off
+    assert np.allclose(dense, offd.toarray(), atol=1e-5)
+
+
+# --- SpMV: FP64 scipy with a Higham bound; sequential float32 on tiny input --
+def test_spmv_f32_higham_bound():
+    L = poisson_layout(12, 10, 9)
+    A = to_scipy(L)
+    x = o.uniform_pm1(3, A.shape[0])
+    y = o.spmv(L, x).astype(np.float64)
+    exact = A @ x.astype(np.float64)
+    k = np.diff(L["row_ptr"]) + 1
+    gamma = k * 2.0 ** -24 / (1 - k * 2.0 ** -24)
+    bound = gamma * (abs(A) @ abs(x.astype(np.float64)))
+    assert np.all(np.abs(y - exact) <= bound + 1e-30)
+
+
+def test_spmv_f32_sequential_order_tiny():
+    rng = np.random.default_rng(1)
+    n = 20
+    rows, cols = np.nonzero(rng.random((n, n)) < 0.3)
+    vals = rng.standard_normal(len(rows)).astype(np.float32)
+    L = oracle_layout(n, rows.astype(np.int64), cols.astype(np.int64), vals)
+    x = rng.standard_normal(n).astype(np.float32)
+    y = o.spmv(L, x)
+    A = np.zeros((n, n), np.float32)
+    A[rows, cols] += vals
+    for i in range(n):  # textbook: diagonal term first, then ascending columns
+        s = np.float32(A[i, i]) * x[i]
+        for j in range(n):
+            if j != i and A[i, j] != 0:
+                s = np.float32(s + np.float32(A[i, j] * x[j]))
+        assert s == y[i]
+
+
+# --- tile layout invariants ---------------------------------------------------
+def test_tile_layout_invariants():
+    nx = 8
+    n = nx ** 3
+    rng = np.random.default_rng(2)
+    r2t = rng.integers(0, 5, n).astype(np.int32)
+    L = poisson_layout(nx, nx, nx, 5, r2t)
+    # every non-halo row exactly once, tiles contiguous and own their rows
+    assert sorted(L["new_to_old"]) == list(range(n))
+    ts = L["tile_start"]
+    for t in range(5):
+        assert np.all(r2t[L["new_to_old"][ts[t]:ts[t + 1]]] == t)
+    # halo of a tile == its foreign columns
+    A = to_scipy(poisson_layout(nx, nx, nx))
+    for t in range(5):
+        mine = np.nonzero(r2t == t)[0]
+        foreign = set(A[mine].indices.tolist()) - set(mine.tolist())
+        h = L["halo_global"][L["halo_start"][t]:L["halo_start"][t + 1]]
+        assert set(h.tolist()) == foreign and len(h) == len(foreign)
+    # same operator after the permutation
+    P = L["new_to_old"]
+    Ap = to_scipy(L)
+    assert abs(Ap - A[P][:, P]).max() == 0
+
+
+# --- level sets ---------------------------------------------------------------
+@pytest.mark.parametrize("m", [8, 16, 32])
+def test_levels_closed_form_poisson(m):
+    L = poisson_layout(m, m, m)
+    lv = o.levels(m ** 3, 1, L["tile_start"], L["row_ptr"], L["col"])
+    assert lv["num_colors"][0] == 3 * (m - 1) + 1   # 94 at 32^3
+
+
+def test_levels_closed_form_27pt():
+    import graphene_amd as g
+    m = 10
+    r, c, v = g.generateConvectionDiffusion27Triplets(m, m, m)
+    L = oracle_layout(m ** 3, r, c, v)
+    lv = o.levels(m ** 3, 1, L["tile_start"], L["row_ptr"], L["col"])
+    assert lv["num_colors"][0] == 7 * (m - 1) + 1
+
+
+def test_levels_valid_layering():
+    nx = 8
+    rng = np.random.default_rng(3)
+    r2t = rng.integers(0, 3, nx ** 3).astype(np.int32)
+    L = poisson_layout(nx, nx, nx, 3, r2t)
+    lv = o.levels(nx ** 3, 3, L["tile_start"], L["row_ptr"], L["col"])
+    ts, rp, col, level = L["tile_start"], L["row_ptr"], L["col"], lv["level"]
+    for t in range(3):
+        for p in range(ts[t], ts[t + 1]):
+            lower = [j for j in col[rp[p]:rp[p + 1]] if ts[t] <= j < p]
+            if lower:
+                assert level[p] == 1 + max(level[j] for j in lower)
+            else:
+                assert level[p] == 0
+
+
+# --- LDU == CRS ---------------------------------------------------------------
+def test_ldu_equals_crs():
+    L = poisson_layout(6, 5, 4)
+    U = o.ldu_build(L)
+    x = np.arange(L["diag"].size, dtype=np.float32) % 7 - 3  # small integers: exact
+    assert np.array_equal(o.ldu_spmv_f32(U, x), o.spmv(L, x))
+    assert U["num_faces"] * 2 == len(L["off"])
+
+
+# --- ILU(0) -------------------------------------------------------------------
+def test_ilu0_pattern_identity():
+    import graphene_amd as g
+    m = 6
+    r, c, v = g.generateConvectionDiffusion27Triplets(m, m, m, 10.0, 3)
+    L = oracle_layout(m ** 3, r, c, v)
+    lu, invD = o.ilu0_factor(L, np.float64)
+    n = m ** 3
+    A = to_scipy(L).toarray()
+    Lm = np.eye(n)
+    Um = np.diag(1.0 / invD)
+    rows = np.repeat(np.arange(n), np.diff(L["row_ptr"]))
+    for i, j, val in zip(rows, L["col"], lu):
+        if j < i:
+            Lm[i, j] = val
+        else:
+            Um[i, j] = val
+    LU = Lm @ Um
+    mask = A != 0
+    assert np.allclose(LU[mask], A[mask], rtol=1e-12, atol=1e-12)
+
+
+def test_ilu0_dense_brute_force():
+    rng = np.random.default_rng(4)
+    n = 12
+    M = (rng.random((n, n)) < 0.35) * rng.standard_normal((n, n))
+    np.fill_diagonal(M, 5 + rng.random(n))
+    pattern = M != 0
+    # textbook IKJ ILU(0) restricted to the pattern (Saad Alg. 10.4)
+    W = M.copy()
+    for i in range(1, n):
+        for k in range(i):
+            if not pattern[i, k]:
+                continue
+            W[i, k] = W[i, k] / W[k, k]
+            for j in range(k + 1, n):
+                if pattern[i, j]:
+                    W[i, j] = W[i, j] - W[i, k] * W[k, j]
+    rows, cols = np.nonzero(pattern)
+    L = oracle_layout(n, rows.astype(np.int64), cols.astype(np.int64),
+                      M[rows, cols].astype(np.float32))
+    lu, invD = o.ilu0_factor(L, np.float64)
+    Mf = M.astype(np.float32).astype(np.float64)
+    W = Mf.copy()
+    for i in range(1, n):
+        for k in range(i):
+            if not pattern[i, k]:
+                continue
+            W[i, k] = W[i, k] / W[k, k]
+            for j in range(k + 1, n):
+                if pattern[i, j]:
+                    W[i, j] = W[i, j] - W[i, k] * W[k, j]
+    rr = np.repeat(np.arange(n), np.diff(L["row_ptr"]))
+    assert np.allclose(lu, W[rr, L["col"]], rtol=1e-13)
+    assert np.allclose(1.0 / invD, np.diag(W), rtol=1e-13)
+
+
+def test_ilu0_apply_vs_triangular_solves():
+    L = poisson_layout(7, 6, 5)
+    lu, invD = o.ilu0_factor(L, np.float64)
+    n = len(invD)
+    rows = np.repeat(np.arange(n), np.diff(L["row_ptr"]))
+    lo = L["col"] < rows
+    Lm = sp.csr_matrix((lu[lo], (rows[lo], L["col"][lo])), shape=(n, n)) + sp.identity(n)
+    Um = sp.csr_matrix((lu[~lo], (rows[~lo], L["col"][~lo])), shape=(n, n)) + sp.diags(1 / invD)
+    b = o.uniform_pm1(5, n).astype(np.float64)
+    y = spla.spsolve_triangular(Lm.tocsr(), b, lower=True)
+    x = spla.spsolve_triangular(Um.tocsr(), y, lower=False)
+    assert np.allclose(o.ilu0_apply(L, lu, invD, b), x, rtol=1e-12, atol=1e-13)
+
+
+# --- Gauss-Seidel ---------------------------------------------------------------
+def test_gs_dense_textbook_and_fixed_point():
+    L = poisson_layout(5, 4, 3)
+    A = to_scipy(L).toarray()
+    n = A.shape[0]
+    b = o.uniform_pm1(6, n).astype(np.float64)
+    x0 = o.uniform_pm1(7, n).astype(np.float64)
+    # forward sweep, then backward sweep
+    x = x0.copy()
+    for i in range(n):
+        x[i] = (b[i] - A[i, :i] @ x[:i] - A[i, i + 1:] @ x[i + 1:]) / A[i, i]
+    assert np.allclose(o.gs_sweep(L, b, x0), x, rtol=1e-13, atol=1e-14)
+    xb = x.copy()
+    for i in reversed(range(n)):
+        xb[i] = (b[i] - A[i, :i] @ xb[:i] - A[i, i + 1:] @ xb[i + 1:]) / A[i, i]
+    assert np.allclose(o.gs_sweep(L, b, x, backward=True), xb, rtol=1e-13, atol=1e-14)
+    y = x0.copy()
+    for _ in range(400):
+        y = o.gs_sweep(L, b, y)
+    assert np.allclose(A @ y, b, atol=1e-10)
+
+
+# --- reductions: reference known answers + the worker chunking ----------------
+def test_reduction_kats():
+    k = json.load(open(os.path.join(HERE, "golden", "tensor_kats.json")))
+    v = np.arange(16, dtype=np.float32).reshape(8, 2)
+    ts = np.array([0, 2, 4, 6, 8])
+    got = [o.reduce_add(np.ascontiguousarray(v[:, c]), ts) for c in range(2)]
+    assert got == k["reduce_dim0_4tiles"]["expected"]
+    w = np.arange(64, dtype=np.float32).reshape(8, 8)
+    got = [o.reduce_add(w[r], np.array([0, 8])) for r in range(8)]
+    assert got == k["reduce_dim1"]["expected"]
+    a = np.array(k["dot"]["a"], np.float32)
+    b = np.array(k["dot"]["b"], np.float32)
+    got = [o.reduce_add(a[i] * b[i], np.array([0, 3])) for i in range(2)]
+    assert got == k["dot"]["expected"]
+    q = np.array(k["square_norm_rows"]["v"], np.float32)
+    got = [o.reduce_add(q[i], np.array([0, 3]), square=True) for i in range(4)]
+    assert got == k["square_norm_rows"]["expected"]
+
+
+def test_reduction_worker_chunks():
+    # 48 elements on one tile -> 6 workers of 8 (Expression.cpp:472-485).
+    # v = [2^24, 1 x 47]: worker 0 loses its 7 ones (2^24 + 1 rounds to 2^24),
+    # workers 1..5 hold 8 each -> 2^24 + 40. A single sequential sum gives 2^24.
+    v = np.ones(48, np.float32)
+    v[0] = 2.0 ** 24
+    assert o.reduce_add(v, np.array([0, 48])) == 2.0 ** 24 + 40
+    # 6 or fewer elements: no worker split
+    assert o.reduce_add(np.array([2.0 ** 24, 1, 1, 1], np.float32), np.array([0, 4])) == 2.0 ** 24
+
+
+# --- Krylov solvers -------------------------------------------------------------
+def test_bicgstab_matches_scipy():
+    L = poisson_layout(10, 10, 10)
+    A = to_scipy(L)
+    b = o.uniform_pm1(8, A.shape[0]).astype(np.float64)
+    S = o.System(L)
+    x, st = S.pbicgstab(np.zeros_like(b), b, S.pbicgstab_cfg(1e-8, 500, "none",
+                                                             working64=True))
+    its = []
+    xs, info = spla.bicgstab(A, b, rtol=1e-8, atol=0.0, maxiter=500,
+                             callback=lambda xk: its.append(1))
+    assert st.converged and info == 0
+    assert abs(st.iterations - len(its)) <= 1
+    assert np.allclose(x, xs, rtol=1e-6, atol=1e-8)
+
+
+def test_pbicgstab_ilu_true_residual_and_mpir():
+    L = poisson_layout(16, 16, 16)
+    A = to_scipy(L)
+    xt = o.uniform_pm1(9, A.shape[0])
+    b = (A @ xt.astype(np.float64)).astype(np.float32)
+    S = o.System(L)
+    x, st = S.pbicgstab(np.zeros_like(b), b, S.pbicgstab_cfg(1e-6, 500))
+    assert st.converged and not st.singular
+    assert np.linalg.norm(b - A @ x.astype(np.float64)) <= 2e-6 * np.linalg.norm(b)
+    phi, st, inner = S.mpir(b, 1e-10, 30)
+    assert st.converged
+    assert np.linalg.norm(b - A @ phi) <= 1.0001e-10 * np.linalg.norm(b)
+
+
+def test_singular_rhs_sets_flag():
+    L = poisson_layout(4, 4, 4)
+    S = o.System(L)
+    b = np.zeros(64, np.float32)
+    x, st = S.pbicgstab(np.zeros_like(b), b, S.pbicgstab_cfg(1e-6, 50))
+    assert st.singular and st.iterations == 0
