@@ -21,9 +21,11 @@ SOURCES = [
     "device_matrix.cpp",
     "solvers.cpp",
     "capi.cpp",
+    "ilu_pack.cpp",
     "k_spmv.hip",
     "k_vec.hip",
     "k_precond.hip",
+    "k_ilu_packed.hip",
 ]
 
 COMMON = [

@@ -1,0 +1,45 @@
+// Level-major packed layout of the ILU(0) factors for the gfx950
+// substitution kernel.
+//
+// For every tile and direction (forward: strictly lower in-tile entries in
+// ascending column order; backward: strictly upper in-tile entries in
+// descending column order, as ilu/Solver.cpp:380-442 visits them), the
+// entries are grouped by level and stored level after level; every level
+// starts on a 64-entry "slab" boundary so one wave loads a slab with one
+// coalesced instruction. A row without entries in a direction gets one
+// zero entry that reads a zero slot, so every row has at least one entry
+// and the substitution keeps the reference's arithmetic exactly.
+// Per row: (local id, entry count, first entry within its level).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "host_matrix.hpp"
+
+namespace graphene::matrix {
+
+constexpr int kPackSlab = 64;   // entries per slab (one per lane)
+constexpr int kPackDepth = 16;  // slabs in flight per wave (prefetch ring)
+
+struct PackedDirection {
+  std::vector<uint16_t> col;      // local column id per slot (zero slot = n_t)
+  std::vector<int32_t> src;       // CRS position of the entry, -1 = zero
+  std::vector<int32_t> slabOff;   // T+1: first slab of each tile's stream
+  std::vector<int32_t> lvlOff;    // T+1: offset of each tile's level table
+  std::vector<int32_t> slabEnd;   // per tile: numLevels (cumulative) + sentinel
+  std::vector<int32_t> rowStart;  // per tile: numLevels+1 entries
+  std::vector<uint32_t> rowInfo;  // 2 words per row: local | len<<16, start
+  int32_t maxLevels = 0;
+  int32_t maxLevelEntries = 0;
+};
+
+struct PackedILU {
+  PackedDirection fwd, bwd;
+  int32_t maxRows = 0;
+  bool ok = false;  // false: a tile exceeds the 16-bit local ids
+};
+
+PackedILU packILU(const host::HostMatrix& m);
+
+}  // namespace graphene::matrix

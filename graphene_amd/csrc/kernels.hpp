@@ -123,4 +123,30 @@ void launchGsSweep(const TileArgs& t, const void* off, const void* diag,
 
 size_t tileSharedBytes(const TileArgs& t, DType xt);
 
+// Level-major packed ILU(0) factors (one direction).
+struct PackedDirArgs {
+  const uint16_t* col = nullptr;     // local column per slot
+  const void* val = nullptr;         // factor value per slot
+  const int32_t* slabOff = nullptr;  // T+1
+  const int32_t* lvlOff = nullptr;   // T+1
+  const int32_t* slabEnd = nullptr;  // per tile nL+1 (last = sentinel)
+  const int32_t* rowStart = nullptr; // per tile nL+1
+  const uint32_t* rowInfo = nullptr; // 2 words per row, tile-major
+};
+struct PackedArgs {
+  int32_t T = 0;
+  const int32_t* tileStart = nullptr;
+  int32_t maxRows = 0, maxLevels = 0, maxLevelEntries = 0;
+  PackedDirArgs fwd, bwd;
+};
+constexpr int kPackedThreads = 64;  // one wave per tile
+size_t packedSharedBytes(const PackedArgs& p, DType xt);
+// out = (LU)^-1 rhs with the packed factors; one wave per tile.
+void launchIluPacked(const PackedArgs& p, const void* invD, DType ft,
+                     const void* rhs, void* out, DType xt, const PreOp& pre,
+                     DevState* st, int checkDone, hipStream_t s);
+// dst[k] = idx[k] < 0 ? 0 : src[idx[k]]  (same element type t)
+void launchGather(const void* src, DType t, const int32_t* idx, void* dst,
+                  int64_t n, hipStream_t s);
+
 }  // namespace gph

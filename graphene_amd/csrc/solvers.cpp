@@ -1,6 +1,8 @@
 // Solver tree implementation (see solvers.hpp).
 #include "solvers.hpp"
 
+#include "ilu_pack.hpp"
+
 #include <chrono>
 #include <cstdio>
 #include <cmath>
@@ -106,12 +108,18 @@ class IluSolver : public Solver {
       lu = lu16_.get();
       inv = invD16_.get();
     }
+    if (packed_ && gph::packedSharedBytes(pargs_, out.type) <= kMaxPackedShared) {
+      gph::launchIluPacked(pargs_, inv, ft, rhs.ptr, out.ptr, out.type, pre, st,
+                           st ? 1 : 0, s);
+      return;
+    }
     gph::launchIluApply(A_.tileArgs(), lu, inv, ft, rhs.ptr, out.ptr, out.type,
                         pre, st, st ? 1 : 0, s);
   }
 
   size_t factorBytes() const {
-    return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes();
+    return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes() +
+           fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes();
   }
 
  private:
@@ -130,12 +138,56 @@ class IluSolver : public Solver {
       gph::launchCast(lu_.get(), work, lu16_.get(), DType::F16, nnz, s);
       gph::launchCast(invD_.get(), work, invD16_.get(), DType::F16, n, s);
     }
+    buildPacked(s);
     GPH_HIP(hipStreamSynchronize(s));
   }
 
+  // Level-major packed copy of the factors for the one-wave-per-tile kernel.
+  void buildPacked(hipStream_t s) {
+    PackedILU P = packILU(A_.hostMatrix());
+    if (!P.ok) return;
+    const void* src = ft_ == DType::F16 ? lu16_.get() : lu_.get();
+    const size_t es = gph::dtypeSize(ft_);
+    auto dir = [&](const PackedDirection& d, DevBuf& col, DevBuf& val, DevBuf& so,
+                   DevBuf& lo, DevBuf& se, DevBuf& rs, DevBuf& ri,
+                   gph::PackedDirArgs& a) {
+      col.upload(d.col.data(), d.col.size(), s);
+      DevBuf idx;
+      idx.upload(d.src.data(), d.src.size(), s);
+      val.alloc(d.src.size() * es);
+      gph::launchGather(src, ft_, idx.as<int32_t>(), val.get(), (int64_t)d.src.size(), s);
+      GPH_HIP(hipStreamSynchronize(s));
+      so.upload(d.slabOff.data(), d.slabOff.size(), s);
+      lo.upload(d.lvlOff.data(), d.lvlOff.size(), s);
+      se.upload(d.slabEnd.data(), d.slabEnd.size(), s);
+      rs.upload(d.rowStart.data(), d.rowStart.size(), s);
+      ri.upload(d.rowInfo.data(), d.rowInfo.size(), s);
+      a.col = col.as<uint16_t>();
+      a.val = val.get();
+      a.slabOff = so.as<int32_t>();
+      a.lvlOff = lo.as<int32_t>();
+      a.slabEnd = se.as<int32_t>();
+      a.rowStart = rs.as<int32_t>();
+      a.rowInfo = ri.as<uint32_t>();
+    };
+    dir(P.fwd, fcol_, fval_, fso_, flo_, fse_, frs_, fri_, pargs_.fwd);
+    dir(P.bwd, bcol_, bval_, bso_, blo_, bse_, brs_, bri_, pargs_.bwd);
+    pargs_.T = A_.numTiles();
+    pargs_.tileStart = A_.tileArgs().tileStart;
+    pargs_.maxRows = P.maxRows;
+    pargs_.maxLevels = std::max(P.fwd.maxLevels, P.bwd.maxLevels);
+    pargs_.maxLevelEntries = std::max(P.fwd.maxLevelEntries, P.bwd.maxLevelEntries);
+    packed_ = true;
+  }
+
+  static constexpr size_t kMaxPackedShared = 156 * 1024;
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_, invD16_;
+  bool packed_ = false;
+  gph::PackedArgs pargs_;
+  DevBuf fcol_, fval_, fso_, flo_, fse_, frs_, fri_;
+  DevBuf bcol_, bval_, bso_, blo_, bse_, brs_, bri_;
 };
 
 // ---------------------------------------------------------------------------
