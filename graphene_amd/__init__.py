@@ -86,6 +86,7 @@ _SIGS = {
     "gph_synchronize": ([], C.c_int),
     "gph_matrix_create": ([_P, C.c_int, C.c_int32, C.POINTER(_P)], C.c_int),
     "gph_matrix_destroy": ([_P], None),
+    "gph_matrix_set_sliced": ([_P, C.c_int32], C.c_int),
     "gph_matrix_device_bytes": ([_P, C.POINTER(C.c_size_t)], C.c_int),
     "gph_spmv": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int], C.c_int),
     "gph_residual": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P], C.c_int),
@@ -430,6 +431,10 @@ class Matrix:
 
     def numTiles(self):
         return self.host.num_tiles
+
+    def set_sliced(self, sliced=True):
+        """Row-gather kernel form: sliced ELL (default) or LDS row blocks."""
+        _check(lib().gph_matrix_set_sliced(self._A, int(bool(sliced))))
 
     def getFormat(self):
         return self.fmt

@@ -394,6 +394,12 @@ int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
   });
 }
 void gph_matrix_destroy(gph_matrix* A) { delete A; }
+int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced) {
+  return guard([&] {
+    need(A, "matrix");
+    A->A->setUseSell(sliced != 0);
+  });
+}
 int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes) {
   return guard([&] {
     need(A, "matrix");

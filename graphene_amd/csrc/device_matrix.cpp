@@ -43,6 +43,7 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
   std::vector<int64_t> rowLen((size_t)n_);
   for (int32_t r = 0; r < n_; r++) rowLen[(size_t)r] = h.rowPtr[(size_t)r + 1] - h.rowPtr[(size_t)r];
   buildRowBlocks(rowLen, rowsPerBlock, blockRowCrs_, numBlocksCrs_);
+  buildSell(h.rowPtr, h.col, h.offDiag, sellCrs_);
 
   if (fmt_ == MatrixFormat::LDU) {
     host::LDUHost L = host::buildLDU(h);
@@ -61,6 +62,24 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
       len[(size_t)r] = (int64_t)(L.ownerStart[(size_t)r + 1] - L.ownerStart[(size_t)r]) +
                        (int64_t)(L.neighbourStart[(size_t)r + 1] - L.neighbourStart[(size_t)r]);
     buildRowBlocks(len, rowsPerBlock, blockRowLdu_, numBlocksLdu_);
+    // per row: owned faces (upper, ascending), then losort faces (lower)
+    std::vector<int64_t> rp((size_t)n_ + 1, 0);
+    std::vector<int32_t> cc;
+    std::vector<float> vv;
+    cc.reserve((size_t)2 * L.numFaces);
+    vv.reserve((size_t)2 * L.numFaces);
+    for (int32_t r = 0; r < n_; r++) {
+      for (int32_t f = L.ownerStart[(size_t)r]; f < L.ownerStart[(size_t)r + 1]; f++) {
+        cc.push_back(L.upperAddr[(size_t)f]);
+        vv.push_back(L.upper[(size_t)f]);
+      }
+      for (int32_t q = L.neighbourStart[(size_t)r]; q < L.neighbourStart[(size_t)r + 1]; q++) {
+        cc.push_back(lowerCol[(size_t)q]);
+        vv.push_back(L.lower[(size_t)q]);
+      }
+      rp[(size_t)r + 1] = (int64_t)cc.size();
+    }
+    buildSell(rp, cc, vv, sellLdu_);
   }
 
   // tile structure and level sets
@@ -77,7 +96,7 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
 
   scratchState_.alloc(sizeof(gph::DevState));
   GPH_HIP(hipMemsetAsync(scratchState_.get(), 0, sizeof(gph::DevState), s));
-  int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096});
+  int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096, gph::kSellGrid});
   partialsBytes_ = (size_t)maxBlocks * 2 * sizeof(double);
   scratchPartials_.alloc(partialsBytes_);
   GPH_HIP(hipStreamSynchronize(s));
@@ -118,6 +137,84 @@ void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
   out.upload(br.data(), br.size(), stream_);
 }
 
+// Sliced ELL: 64 rows per slice, slot (k, lane) = sliceOff[s] + 64k + lane.
+// Rows keep their stored entry order. When row lengths vary strongly (max >
+// 2x mean), rows are sorted by length inside windows of 4096 rows so that a
+// slice holds rows of similar length (perm gives each lane's row).
+void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
+                             const std::vector<int32_t>& cols,
+                             const std::vector<float>& vals, Sell& out) {
+  const int64_t n = n_;
+  int64_t maxLen = 0;
+  for (int64_t r = 0; r < n; r++)
+    maxLen = std::max(maxLen, rowPtr[(size_t)r + 1] - rowPtr[(size_t)r]);
+  if (maxLen > 65535) return;  // row-block kernel only
+  const double mean = n > 0 ? (double)rowPtr[(size_t)n] / (double)n : 0.0;
+  const bool sorted = maxLen > 2.0 * mean + 1.0;
+  const int64_t numSlices = (n + 63) / 64;
+  std::vector<int32_t> perm((size_t)numSlices * 64, (int32_t)n);
+  for (int64_t r = 0; r < n; r++) perm[(size_t)r] = (int32_t)r;
+  if (sorted) {
+    const int64_t W = 4096;
+    for (int64_t w0 = 0; w0 < n; w0 += W) {
+      const int64_t w1 = std::min(n, w0 + W);
+      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int32_t a, int32_t b) {
+        return rowPtr[(size_t)a + 1] - rowPtr[(size_t)a] > rowPtr[(size_t)b + 1] - rowPtr[(size_t)b];
+      });
+    }
+  }
+  std::vector<int32_t> sliceOff((size_t)numSlices + 1, 0);
+  std::vector<uint16_t> len((size_t)numSlices * 64, 0);
+  int64_t slots = 0;
+  for (int64_t sl = 0; sl < numSlices; sl++) {
+    int64_t K = 0;
+    for (int l = 0; l < 64; l++) {
+      int32_t r = perm[(size_t)(sl * 64 + l)];
+      if (r < n) {
+        int64_t ln = rowPtr[(size_t)r + 1] - rowPtr[(size_t)r];
+        len[(size_t)(sl * 64 + l)] = (uint16_t)ln;
+        K = std::max(K, ln);
+      }
+    }
+    sliceOff[(size_t)sl] = (int32_t)slots;
+    slots += 64 * K;
+    if (slots > INT32_MAX) return;
+  }
+  sliceOff[(size_t)numSlices] = (int32_t)slots;
+  std::vector<int32_t> c((size_t)std::max<int64_t>(slots, 1), 0);
+  std::vector<float> v((size_t)std::max<int64_t>(slots, 1), 0.0f);
+  for (int64_t sl = 0; sl < numSlices; sl++)
+    for (int l = 0; l < 64; l++) {
+      int32_t r = perm[(size_t)(sl * 64 + l)];
+      if (r >= n) continue;
+      for (int64_t k = 0; k < rowPtr[(size_t)r + 1] - rowPtr[(size_t)r]; k++) {
+        size_t slot = (size_t)(sliceOff[(size_t)sl] + 64 * k + l);
+        c[slot] = cols[(size_t)(rowPtr[(size_t)r] + k)];
+        v[slot] = vals[(size_t)(rowPtr[(size_t)r] + k)];
+      }
+    }
+  hipStream_t s = stream_;
+  out.sliceOff.upload(sliceOff.data(), sliceOff.size(), s);
+  out.rowLen.upload(len.data(), len.size(), s);
+  if (sorted) out.perm.upload(perm.data(), perm.size(), s);
+  out.col.upload(c.data(), c.size(), s);
+  out.val32.upload(v.data(), v.size(), s);
+  out.slots = slots;
+  out.numSlices = (int32_t)numSlices;
+}
+
+const void* DeviceMatrix::sellVal(const Sell& s, DType vt) const {
+  switch (vt) {
+    case DType::F32:
+      return s.val32.get();
+    case DType::F64:
+      return ensureCopy(s.val32, s.val64, vt, s.slots);
+    case DType::F16:
+      return ensureCopy(s.val32, s.val16, vt, s.slots);
+  }
+  return nullptr;
+}
+
 const void* DeviceMatrix::ensureCopy(const DevBuf& src32, DevBuf& dst,
                                      DType vt, int64_t count) const {
   std::lock_guard<std::mutex> lk(copyMutex_);
@@ -156,6 +253,18 @@ gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
   gph::SpmvArgs a;
   a.n = n_;
   a.diag = crsDiag(vt);
+  if (useSell_) {
+    const Sell& sl = fmt_ == MatrixFormat::CRS ? sellCrs_ : sellLdu_;
+    if (sl.numSlices > 0) {
+      a.numSlices = sl.numSlices;
+      a.sliceOff = sl.sliceOff.as<int32_t>();
+      a.rowLen = sl.rowLen.as<uint16_t>();
+      a.perm = sl.perm ? sl.perm.as<int32_t>() : nullptr;
+      a.scol = sl.col.as<int32_t>();
+      a.sval = sellVal(sl, vt);
+      return a;
+    }
+  }
   if (fmt_ == MatrixFormat::CRS) {
     a.numBlocks = numBlocksCrs_;
     a.blockRow = blockRowCrs_.as<int32_t>();
@@ -251,6 +360,11 @@ size_t DeviceMatrix::deviceBytes() const {
         &lvlPtrOff_, &ulvlRows_, &ulvlPtr_, &ulvlPtrOff_, &scratchState_,
         &scratchPartials_})
     b += p->bytes();
+  for (const Sell* sl : {&sellCrs_, &sellLdu_})
+    for (const DevBuf* p : std::initializer_list<const DevBuf*>{
+             &sl->sliceOff, &sl->rowLen, &sl->perm, &sl->col, &sl->val32,
+             &sl->val64, &sl->val16})
+      b += p->bytes();
   return b;
 }
 

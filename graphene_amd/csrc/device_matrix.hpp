@@ -63,6 +63,10 @@ class DeviceMatrix {
 
   size_t deviceBytes() const;
 
+  // Kernel form of the row gather: sliced ELL (default) or row blocks.
+  void setUseSell(bool v) { useSell_ = v && sellCrs_.numSlices > 0; }
+  bool useSell() const { return useSell_; }
+
  private:
   void buildRowBlocks(const std::vector<int64_t>& rowLen, int32_t rowsPerBlock,
                       gph::DevBuf& out, int32_t& nb);
@@ -90,6 +94,19 @@ class DeviceMatrix {
   gph::DevBuf tileStart_, lvlRows_, lvlPtr_, lvlPtrOff_, ulvlRows_, ulvlPtr_,
       ulvlPtrOff_;
   int32_t maxTileRows_ = 0;
+
+  // sliced-ELL copies of the row streams (CRS order / LDU order)
+  struct Sell {
+    int32_t numSlices = 0;
+    gph::DevBuf sliceOff, rowLen, perm, col, val32;
+    mutable gph::DevBuf val64, val16;
+    int64_t slots = 0;
+  };
+  void buildSell(const std::vector<int64_t>& rowPtr, const std::vector<int32_t>& cols,
+                 const std::vector<float>& vals, Sell& out);
+  const void* sellVal(const Sell& s, gph::DType vt) const;
+  Sell sellCrs_, sellLdu_;
+  bool useSell_ = true;
 
   gph::DevBuf scratchState_, scratchPartials_;
   size_t partialsBytes_ = 0;

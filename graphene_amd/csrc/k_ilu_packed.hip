@@ -97,7 +97,15 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
           const int len = (int)(info.x >> 16);
           const X* pp = prod + info.y;
           X acc = BACKWARD ? (X)0 : xs[loc];
-          for (int a = 0; a < len; a++) acc = acc - pp[a];
+          // issue the LDS reads of 8 products at once, then subtract in order
+          for (int a0 = 0; a0 < len; a0 += 8) {
+            X pv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) pv[j] = (a0 + j < len) ? pp[a0 + j] : (X)0;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+              if (a0 + j < len) acc = acc - pv[j];
+          }
           xs[loc] = BACKWARD ? (xs[loc] + acc) * invd[loc] : acc;
         }
         __syncthreads();

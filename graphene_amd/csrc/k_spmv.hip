@@ -19,6 +19,60 @@
 
 namespace gph {
 
+template <typename AT, int RED>
+struct RedAcc {
+  static constexpr int K = (RED == kRedSelfW) ? 2 : 1;
+  AT v[K];
+  __device__ RedAcc() {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = (AT)0;
+  }
+};
+
+// Row result: residual, stores, dot-product contributions.
+template <typename XT, typename BT, typename OT, typename AT, int MODE, int RED>
+__device__ __forceinline__ void spmvRowOut(const SpmvArgs& a, int row, bool has,
+                                           AT sum, RedAcc<AT, RED>& acc) {
+  AT o = sum;
+  if constexpr (MODE == kModeResidual) {
+    if (has) o = (AT)static_cast<const BT*>(a.b)[row] - sum;
+  }
+  if (has) {
+    const OT ov = (OT)o;
+    static_cast<OT*>(a.out)[row] = ov;
+    if (a.out2) static_cast<OT*>(a.out2)[row] = ov;
+    if (a.out3) static_cast<OT*>(a.out3)[row] = ov;
+    if (a.out64) a.out64[row] = (double)o;
+    if constexpr (RED != kRedNone) {
+      AT wv = 0;
+      if constexpr (RED == kRedW || RED == kRedSelfW)
+        wv = (AT) static_cast<const XT*>(a.w)[row];
+      if constexpr (RED == kRedSelf) acc.v[0] = acc.v[0] + o * o;
+      if constexpr (RED == kRedW) acc.v[0] = acc.v[0] + o * wv;
+      if constexpr (RED == kRedSelfW) {
+        acc.v[0] = acc.v[0] + o * o;
+        acc.v[1] = acc.v[1] + o * wv;
+      }
+    }
+  }
+}
+
+// Block reduction of the dot contributions + last-block finalisation.
+template <typename AT, int RED, int NT>
+__device__ __forceinline__ void spmvFinish(const SpmvArgs& a,
+                                           RedAcc<AT, RED>& acc, AT* redLds) {
+  if constexpr (RED != kRedNone) {
+    constexpr int K = RedAcc<AT, RED>::K;
+    AT tot[K];
+    if (gridReduce<AT, K, NT>(acc.v, static_cast<AT*>(a.partials),
+                              &a.state->counter[0], tot, redLds)) {
+      if (threadIdx.x == 0)
+        finalize<AT>(a.state, a.fin, (double)tot[0],
+                     K > 1 ? (double)tot[1] : 0.0);
+    }
+  }
+}
+
 template <int S, typename VT, typename XT, typename BT, typename OT,
           typename AT, int MODE, int RED>
 __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
@@ -74,46 +128,81 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
     }
   }
 
-  AT o = sum;
-  if constexpr (MODE == kModeResidual) {
-    if (has) o = (AT)static_cast<const BT*>(a.b)[row] - sum;
-  }
-  if (has) {
-    const OT ov = (OT)o;
-    static_cast<OT*>(a.out)[row] = ov;
-    if (a.out2) static_cast<OT*>(a.out2)[row] = ov;
-    if (a.out3) static_cast<OT*>(a.out3)[row] = ov;
-    if (a.out64) a.out64[row] = (double)o;
-  }
-  if constexpr (RED != kRedNone) {
-    constexpr int K = (RED == kRedSelfW) ? 2 : 1;
-    AT v[K];
-    AT wv = 0;
-    if constexpr (RED == kRedW || RED == kRedSelfW) {
-      if (has) wv = (AT) static_cast<const XT*>(a.w)[row];
+  RedAcc<AT, RED> acc;
+  spmvRowOut<XT, BT, OT, AT, MODE, RED>(a, row, has, sum, acc);
+  spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);
+}
+
+// Sliced ELL: one wave per 64-row slice; lane l owns row perm[s*64+l]
+// (or s*64+l) and walks its entries k = 0..len-1 at slot
+// sliceOff[s] + k*64 + l, so every wave-level load is one contiguous
+// 256-byte (4-byte types) segment. Per row the order is the stored order,
+// as in the row-block kernel.
+template <typename VT, typename XT, typename BT, typename OT, typename AT,
+          int MODE, int RED>
+__global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
+  __shared__ AT redLds[2 * kSpmvThreads / 64];
+  if (a.checkDone && a.state->done) return;
+  // each block owns one contiguous range of slice groups (4 slices = 4
+  // waves per group); blocks are remapped so an XCD sweeps adjacent ranges
+  const int lb = a.remap ? xcdRemap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  constexpr int W = kSpmvThreads / 64;
+  const int groups = (a.numSlices + W - 1) / W;
+  const int per = (groups + gridDim.x - 1) / gridDim.x;
+  const int g0 = lb * per, g1 = min(groups, g0 + per);
+  const XT* __restrict__ x = static_cast<const XT*>(a.x);
+  RedAcc<AT, RED> acc;
+  for (int g = g0; g < g1; g++) {
+    const int slice = g * W + wave;
+    const bool sliceOk = slice < a.numSlices;
+    const int slot = slice * 64 + lane;
+    int row = 0, len = 0;
+    if (sliceOk) {
+      row = a.perm ? a.perm[slot] : slot;
+      len = a.rowLen[slot];
     }
-    if constexpr (RED == kRedSelf) v[0] = has ? o * o : (AT)0;
-    if constexpr (RED == kRedW) v[0] = has ? o * wv : (AT)0;
-    if constexpr (RED == kRedSelfW) {
-      v[0] = has ? o * o : (AT)0;
-      v[1] = has ? o * wv : (AT)0;
+    const bool has = sliceOk && row < a.n;
+    AT sum = 0;
+    if (has) {
+      sum = cvt<AT>(static_cast<const VT*>(a.diag)[row]) * (AT)x[row];
+      const int base = a.sliceOff[slice] + lane;
+      const int32_t* __restrict__ col = a.scol + base;
+      const VT* __restrict__ val = static_cast<const VT*>(a.sval) + base;
+      int k = 0;
+      for (; k + 4 <= len; k += 4) {
+        const int c0 = col[(k + 0) * 64], c1 = col[(k + 1) * 64];
+        const int c2 = col[(k + 2) * 64], c3 = col[(k + 3) * 64];
+        const VT v0 = val[(k + 0) * 64], v1 = val[(k + 1) * 64];
+        const VT v2 = val[(k + 2) * 64], v3 = val[(k + 3) * 64];
+        const XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        sum = sum + cvt<AT>(v0) * (AT)x0;
+        sum = sum + cvt<AT>(v1) * (AT)x1;
+        sum = sum + cvt<AT>(v2) * (AT)x2;
+        sum = sum + cvt<AT>(v3) * (AT)x3;
+      }
+      for (; k < len; k++) sum = sum + cvt<AT>(val[k * 64]) * (AT)x[col[k * 64]];
     }
-    AT tot[K];
-    if (gridReduce<AT, K, kSpmvThreads>(v, static_cast<AT*>(a.partials),
-                                        &a.state->counter[0], tot, redLds)) {
-      if (tid == 0)
-        finalize<AT>(a.state, a.fin, (double)tot[0],
-                     K > 1 ? (double)tot[1] : 0.0);
-    }
+    spmvRowOut<XT, BT, OT, AT, MODE, RED>(a, row, has, sum, acc);
   }
+  spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);
 }
 
 template <int S, typename VT, typename XT, typename BT, typename OT,
           typename AT, int MODE, int RED>
 static void launchOne(const SpmvArgs& a, hipStream_t s) {
-  if (a.numBlocks <= 0) return;
-  hipLaunchKernelGGL((spmvKernel<S, VT, XT, BT, OT, AT, MODE, RED>),
-                     dim3(a.numBlocks), dim3(kSpmvThreads), 0, s, a);
+  if (S == 0) {
+    if (a.numSlices <= 0) return;
+    const int groups = (a.numSlices + kSpmvThreads / 64 - 1) / (kSpmvThreads / 64);
+    const int nb = groups < kSellGrid ? groups : kSellGrid;
+    hipLaunchKernelGGL((sellKernel<VT, XT, BT, OT, AT, MODE, RED>), dim3(nb),
+                       dim3(kSpmvThreads), 0, s, a);
+  } else {
+    if (a.numBlocks <= 0) return;
+    hipLaunchKernelGGL((spmvKernel<S == 0 ? 1 : S, VT, XT, BT, OT, AT, MODE, RED>),
+                       dim3(a.numBlocks), dim3(kSpmvThreads), 0, s, a);
+  }
   GPH_HIP(hipGetLastError());
 }
 
@@ -138,6 +227,7 @@ static bool dispatchRed(const SpmvArgs& a, int red, hipStream_t s) {
     }
     return false;
   };
+  if (a.numSlices > 0) return go(std::integral_constant<int, 0>{});
   if (a.nstreams == 1) return go(std::integral_constant<int, 1>{});
   if (a.nstreams == 2) return go(std::integral_constant<int, 2>{});
   return false;

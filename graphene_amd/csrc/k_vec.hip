@@ -12,7 +12,7 @@ namespace gph {
 int vecGrid(int64_t n) {
   int64_t g = (n + kVecThreads * 4 - 1) / (kVecThreads * 4);
   if (g < 1) g = 1;
-  if (g > 4096) g = 4096;
+  if (g > 1024) g = 1024;
   return (int)g;
 }
 

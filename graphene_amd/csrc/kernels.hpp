@@ -12,6 +12,7 @@ namespace gph {
 
 constexpr int kSpmvThreads = 256;   // one row per thread, <= 256 rows/block
 constexpr int kSpmvCap = 2048;      // staged products per chunk
+constexpr int kSellGrid = 2048;     // blocks of the sliced-ELL kernel
 constexpr int kVecThreads = 256;
 constexpr int kTileThreads = 256;
 
@@ -40,6 +41,14 @@ struct SpmvArgs {
   int32_t fin = kFinNone;
   int32_t checkDone = 0;
   int32_t remap = 1;
+  // Sliced-ELL form (64 rows per slice, entries column-major inside a
+  // slice): used when numSlices > 0.
+  int32_t numSlices = 0;
+  const int32_t* sliceOff = nullptr;   // numSlices+1, in slots
+  const uint16_t* rowLen = nullptr;    // numSlices*64 entries per lane
+  const int32_t* perm = nullptr;       // lane -> row (null: slice*64+lane)
+  const int32_t* scol = nullptr;
+  const void* sval = nullptr;
 };
 
 enum SpmvMode : int32_t { kModeSpmv = 0, kModeResidual = 1 };

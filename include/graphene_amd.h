@@ -178,6 +178,9 @@ int gph_synchronize(void);
 int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
                       int32_t rows_per_block, gph_matrix** out);
 void gph_matrix_destroy(gph_matrix* A);
+/* No reference counterpart: kernel form of the row gather, 1 = sliced ELL
+ * (default), 0 = row blocks staged in LDS. Both give identical results. */
+int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced);
 int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes);
 
 /* y = A*x (Matrix::operator*, lg/matrix/Matrix.hpp:95-97; MatrixBase::spmv,

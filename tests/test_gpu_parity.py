@@ -30,12 +30,34 @@ def dev(a):
     return g.DeviceVector.from_numpy(np.ascontiguousarray(a))
 
 
-def test_spmv_f32_bit_exact(case):
+@pytest.mark.parametrize("sliced", [True, False])
+def test_spmv_f32_bit_exact(case, sliced):
     name, h, L, A = case
-    x = o.uniform_pm1(11, h.nrows)
-    y = (A * dev(x)).numpy()
-    ref = o.spmv(L, x)
-    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32)), name
+    A.set_sliced(sliced)
+    try:
+        x = o.uniform_pm1(11, h.nrows)
+        y = (A * dev(x)).numpy()
+        ref = o.spmv(L, x)
+        assert np.array_equal(y.view(np.uint32), ref.view(np.uint32)), name
+        r64 = g.DeviceVector(h.nrows, np.float64)
+        x64 = x.astype(np.float64) / 3.0
+        b = o.uniform_pm1(14, h.nrows)
+        r = A.residual(dev(x64), dev(b), destType=g.FLOAT32, intermediateType=g.FLOAT64,
+                       r64=r64)
+        ref32, ref64 = o.residual_mixed(L, x64, b)
+        assert np.array_equal(r.numpy(), ref32) and np.array_equal(r64.numpy(), ref64)
+    finally:
+        A.set_sliced(True)
+
+
+def test_spmv_fp16_values(case):
+    """FP16-stored values: equal to the oracle run on the FP16-rounded values."""
+    name, h, L, A = case
+    x = o.uniform_pm1(21, h.nrows)
+    y = A.spmv(dev(x), value_type=g.FLOAT16).numpy()
+    L16 = dict(L, off=L["off"].astype(np.float16).astype(np.float32),
+               diag=L["diag"].astype(np.float16).astype(np.float32))
+    assert np.array_equal(y, o.spmv(L16, x)), name
 
 
 def test_spmv_f64(case):
@@ -57,11 +79,13 @@ def test_residual_mixed_bit_exact(case):
     assert np.array_equal(r64.numpy(), ref64), name
 
 
-def test_ldu_spmv_bit_exact(case):
+@pytest.mark.parametrize("sliced", [True, False])
+def test_ldu_spmv_bit_exact(case, sliced):
     name, h, L, A = case
     if name.startswith("powerlaw"):
         pytest.skip("LDU needs a structurally symmetric pattern")
     A_ldu = h.copyToTile(g.LDU)
+    A_ldu.set_sliced(sliced)
     U = o.ldu_build(L)
     x = o.uniform_pm1(15, h.nrows)
     y = (A_ldu * dev(x)).numpy()
@@ -173,7 +197,9 @@ def test_pbicgstab_f64_and_mpir(case):
     # the inner solves stop on rounding-sensitive thresholds: outer counts may
     # differ by one, total inner iterations by 10%
     assert abs(st.iterations - sto.iterations) <= 1, (st.iterations, sto.iterations)
-    assert abs(st.innerIterations - inner) <= max(3, 0.1 * inner), (st.innerIterations, inner)
+    if st.iterations == sto.iterations:
+        assert abs(st.innerIterations - inner) <= max(3, 0.1 * inner), \
+            (st.innerIterations, inner)
 
 
 def test_solver_deterministic(case):
