@@ -6,11 +6,33 @@
 
 namespace graphene::matrix {
 
-static void packDirection(const host::HostMatrix& m, bool backward,
+// Consecutive tiles are grouped until a group holds about `targetRows`
+// rows; a group is one wave's unit of work. Tiles are independent blocks,
+// so level L of a group is the union of its tiles' level-L rows and the
+// group's level count is the largest tile level count.
+static std::vector<int32_t> groupTiles(const host::HostMatrix& m,
+                                       int64_t targetRows) {
+  std::vector<int32_t> g;
+  g.push_back(0);
+  int64_t rows = 0;
+  for (int32_t t = 0; t < m.numTiles; t++) {
+    const int64_t nt = m.tileRows(t);
+    if (rows > 0 && rows + nt > targetRows) {
+      g.push_back(t);
+      rows = 0;
+    }
+    rows += nt;
+  }
+  g.push_back(m.numTiles);
+  return g;
+}
+
+static void packDirection(const host::HostMatrix& m,
+                          const std::vector<int32_t>& groups, bool backward,
                           PackedDirection& d, bool& ok) {
-  const int32_t T = m.numTiles;
-  d.slabOff.assign((size_t)T + 1, 0);
-  d.lvlOff.assign((size_t)T + 1, 0);
+  const int32_t G = (int32_t)groups.size() - 1;
+  d.slabOff.assign((size_t)G + 1, 0);
+  d.lvlOff.assign((size_t)G + 1, 0);
   d.rowInfo.assign((size_t)m.nrows * 2, 0);
   const std::vector<int32_t>& rows = backward ? m.upperLevelRows : m.colorSortAddr;
   const std::vector<int64_t>& starts = backward ? m.upperColorStart : m.colorStart;
@@ -18,71 +40,85 @@ static void packDirection(const host::HostMatrix& m, bool backward,
       backward ? m.upperColorStartOffset : m.colorStartOffset;
   int64_t slab = 0;
   std::vector<int32_t> ents;
-  for (int32_t t = 0; t < T; t++) {
-    const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
-    const int64_t nt = t1 - t0;
-    if (nt + 1 > 65535) {
+  for (int32_t gi = 0; gi < G; gi++) {
+    const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
+    const int64_t g0 = m.tileStart[(size_t)ta];
+    const int64_t ng = m.tileStart[(size_t)tb] - g0;
+    if (ng + 1 > 65535) {
       ok = false;
       return;
     }
-    d.slabOff[(size_t)t] = (int32_t)slab;
-    d.lvlOff[(size_t)t] = (int32_t)d.slabEnd.size();
-    const int64_t* ls = starts.data() + startOff[(size_t)t];
-    const int32_t nL = (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1);
+    d.slabOff[(size_t)gi] = (int32_t)slab;
+    d.lvlOff[(size_t)gi] = (int32_t)d.slabEnd.size();
+    int32_t nL = 0;
+    for (int32_t t = ta; t < tb; t++)
+      nL = std::max<int32_t>(
+          nL, (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1));
     d.maxLevels = std::max(d.maxLevels, nL);
-    int64_t tileSlab = 0;
+    int64_t groupSlab = 0;
+    int64_t slot = 0;  // row slot within the group, level order
     for (int32_t L = 0; L < nL; L++) {
-      d.rowStart.push_back((int32_t)ls[L]);
+      d.rowStart.push_back((int32_t)slot);
       int64_t e = 0;  // entries of this level
-      for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
-        const int64_t li = rows[(size_t)(t0 + q)];
-        const int64_t i = t0 + li;
-        ents.clear();
-        const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
-        if (!backward) {
-          for (int64_t a = a0; a < a1; a++) {
-            const int64_t j = m.col[(size_t)a];
-            if (i < j || j < t0 || j >= t1) break;
-            ents.push_back((int32_t)a);
+      for (int32_t t = ta; t < tb; t++) {
+        const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
+        const int64_t* ls = starts.data() + startOff[(size_t)t];
+        const int32_t tL = (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1);
+        if (L >= tL) continue;
+        for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
+          const int64_t i = t0 + rows[(size_t)(t0 + q)];
+          ents.clear();
+          const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
+          if (!backward) {
+            for (int64_t a = a0; a < a1; a++) {
+              const int64_t j = m.col[(size_t)a];
+              if (i < j || j < t0 || j >= t1) break;
+              ents.push_back((int32_t)a);
+            }
+          } else {
+            for (int64_t a = a1 - 1; a >= a0; a--) {
+              const int64_t j = m.col[(size_t)a];
+              if (j < t0 || j >= t1) continue;
+              if (j < i) break;
+              ents.push_back((int32_t)a);
+            }
           }
-        } else {
-          for (int64_t a = a1 - 1; a >= a0; a--) {
-            const int64_t j = m.col[(size_t)a];
-            if (j < t0 || j >= t1) continue;
-            if (j < i) break;
-            ents.push_back((int32_t)a);
+          const int64_t len = ents.empty() ? 1 : (int64_t)ents.size();
+          const size_t rs = (size_t)(g0 + slot) * 2;
+          d.rowInfo[rs] = (uint32_t)(i - g0) | ((uint32_t)len << 16);
+          d.rowInfo[rs + 1] = (uint32_t)e;
+          slot++;
+          if (ents.empty()) {
+            d.col.push_back((uint16_t)ng);
+            d.src.push_back(-1);
+          } else {
+            for (int32_t a : ents) {
+              d.col.push_back((uint16_t)(m.col[(size_t)a] - g0));
+              d.src.push_back(a);
+            }
           }
+          e += len;
         }
-        const int64_t len = ents.empty() ? 1 : (int64_t)ents.size();
-        // row slot q (level order) holds this row's descriptor
-        d.rowInfo[(size_t)(t0 + q) * 2] = (uint32_t)li | ((uint32_t)len << 16);
-        d.rowInfo[(size_t)(t0 + q) * 2 + 1] = (uint32_t)e;
-        if (ents.empty()) {
-          d.col.push_back((uint16_t)nt);
-          d.src.push_back(-1);
-        } else {
-          for (int32_t a : ents) {
-            d.col.push_back((uint16_t)(m.col[(size_t)a] - t0));
-            d.src.push_back(a);
-          }
-        }
-        e += len;
       }
       d.maxLevelEntries = std::max<int32_t>(d.maxLevelEntries, (int32_t)e);
       const int64_t ns = (e + kPackSlab - 1) / kPackSlab;
       for (int64_t p = e; p < ns * kPackSlab; p++) {  // pad the level
-        d.col.push_back((uint16_t)nt);
+        d.col.push_back((uint16_t)ng);
         d.src.push_back(-1);
       }
-      tileSlab += ns;
-      d.slabEnd.push_back((int32_t)tileSlab);
+      groupSlab += ns;
+      d.slabEnd.push_back((int32_t)groupSlab);
     }
-    d.rowStart.push_back((int32_t)nt);
+    if (slot != ng) {
+      ok = false;
+      return;
+    }
+    d.rowStart.push_back((int32_t)ng);
     d.slabEnd.push_back(INT32_MAX);  // sentinel: both tables hold nL+1 entries
-    // The prefetch ring reads up to two depths past a tile's last slab; those
-    // reads land in the next tile's stream (values unused) or in the padding
-    // after the last tile, so no bounds test is needed in the kernel.
-    slab += tileSlab;
+    // The prefetch ring reads up to two depths past a group's last slab;
+    // those reads land in the next group's stream (values unused) or in the
+    // padding after the last group, so no bounds test is needed in the kernel.
+    slab += groupSlab;
     if ((slab + 2 * kPackDepth) * kPackSlab > INT32_MAX) {
       ok = false;
       return;
@@ -92,17 +128,22 @@ static void packDirection(const host::HostMatrix& m, bool backward,
     d.col.push_back(0);
     d.src.push_back(-1);
   }
-  d.slabOff[(size_t)T] = (int32_t)slab;
-  d.lvlOff[(size_t)T] = (int32_t)d.slabEnd.size();
+  d.slabOff[(size_t)G] = (int32_t)slab;
+  d.lvlOff[(size_t)G] = (int32_t)d.slabEnd.size();
 }
 
-PackedILU packILU(const host::HostMatrix& m) {
+PackedILU packILU(const host::HostMatrix& m, int64_t targetRows) {
   PackedILU p;
   p.ok = true;
-  for (int32_t t = 0; t < m.numTiles; t++)
-    p.maxRows = std::max<int32_t>(p.maxRows, (int32_t)m.tileRows(t));
-  packDirection(m, false, p.fwd, p.ok);
-  if (p.ok) packDirection(m, true, p.bwd, p.ok);
+  std::vector<int32_t> groups = groupTiles(m, targetRows);
+  p.groupStart.resize(groups.size());
+  for (size_t q = 0; q < groups.size(); q++) {
+    p.groupStart[q] = (int32_t)m.tileStart[(size_t)groups[q]];
+    if (q > 0)
+      p.maxRows = std::max(p.maxRows, p.groupStart[q] - p.groupStart[q - 1]);
+  }
+  packDirection(m, groups, false, p.fwd, p.ok);
+  if (p.ok) packDirection(m, groups, true, p.bwd, p.ok);
   return p;
 }
 

@@ -1,5 +1,7 @@
 // Level-major packed layout of the ILU(0) factors for the gfx950
-// substitution kernel.
+// substitution kernel. Consecutive tiles are grouped into one wave's unit
+// of work (the tiles are independent blocks, so a group is a block-diagonal
+// "super tile" whose levels are the unions of its tiles' levels).
 //
 // For every tile and direction (forward: strictly lower in-tile entries in
 // ascending column order; backward: strictly upper in-tile entries in
@@ -36,10 +38,13 @@ struct PackedDirection {
 
 struct PackedILU {
   PackedDirection fwd, bwd;
+  std::vector<int32_t> groupStart;  // G+1 row offsets of the wave groups
   int32_t maxRows = 0;
-  bool ok = false;  // false: a tile exceeds the 16-bit local ids
+  bool ok = false;  // false: a group exceeds the 16-bit local ids
 };
 
-PackedILU packILU(const host::HostMatrix& m);
+// Groups consecutive tiles into units of about targetRows rows (one wave
+// each) and packs both substitution directions.
+PackedILU packILU(const host::HostMatrix& m, int64_t targetRows);
 
 }  // namespace graphene::matrix

@@ -170,17 +170,22 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
       const int base = a.sliceOff[slice] + lane;
       const int32_t* __restrict__ col = a.scol + base;
       const VT* __restrict__ val = static_cast<const VT*>(a.sval) + base;
+      // 8 entries per step: 16 independent stream loads, then 8 gathers,
+      // then the adds in stored order
       int k = 0;
-      for (; k + 4 <= len; k += 4) {
-        const int c0 = col[(k + 0) * 64], c1 = col[(k + 1) * 64];
-        const int c2 = col[(k + 2) * 64], c3 = col[(k + 3) * 64];
-        const VT v0 = val[(k + 0) * 64], v1 = val[(k + 1) * 64];
-        const VT v2 = val[(k + 2) * 64], v3 = val[(k + 3) * 64];
-        const XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        sum = sum + cvt<AT>(v0) * (AT)x0;
-        sum = sum + cvt<AT>(v1) * (AT)x1;
-        sum = sum + cvt<AT>(v2) * (AT)x2;
-        sum = sum + cvt<AT>(v3) * (AT)x3;
+      for (; k + 8 <= len; k += 8) {
+        int c[8];
+        VT v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          c[j] = col[(k + j) * 64];
+          v[j] = val[(k + j) * 64];
+        }
+        XT xv[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
       }
       for (; k < len; k++) sum = sum + cvt<AT>(val[k * 64]) * (AT)x[col[k * 64]];
     }

@@ -144,8 +144,9 @@ class IluSolver : public Solver {
 
   // Level-major packed copy of the factors for the one-wave-per-tile kernel.
   void buildPacked(hipStream_t s) {
-    PackedILU P = packILU(A_.hostMatrix());
+    PackedILU P = packILU(A_.hostMatrix(), kRowsPerWave);
     if (!P.ok) return;
+    groupStart_.upload(P.groupStart.data(), P.groupStart.size(), s);
     const void* src = ft_ == DType::F16 ? lu16_.get() : lu_.get();
     const size_t es = gph::dtypeSize(ft_);
     auto dir = [&](const PackedDirection& d, DevBuf& col, DevBuf& val, DevBuf& so,
@@ -172,8 +173,8 @@ class IluSolver : public Solver {
     };
     dir(P.fwd, fcol_, fval_, fso_, flo_, fse_, frs_, fri_, pargs_.fwd);
     dir(P.bwd, bcol_, bval_, bso_, blo_, bse_, brs_, bri_, pargs_.bwd);
-    pargs_.T = A_.numTiles();
-    pargs_.tileStart = A_.tileArgs().tileStart;
+    pargs_.T = (int32_t)P.groupStart.size() - 1;
+    pargs_.tileStart = groupStart_.as<int32_t>();
     pargs_.maxRows = P.maxRows;
     pargs_.maxLevels = std::max(P.fwd.maxLevels, P.bwd.maxLevels);
     pargs_.maxLevelEntries = std::max(P.fwd.maxLevelEntries, P.bwd.maxLevelEntries);
@@ -181,6 +182,8 @@ class IluSolver : public Solver {
   }
 
   static constexpr size_t kMaxPackedShared = 156 * 1024;
+  static constexpr int64_t kRowsPerWave = 2048;  // rows per wave group
+  DevBuf groupStart_;
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_, invD16_;
