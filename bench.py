@@ -34,20 +34,22 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def mpir_cfg(inner_rel, inner_max, storage="float32"):
+def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=2048):
     return {
         "type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 50,
         "extendedPrecisionType": "float64", "workingType": "float32",
         "innerSolver": {
             "type": "PBiCGStab", "relResidual": inner_rel, "maxIterations": inner_max,
             "workingType": "float32", "matrixValueType": storage,
-            "preconditioner": {"type": "ILU", "factorType": storage}}}
+            "preconditioner": {"type": "ILU", "factorType": factors or storage,
+                               "rowsPerWave": rows_per_wave}}}
 
 
-def fp64_cfg(values="float64"):
+def fp64_cfg(values="float64", rows_per_wave=2048):
     return {"type": "PBiCGStab", "relResidual": 1e-10, "maxIterations": 5000,
             "workingType": "float64", "matrixValueType": values,
-            "preconditioner": {"type": "ILU", "factorType": "float64"}}
+            "preconditioner": {"type": "ILU", "factorType": "float64",
+                               "rowsPerWave": rows_per_wave}}
 
 
 def main():

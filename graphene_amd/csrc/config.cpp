@@ -321,6 +321,7 @@ ilu::Configuration::Configuration(const Json& c) {
   readMulticolor(c, "factorizeMulticolor", factorizeMulticolor);
   readBool(c, "diagonalBased", diagonalBased);
   readType(c, "factorType", factorType);
+  readInt(c, "rowsPerWave", rowsPerWave);
 }
 
 gaussseidel::Configuration::Configuration(const Json& c) {

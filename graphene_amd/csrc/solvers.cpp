@@ -144,7 +144,7 @@ class IluSolver : public Solver {
 
   // Level-major packed copy of the factors for the one-wave-per-tile kernel.
   void buildPacked(hipStream_t s) {
-    PackedILU P = packILU(A_.hostMatrix(), kRowsPerWave);
+    PackedILU P = packILU(A_.hostMatrix(), std::max(1, cfg_->rowsPerWave));
     if (!P.ok) return;
     groupStart_.upload(P.groupStart.data(), P.groupStart.size(), s);
     const void* src = ft_ == DType::F16 ? lu16_.get() : lu_.get();
@@ -182,7 +182,6 @@ class IluSolver : public Solver {
   }
 
   static constexpr size_t kMaxPackedShared = 156 * 1024;
-  static constexpr int64_t kRowsPerWave = 2048;  // rows per wave group
   DevBuf groupStart_;
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
