@@ -9,10 +9,22 @@ mixed-precision iterative-refinement solve from x = 0 to relative residual
 
 Printed as one JSON line: value = solves/s of that MPIR solve, plus the
 plain-FP64 PBiCGStab+ILU(0) solve time on the same GPU, the FP16-storage
-variant, the SpMV HBM roofline (HIP-event timing of the LDU SpMV kernel) and
-a CPU baseline (the oracle restatement, one core, bounded sample).
+variant, HBM rooflines of the two kernels an inner iteration runs twice (the
+sliced-ELL SpMV and the packed ILU(0) substitution, HIP-event timing on their
+launch stream) and a CPU baseline (the oracle restatement, one core, bounded
+sample).
+
+Algorithmic bytes per launch (FP32 values, FP32 vectors):
+  SpMV   8 nnz_off + 16 N + 4    (value + int32 column per off-diagonal; diagonal,
+                                  x, y, row pointer per row; one end pointer)
+  ILU    6 nnz_off + 12 N        (value + 16-bit tile-local column per factor
+                                  entry; 1/diag, right-hand side, result per row)
+HBM traffic per launch comes from the FETCH_SIZE / WRITE_SIZE counter summary
+under profiles/ (scaled by the same counters on a plain copy kernel), null when
+that summary is absent.
 
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--n 128]
+       python bench.py --kernels-only   # SpMV, ILU apply and copy kernels only
 """
 import argparse
 import json
@@ -34,7 +46,44 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=2048):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_convdiff27_128.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch for `key` from the counter summary, else None.
+
+    The summary holds lines `traffic <key>: <bytes> bytes/launch`."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if line.startswith(f"traffic {key}:"):
+                    return float(line.split(":")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(achieved_gbs, algo_bytes, traffic_key, kernel):
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": measured_traffic(traffic_key), "algorithmic_bytes": algo_bytes}
+
+
+def copy_calibration(g, n=1 << 28, iters=5):
+    """A plain FP32 copy kernel of n elements (4n bytes read, 4n written)."""
+    a = g.DeviceVector(n, np.float32)
+    a.fill(1.0)
+    t = time.perf_counter()
+    for _ in range(iters):
+        b = a.cast(np.float32)
+        del b
+    dt = (time.perf_counter() - t) / iters
+    return {"elements": n, "bytes_read": 4 * n, "bytes_written": 4 * n,
+            "launches": iters, "host_ms_per_launch": dt * 1e3}
+
+
+def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=1024):
     return {
         "type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 50,
         "extendedPrecisionType": "float64", "workingType": "float32",
@@ -45,7 +94,7 @@ def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wav
                                "rowsPerWave": rows_per_wave}}}
 
 
-def fp64_cfg(values="float64", rows_per_wave=2048):
+def fp64_cfg(values="float64", rows_per_wave=1024):
     return {"type": "PBiCGStab", "relResidual": 1e-10, "maxIterations": 5000,
             "workingType": "float64", "matrixValueType": values,
             "preconditioner": {"type": "ILU", "factorType": "float64",
@@ -58,13 +107,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--block", default="16,16,8")
+    ap.add_argument("--block", default="16,8,8")
     ap.add_argument("--inner-rel", type=float, default=1e-3)
     ap.add_argument("--inner-max", type=int, default=100)
     ap.add_argument("--format", default="ldu", choices=["ldu", "crs"])
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--spmv-only", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="SpMV, ILU apply and copy-calibration kernels only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -94,20 +144,35 @@ def main():
     b = g.DeviceVector.from_numpy(bx64.numpy().astype(np.float32))
     bnorm = float(np.linalg.norm(b.numpy().astype(np.float64)))
 
-    # ---- SpMV roofline (FP32 values / int32 columns) -------------------------
+    # ---- kernel rooflines: SpMV and ILU(0) apply (FP32) -----------------------
     nnz = h.nnz_off
     spmv_bytes = 8 * nnz + 16 * N + 4
+    ilu_bytes = 6 * nnz + 12 * N
     x32 = g.DeviceVector.from_numpy(xt.astype(np.float32))
     y32 = g.DeviceVector(N, np.float32)
     spmv = {}
-    for name, M in (("ldu", A), ("crs", A_crs)):
+    forms = (("ldu", A), ("crs", A_crs)) if not args.kernels_only else ((args.format, A),)
+    for name, M in forms:
         ms = M.bench_spmv(x32, y32, iters=50)
         spmv[name] = {"ms": ms, "GBs": spmv_bytes / ms / 1e6,
                       "frac": spmv_bytes / ms / 1e6 / HBM_PEAK_GBS}
         log(f"SpMV {name}: {ms*1e3:.1f} us, {spmv[name]['GBs']:.0f} GB/s "
             f"({100*spmv[name]['frac']:.1f}% of HBM peak)")
-    if args.spmv_only:
-        print(json.dumps({"spmv": spmv}))
+    P = g.Solver(A, {"type": "ILU", "factorType": "float32"})
+    ilu_ms = P.bench_apply(x32, y32, iters=20)
+    ilu = {"ms": ilu_ms, "GBs": ilu_bytes / ilu_ms / 1e6}
+    log(f"ILU(0) apply: {ilu_ms*1e3:.1f} us, {ilu['GBs']:.0f} GB/s "
+        f"({100*ilu['GBs']/HBM_PEAK_GBS:.1f}% of HBM peak)")
+    del P
+    sp = spmv[args.format]
+    rl_spmv = roofline(sp["GBs"], spmv_bytes, f"spmv-{args.format}",
+                       f"sellKernel<float> SpMV ({args.format.upper()})")
+    rl_ilu = roofline(ilu["GBs"], ilu_bytes, "ilu-apply",
+                      "iluPackedKernel<float,float> (ILU(0) fwd+bwd substitution)")
+    if args.kernels_only:
+        cal = copy_calibration(g)
+        print(json.dumps({"spmv": spmv, "ilu": ilu, "copy": cal,
+                          "roofline_spmv": rl_spmv, "roofline_ilu": rl_ilu}))
         return
 
     def run(cfg, xdtype, label, reps=1):
@@ -193,7 +258,9 @@ def main():
                          f"MPIR solve"}
         log(f"CPU oracle: {per_it:.2f} s/iteration -> {est:.1f} s per solve (estimate)")
 
-    ldu = spmv["ldu"]
+    # the kernel with the larger per-launch time (both run twice per inner
+    # iteration) is the dominant one
+    dominant, other = (rl_ilu, rl_spmv) if ilu_ms >= sp["ms"] else (rl_spmv, rl_ilu)
     out = {
         "metric": METRIC, "value": 1e3 / ms_step, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -206,11 +273,10 @@ def main():
         "wall_s": wall,
         "mpir": {"outer": stats.iterations, "inner": stats.innerIterations,
                  "true_rel_residual": mpir_rel, "converged": stats.converged},
-        "roofline": {"bound": "hbm", "kernel": f"spmv {args.format}",
-                     "achieved": ldu["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ldu["frac"], "traffic": None,
-                     "algorithmic_bytes": spmv_bytes},
+        "roofline": dominant,
+        "roofline_other": other,
         "spmv": spmv,
+        "ilu_apply": ilu,
         "cpu_baseline": cpu,
     }
     out.update(extra)

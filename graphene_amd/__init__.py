@@ -84,6 +84,7 @@ _SIGS = {
     "gph_memcpy_dtoh": ([_P, _P, C.c_size_t], C.c_int),
     "gph_memset": ([_P, C.c_int, C.c_size_t], C.c_int),
     "gph_synchronize": ([], C.c_int),
+    "gph_vector_cast": ([C.c_int, _P, C.c_int, _P, C.c_int64], C.c_int),
     "gph_matrix_create": ([_P, C.c_int, C.c_int32, C.POINTER(_P)], C.c_int),
     "gph_matrix_destroy": ([_P], None),
     "gph_matrix_set_sliced": ([_P, C.c_int32], C.c_int),
@@ -95,6 +96,7 @@ _SIGS = {
     "gph_solver_create": ([_P, C.c_char_p, C.POINTER(_P)], C.c_int),
     "gph_solver_destroy": ([_P], None),
     "gph_solve": ([_P, C.c_int, _P, C.c_int, _P, C.POINTER(_Stats)], C.c_int),
+    "gph_bench_apply": ([_P, C.c_int, _P, _P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "gph_solver_name": ([_P, C.c_char_p, C.c_size_t], C.c_int),
     "gph_solver_uses_initial_guess": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "gph_config_check": ([C.c_char_p, C.c_char_p, C.c_size_t], C.c_int),
@@ -351,6 +353,12 @@ class DeviceVector:
     def fill(self, value):
         self.upload(np.full(self.n, value, self.dtype))
 
+    def cast(self, dtype):
+        """Expression::cast (libgraphene/dsl/tensor/Expression.hpp:152) on the device."""
+        out = DeviceVector(self.n, dtype)
+        _check(lib().gph_vector_cast(self.type, self.ptr, out.type, out.ptr, self.n))
+        return out
+
 
 class SolverStats:
     """lg/matrix/solver/SolverStats.hpp:29-69."""
@@ -406,6 +414,13 @@ class Solver:
         o = C.c_int32()
         _check(lib().gph_solver_uses_initial_guess(self._S, C.byref(o)))
         return bool(o.value)
+
+    def bench_apply(self, rhs, out, iters=20):
+        """Mean device ms of one preconditioner application out = M^-1 rhs."""
+        ms = C.c_double()
+        _check(lib().gph_bench_apply(self._S, rhs.type, rhs.ptr, out.ptr, iters,
+                                     C.byref(ms)))
+        return ms.value
 
     def solve(self, x, b):
         st = _Stats()

@@ -373,6 +373,17 @@ int gph_memset(void* dst, int value, size_t bytes) {
 int gph_synchronize(void) {
   return guard([&] { GPH_HIP(hipDeviceSynchronize()); });
 }
+int gph_vector_cast(gph_type in_type, const void* in, gph_type out_type,
+                    void* out, int64_t n) {
+  return guard([&] {
+    if (n < 0) gph::fail(gph::kInvalid, "negative length");
+    if (n == 0) return;
+    need(in, "in");
+    need(out, "out");
+    gph::launchCast(in, dt(in_type), out, dt(out_type), n, nullptr);
+    GPH_HIP(hipStreamSynchronize(nullptr));
+  });
+}
 
 int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
                       int32_t rows_per_block, gph_matrix** out) {
@@ -507,6 +518,33 @@ int gph_solve(gph_solver* S, gph_type x_type, void* x, gph_type b_type,
       std::memset(stats->name, 0, sizeof(stats->name));
       std::strncpy(stats->name, st.solverName.c_str(), sizeof(stats->name) - 1);
     }
+  });
+}
+int gph_bench_apply(gph_solver* S, gph_type t, const void* rhs, void* out,
+                    int32_t iters, double* avg_ms) {
+  return guard([&] {
+    need(S, "solver");
+    need(rhs, "rhs");
+    need(out, "out");
+    need(avg_ms, "avg_ms");
+    if (iters < 1) gph::fail(gph::kInvalid, "iters must be >= 1");
+    if (!S->S->supportsDeviceApply())
+      gph::fail(gph::kUnsupported, S->S->name() + " has no preconditioner apply");
+    hipStream_t s = S->S->matrix().stream();
+    Vec in{const_cast<void*>(rhs), dt(t)}, o{out, dt(t)};
+    S->S->applyDevice(in, o, gph::PreOp{}, nullptr, s);  // warm-up
+    hipEvent_t e0, e1;
+    GPH_HIP(hipEventCreate(&e0));
+    GPH_HIP(hipEventCreate(&e1));
+    GPH_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; i++) S->S->applyDevice(in, o, gph::PreOp{}, nullptr, s);
+    GPH_HIP(hipEventRecord(e1, s));
+    GPH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
   });
 }
 int gph_solver_name(const gph_solver* S, char* buf, size_t len) {

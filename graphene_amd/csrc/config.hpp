@@ -108,7 +108,7 @@ struct Configuration : solver::Configuration {
   // extension: storage type of the factors (default: matrix type float32)
   std::optional<gph::DType> factorType;
   // extension: rows per wave group of the packed substitution kernel
-  int rowsPerWave = 2048;
+  int rowsPerWave = 1024;
   explicit Configuration(const Json& c);
   std::string solverName() const override {
     return diagonalBased ? "DILU" : "ILU(0)";

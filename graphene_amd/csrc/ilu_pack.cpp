@@ -115,6 +115,13 @@ static void packDirection(const host::HostMatrix& m,
     }
     d.rowStart.push_back((int32_t)ng);
     d.slabEnd.push_back(INT32_MAX);  // sentinel: both tables hold nL+1 entries
+    if (groupSlab % 2) {  // streams start on an even slab (pairSlabs)
+      for (int64_t p = 0; p < kPackSlab; p++) {
+        d.col.push_back((uint16_t)ng);
+        d.src.push_back(-1);
+      }
+      groupSlab++;
+    }
     // The prefetch ring reads up to two depths past a group's last slab;
     // those reads land in the next group's stream (values unused) or in the
     // padding after the last group, so no bounds test is needed in the kernel.

@@ -44,7 +44,21 @@ struct PackedILU {
 };
 
 // Groups consecutive tiles into units of about targetRows rows (one wave
-// each) and packs both substitution directions.
+// each) and packs both substitution directions. Every group's stream has an
+// even number of slabs.
 PackedILU packILU(const host::HostMatrix& m, int64_t targetRows);
+
+// Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
+// that lane l loads both of its slots with one 32-bit load,
+// out[p*128 + 2l + (s&1)] = in[s*64 + l].
+template <typename T>
+std::vector<T> pairSlabs(const std::vector<T>& in) {
+  std::vector<T> out(in.size());
+  const size_t nSlabs = in.size() / kPackSlab;
+  for (size_t s = 0; s < nSlabs; s++)
+    for (size_t l = 0; l < (size_t)kPackSlab; l++)
+      out[(s >> 1) * 2 * kPackSlab + 2 * l + (s & 1)] = in[s * kPackSlab + l];
+  return out;
+}
 
 }  // namespace graphene::matrix

@@ -55,7 +55,7 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
                                             int t0, int nt, X* xs,
                                             const X* invd, X* prod,
                                             uint2* rows, int* slabEnd,
-                                            int* rowStart) {
+                                            int* rowStart, int dilu) {
   const int lane = threadIdx.x;
   const int lo = d.lvlOff[tile];
   const int nL = d.lvlOff[tile + 1] - lo - 1;
@@ -68,26 +68,47 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
   __syncthreads();
   const int S = nL > 0 ? slabEnd[nL - 1] : 0;
   const int Stot = ((S + kDepth - 1) / kDepth) * kDepth;
-  const size_t base = (size_t)d.slabOff[tile] * kPackedThreads + lane;
-  const uint16_t* __restrict__ cp = d.col + base;
-  const FT* __restrict__ vp = static_cast<const FT*>(d.val) + base;
+  // 16-bit slots are stored slab-pair interleaved (the stream starts on an
+  // even slab): lane l finds slots (2p, l) and (2p+1, l) in the low and high
+  // half of 32-bit word p*64 + l. The ring holds these words as loaded;
+  // narrowing them in the loop would make every iteration wait on the loads.
+  constexpr bool kHalfVals = sizeof(FT) == 2;
+  const size_t wbase = (size_t)d.slabOff[tile] * (kPackedThreads / 2) + lane;
+  const uint32_t* __restrict__ cp = reinterpret_cast<const uint32_t*>(d.col) + wbase;
+  const uint32_t* __restrict__ hp = reinterpret_cast<const uint32_t*>(d.val) + wbase;
+  const FT* __restrict__ vp =
+      static_cast<const FT*>(d.val) + (size_t)d.slabOff[tile] * kPackedThreads + lane;
 
-  uint32_t c[kDepth];
-  FT v[kDepth];
-#pragma unroll
-  for (int u = 0; u < kDepth; u++) {
-    c[u] = cp[u * kPackedThreads];
-    v[u] = vp[u * kPackedThreads];
-  }
+  // The ring starts empty (zeros) and the loop starts one depth early: the
+  // first pass only issues the loads, in the same order as every later
+  // pass, so the load waits the compiler places hold for the steady state.
+  uint32_t cw[kDepth / 2] = {};
+  uint32_t hw[kHalfVals ? kDepth / 2 : 1] = {};
+  FT v[kHalfVals ? 1 : kDepth] = {};
   int lvl = 0, lvlFirst = 0, lvlEnd = slabEnd[0];
-  for (int sb = 0; sb < Stot; sb += kDepth) {
+  for (int sb = -kDepth; sb < Stot; sb += kDepth) {
 #pragma unroll
     for (int u = 0; u < kDepth; u++) {
       const int s = sb + u;
-      if (s < S) prod[(s - lvlFirst) * kPackedThreads + lane] = cvt<X>(v[u]) * xs[c[u]];
-      const size_t nx = (size_t)(s + kDepth) * kPackedThreads;
-      c[u] = cp[nx];
-      v[u] = vp[nx];
+      const uint32_t c = (u & 1) ? cw[u >> 1] >> 16 : cw[u >> 1] & 0xffffu;
+      X a;
+      if constexpr (kHalfVals)
+        a = cvt<X>(__ushort_as_half(
+            (unsigned short)((u & 1) ? hw[u >> 1] >> 16 : hw[u >> 1] & 0xffffu)));
+      else
+        a = cvt<X>(v[u]);
+      // slabs before 0 (empty ring) and past S (ring overrun into the next
+      // group's stream or the padding, after the last level's row sums)
+      // write their product to a dead slot; every ring word is consumed on
+      // every path, so no refill has to wait for an unused load
+      const int pi = (s >= 0 && s < S) ? (s - lvlFirst) * kPackedThreads + lane : lane;
+      prod[pi] = a * xs[c];
+      if (u & 1) {  // both slots of the word consumed: refill it
+        const size_t nw = (size_t)((s + kDepth) >> 1) * kPackedThreads;
+        cw[u >> 1] = cp[nw];
+        if constexpr (kHalfVals) hw[u >> 1] = hp[nw];
+      }
+      if constexpr (!kHalfVals) v[u] = vp[(size_t)(s + kDepth) * kPackedThreads];
       if (s + 1 == lvlEnd) {
         __syncthreads();
         const int r1 = rowStart[lvl + 1];
@@ -106,7 +127,10 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
             for (int j = 0; j < 8; j++)
               if (a0 + j < len) acc = acc - pv[j];
           }
-          xs[loc] = BACKWARD ? (xs[loc] + acc) * invd[loc] : acc;
+          if (BACKWARD)
+            xs[loc] = dilu ? xs[loc] + acc * invd[loc] : (xs[loc] + acc) * invd[loc];
+          else
+            xs[loc] = dilu ? acc * invd[loc] : acc;
         }
         __syncthreads();
         lvl++;
@@ -168,9 +192,9 @@ __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
   if (lane == 0) xs[nt] = (X)0;
   __syncthreads();
   packedSweep<FT, X, false>(p.fwd, tile, t0, nt, xs, invd, prod, rows, slabEnd,
-                            rowStart);
+                            rowStart, p.dilu);
   packedSweep<FT, X, true>(p.bwd, tile, t0, nt, xs, invd, prod, rows, slabEnd,
-                           rowStart);
+                           rowStart, p.dilu);
   for (int q = lane; q < nt; q += kPackedThreads) out[t0 + q] = xs[q];
 }
 

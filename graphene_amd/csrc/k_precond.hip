@@ -25,12 +25,14 @@ static size_t maxDynamicShared() { return 160 * 1024 - 4096; }
 // Row i: for each k = col(ik) < i in-tile: a_ik /= a_kk; a_ij -= a_ik*a_kj
 // for the later in-tile entries j of row i (a_kj = 0 if absent); then
 // a_ii -= a_ik*a_ki. ilu/Solver.cpp:167-218. Afterwards diag = 1/diag
-// (:290-291).
+// (:290-291). DILU (diagonalBased, :191-216): the off-diagonals stay A's
+// and a_ii -= (a_ik*a_ki)/a_kk.
 // ---------------------------------------------------------------------------
 template <typename FT>
 __global__ __launch_bounds__(kTileThreads) void iluFactorKernel(TileArgs t,
                                                                 FT* off,
-                                                                FT* diag) {
+                                                                FT* diag,
+                                                                int dilu) {
   const int tile = blockIdx.x;
   const int t0 = t.tileStart[tile], t1 = t.tileStart[tile + 1];
   const int* lp = t.lvlPtr + t.lvlPtrOff[tile];
@@ -45,10 +47,20 @@ __global__ __launch_bounds__(kTileThreads) void iluFactorKernel(TileArgs t,
         const int k = col[ik];
         if (k >= i || k < t0) break;  // upper part or halo reached
         const FT a_kk = diag[k];
+        const int kEnd = rowPtr[k + 1];
+        if (dilu) {
+          FT a_ki = 0;
+          for (int qq = rowPtr[k]; qq < kEnd; qq++) {
+            const int c = col[qq];
+            if (c < t0 || c >= t1 || c > i) break;
+            if (c == i) a_ki = off[qq];
+          }
+          diag[i] = diag[i] - off[ik] * a_ki / a_kk;
+          continue;
+        }
         const FT a_ik = off[ik] / a_kk;
         off[ik] = a_ik;
         int qk = rowPtr[k];
-        const int kEnd = rowPtr[k + 1];
         for (int ij = ik + 1; ij < iEnd; ij++) {
           const int j = col[ij];
           if (j < t0 || j >= t1) break;  // stop at halo coefficients
@@ -76,16 +88,16 @@ __global__ __launch_bounds__(kTileThreads) void iluFactorKernel(TileArgs t,
 }
 
 void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
-                     hipStream_t s) {
+                     bool dilu, hipStream_t s) {
   if (t.T <= 0) return;
   if (ft == DType::F32)
     hipLaunchKernelGGL(iluFactorKernel<float>, dim3(t.T), dim3(kTileThreads),
                        0, s, t, static_cast<float*>(off),
-                       static_cast<float*>(diag));
+                       static_cast<float*>(diag), dilu ? 1 : 0);
   else if (ft == DType::F64)
     hipLaunchKernelGGL(iluFactorKernel<double>, dim3(t.T), dim3(kTileThreads),
                        0, s, t, static_cast<double*>(off),
-                       static_cast<double*>(diag));
+                       static_cast<double*>(diag), dilu ? 1 : 0);
   else
     fail(kUnsupported, "ILU factorization type must be float32 or float64");
   GPH_HIP(hipGetLastError());
@@ -104,7 +116,7 @@ template <typename FT, typename X, bool USE_LDS>
 __global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
     TileArgs t, const FT* __restrict__ lu, const FT* __restrict__ invD,
     const X* __restrict__ rhs, X* __restrict__ out, PreOp pre, DevState* st,
-    int checkDone) {
+    int checkDone, int dilu) {
   extern __shared__ unsigned char smemRaw[];
   if (checkDone && st->done) return;
   const int tile = blockIdx.x;
@@ -153,7 +165,7 @@ __global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
         if (j < t0 || j >= t1) break;
         s = s - cvt<X>(lu[a]) * xs[j - t0];
       }
-      xs[li] = s;
+      xs[li] = dilu ? s * cvt<X>(invD[i]) : s;
     }
     __syncthreads();
   }
@@ -171,7 +183,7 @@ __global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
         if (j < i) break;
         s = s - cvt<X>(lu[a]) * xs[j - t0];
       }
-      xs[li] = (xs[li] + s) * cvt<X>(invD[i]);
+      xs[li] = dilu ? xs[li] + s * cvt<X>(invD[i]) : (xs[li] + s) * cvt<X>(invD[i]);
     }
     __syncthreads();
   }
@@ -183,7 +195,7 @@ __global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
 template <typename FT, typename X>
 static void iluApplyTyped(const TileArgs& t, const void* lu, const void* invD,
                           const void* rhs, void* out, const PreOp& pre,
-                          DevState* st, int checkDone, hipStream_t s) {
+                          DevState* st, int checkDone, int dilu, hipStream_t s) {
   const size_t shm = (size_t)t.maxTileRows * sizeof(X);
   if (shm <= maxDynamicShared()) {
     auto k = iluApplyKernel<FT, X, true>;
@@ -195,31 +207,32 @@ static void iluApplyTyped(const TileArgs& t, const void* lu, const void* invD,
                        static_cast<const FT*>(lu),
                        static_cast<const FT*>(invD),
                        static_cast<const X*>(rhs), static_cast<X*>(out), pre,
-                       st, checkDone);
+                       st, checkDone, dilu);
   } else {
     hipLaunchKernelGGL((iluApplyKernel<FT, X, false>), dim3(t.T),
                        dim3(kTileThreads), 0, s, t,
                        static_cast<const FT*>(lu),
                        static_cast<const FT*>(invD),
                        static_cast<const X*>(rhs), static_cast<X*>(out), pre,
-                       st, checkDone);
+                       st, checkDone, dilu);
   }
   GPH_HIP(hipGetLastError());
 }
 
 void launchIluApply(const TileArgs& t, const void* lu, const void* invD,
                     DType ft, const void* rhs, void* out, DType xt,
-                    const PreOp& pre, DevState* st, int checkDone,
+                    const PreOp& pre, DevState* st, int checkDone, bool dilu,
                     hipStream_t s) {
   if (t.T <= 0) return;
+  const int d = dilu ? 1 : 0;
   if (ft == DType::F32 && xt == DType::F32)
-    iluApplyTyped<float, float>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+    iluApplyTyped<float, float>(t, lu, invD, rhs, out, pre, st, checkDone, d, s);
   else if (ft == DType::F16 && xt == DType::F32)
-    iluApplyTyped<__half, float>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+    iluApplyTyped<__half, float>(t, lu, invD, rhs, out, pre, st, checkDone, d, s);
   else if (ft == DType::F64 && xt == DType::F64)
-    iluApplyTyped<double, double>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+    iluApplyTyped<double, double>(t, lu, invD, rhs, out, pre, st, checkDone, d, s);
   else if (ft == DType::F32 && xt == DType::F64)
-    iluApplyTyped<float, double>(t, lu, invD, rhs, out, pre, st, checkDone, s);
+    iluApplyTyped<float, double>(t, lu, invD, rhs, out, pre, st, checkDone, d, s);
   else
     fail(kUnsupported, std::string("unsupported ILU apply types: factors ") +
                            dtypeName(ft) + ", vectors " + dtypeName(xt));

@@ -116,13 +116,14 @@ struct PreOp {
 };
 
 // In-place ILU(0) factorization of copies of off/diag (ft = f32 or f64),
-// diagonal inverted at the end.
+// diagonal inverted at the end. dilu: diagonal-based ILU, only diag changes.
 void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
-                     hipStream_t s);
-// out = (LU)^-1 rhs per tile; ft factor type, xt vector type.
+                     bool dilu, hipStream_t s);
+// out = (LU)^-1 rhs per tile; ft factor type, xt vector type. dilu: the
+// factors are A's off-diagonals and the DILU diagonal.
 void launchIluApply(const TileArgs& t, const void* lu, const void* invD,
                     DType ft, const void* rhs, void* out, DType xt,
-                    const PreOp& pre, DevState* st, int checkDone,
+                    const PreOp& pre, DevState* st, int checkDone, bool dilu,
                     hipStream_t s);
 // One Gauss-Seidel sweep per tile, xout != xin.
 void launchGsSweep(const TileArgs& t, const void* off, const void* diag,
@@ -146,6 +147,7 @@ struct PackedArgs {
   int32_t T = 0;
   const int32_t* tileStart = nullptr;
   int32_t maxRows = 0, maxLevels = 0, maxLevelEntries = 0;
+  int32_t dilu = 0;  // diagonal-based ILU: x_i = t*invD_i / x_i + t*invD_i
   PackedDirArgs fwd, bwd;
 };
 constexpr int kPackedThreads = 64;  // one wave per tile

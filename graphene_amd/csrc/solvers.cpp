@@ -70,14 +70,13 @@ class PinnedState {
 
 // ---------------------------------------------------------------------------
 // ILU(0): factorization in the constructor (ilu/Solver.cpp:44-51), apply per
-// tile. DILU (diagonalBased) is not part of this build.
+// tile. DILU (diagonalBased, ilu/Solver.cpp:191-216, :405-441) keeps A's
+// off-diagonals as factors and only modifies the diagonal.
 // ---------------------------------------------------------------------------
 class IluSolver : public Solver {
  public:
   IluSolver(const DeviceMatrix& A, std::shared_ptr<ilu::Configuration> c)
       : Solver(A), cfg_(std::move(c)) {
-    if (cfg_->diagonalBased)
-      fail(kUnsupported, "DILU (diagonalBased) is not supported by this build");
     ft_ = cfg_->factorType.value_or(DType::F32);
     factorize();
   }
@@ -114,7 +113,7 @@ class IluSolver : public Solver {
       return;
     }
     gph::launchIluApply(A_.tileArgs(), lu, inv, ft, rhs.ptr, out.ptr, out.type,
-                        pre, st, st ? 1 : 0, s);
+                        pre, st, st ? 1 : 0, cfg_->diagonalBased, s);
   }
 
   size_t factorBytes() const {
@@ -131,7 +130,8 @@ class IluSolver : public Solver {
     invD_.alloc((size_t)n * gph::dtypeSize(work));
     gph::launchCast(A_.crsOff(DType::F32), DType::F32, lu_.get(), work, nnz, s);
     gph::launchCast(A_.crsDiag(DType::F32), DType::F32, invD_.get(), work, n, s);
-    gph::launchIluFactor(A_.tileArgs(), lu_.get(), invD_.get(), work, s);
+    gph::launchIluFactor(A_.tileArgs(), lu_.get(), invD_.get(), work,
+                         cfg_->diagonalBased, s);
     if (ft_ == DType::F16) {
       lu16_.alloc((size_t)std::max<int64_t>(nnz, 1) * 2);
       invD16_.alloc((size_t)n * 2);
@@ -152,9 +152,14 @@ class IluSolver : public Solver {
     auto dir = [&](const PackedDirection& d, DevBuf& col, DevBuf& val, DevBuf& so,
                    DevBuf& lo, DevBuf& se, DevBuf& rs, DevBuf& ri,
                    gph::PackedDirArgs& a) {
-      col.upload(d.col.data(), d.col.size(), s);
+      if ((d.col.size() / kPackSlab) % 2 != 0)
+        fail(gph::kRuntime, "packed ILU stream has an odd number of slabs");
+      const std::vector<uint16_t> colDev = pairSlabs(d.col);
+      col.upload(colDev.data(), colDev.size(), s);
+      // 16-bit factor values use the paired slab order as well
+      const std::vector<int32_t> srcDev = es == 2 ? pairSlabs(d.src) : d.src;
       DevBuf idx;
-      idx.upload(d.src.data(), d.src.size(), s);
+      idx.upload(srcDev.data(), srcDev.size(), s);
       val.alloc(d.src.size() * es);
       gph::launchGather(src, ft_, idx.as<int32_t>(), val.get(), (int64_t)d.src.size(), s);
       GPH_HIP(hipStreamSynchronize(s));
@@ -176,6 +181,7 @@ class IluSolver : public Solver {
     pargs_.T = (int32_t)P.groupStart.size() - 1;
     pargs_.tileStart = groupStart_.as<int32_t>();
     pargs_.maxRows = P.maxRows;
+    pargs_.dilu = cfg_->diagonalBased ? 1 : 0;
     pargs_.maxLevels = std::max(P.fwd.maxLevels, P.bwd.maxLevels);
     pargs_.maxLevelEntries = std::max(P.fwd.maxLevelEntries, P.bwd.maxLevelEntries);
     packed_ = true;

@@ -170,6 +170,11 @@ int gph_memcpy_htod(void* dst, const void* src, size_t bytes);
 int gph_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int gph_memset(void* dst, int value, size_t bytes);
 int gph_synchronize(void);
+/* out = (out_type) in, n elements, one streaming kernel on the default stream
+ * (Expression::cast, libgraphene/dsl/tensor/Expression.hpp:152). With
+ * in_type == out_type it is a plain copy kernel. */
+int gph_vector_cast(gph_type in_type, const void* in, gph_type out_type,
+                    void* out, int64_t n);
 
 /* ---- device matrices ------------------------------------------------------ */
 /* HostMatrix::copyToTile, lg/matrix/host/HostMatrix.hpp:53 (CRS) and the
@@ -211,6 +216,12 @@ void gph_solver_destroy(gph_solver* S);
  * (lg/matrix/Matrix.cpp:26-30) is create + solve. */
 int gph_solve(gph_solver* S, gph_type x_type, void* x, gph_type b_type,
               const void* b, gph_solver_stats* stats);
+/* Mean device time (ms) of `iters` back-to-back preconditioner applications
+ * out = M^-1 rhs of an ILU or Gauss-Seidel solver (the apply inside
+ * pbicgstab/Solver.cpp's iteration), HIP events on the launch stream. No
+ * reference counterpart: a timing hook for the substitution kernels. */
+int gph_bench_apply(gph_solver* S, gph_type t, const void* rhs, void* out,
+                    int32_t iters, double* avg_ms);
 /* Solver::name(), Solver::usesInitialGuess() */
 int gph_solver_name(const gph_solver* S, char* buf, size_t len);
 int gph_solver_uses_initial_guess(const gph_solver* S, int32_t* out);

@@ -249,25 +249,32 @@ def ldu_residual_mixed(U, x64, b32):
     return r32, r64
 
 
-def ilu0_factor(L, dtype=np.float32):
+def ilu0_factor(L, dtype=np.float32, diagonalBased=False):
+    """ILU(0) factors (lu, inverted diag); DILU: lu = A's off-diagonals."""
     rp, col, off, diag = _sys_arrays(L)
     n = len(diag)
     ts = np.ascontiguousarray(L["tile_start"], np.int64)
     T = len(ts) - 1
     lu = off.astype(dtype).copy()
     d = diag.astype(dtype).copy()
-    f = lib().orc_ilu0_factor_f32 if dtype == np.float32 else lib().orc_ilu0_factor_f64
+    if diagonalBased:
+        f = lib().orc_dilu_factor_f32 if dtype == np.float32 else lib().orc_dilu_factor_f64
+    else:
+        f = lib().orc_ilu0_factor_f32 if dtype == np.float32 else lib().orc_ilu0_factor_f64
     f(_i64(n), C.c_int32(T), _p(ts), _p(rp), _p(col), _p(lu), _p(d))
     return lu, d
 
 
-def ilu0_apply(L, lu, invD, b):
+def ilu0_apply(L, lu, invD, b, diagonalBased=False):
     rp, col, _, _ = _sys_arrays(L)
     ts = np.ascontiguousarray(L["tile_start"], np.int64)
     T = len(ts) - 1
     x = np.zeros(len(b), lu.dtype)
     b = np.ascontiguousarray(b, lu.dtype)
-    f = lib().orc_ilu0_apply_f32 if lu.dtype == np.float32 else lib().orc_ilu0_apply_f64
+    if diagonalBased:
+        f = lib().orc_dilu_apply_f32 if lu.dtype == np.float32 else lib().orc_dilu_apply_f64
+    else:
+        f = lib().orc_ilu0_apply_f32 if lu.dtype == np.float32 else lib().orc_ilu0_apply_f64
     f(C.c_int32(T), _p(ts), _p(rp), _p(col), _p(lu), _p(invD), _p(b), _p(x))
     return x
 
@@ -322,7 +329,7 @@ class System:
         c.maxIterations = maxIterations
         c.minIterations = minIterations
         c.working64 = int(working64)
-        c.precond = {"none": 0, None: 0, "ilu": 1, "gs": 2}[precond]
+        c.precond = {"none": 0, None: 0, "ilu": 1, "gs": 2, "dilu": 3}[precond]
         c.factor64 = int(working64 if factor64 is None else factor64)
         c.gsSweeps = gsSweeps
         c.gsSymmetric = int(gsSymmetric)

@@ -499,8 +499,11 @@ ORC_API void orc_ldu_residual_mixed(int64_t n, const int64_t* lowerAddr,
 //     for ij > ik with j in-tile: a_ij -= a_ik * a(k, j)     (a(k,j) = 0 if absent)
 //     a_ii -= a_ik * a(k, i)
 // then diag := 1 / diag.
+// DILU (diagonalBased, :191-216): off-diagonals stay as in A, only
+//     a_ii -= (a_ik * a(k, i)) / a_kk
+// with a_kk the already updated diagonal of row k.
 // ---------------------------------------------------------------------------
-template <typename F>
+template <typename F, bool DILU = false>
 static void ilu0_factor(int64_t n, int32_t T, const int64_t* tileStart,
                         const int64_t* rowPtr, const int64_t* col, F* off,
                         F* diag) {
@@ -518,6 +521,11 @@ static void ilu0_factor(int64_t n, int32_t T, const int64_t* tileStart,
         int64_t k = col[ik];
         if (k >= i || k < t0) break;  // k >= i; halo columns sort last
         F a_kk = diag[k];
+        if (DILU) {
+          F a_ki = value(k, i);
+          diag[i] = diag[i] - off[ik] * a_ki / a_kk;
+          continue;
+        }
         off[ik] = off[ik] / a_kk;
         F a_ik = off[ik];
         for (int64_t ij = ik + 1; ij < iEnd; ij++) {
@@ -543,6 +551,19 @@ ORC_API void orc_ilu0_factor_f64(int64_t n, int32_t T, const int64_t* tileStart,
                          double* diag) {
   ilu0_factor<double>(n, T, tileStart, rowPtr, col, off, diag);
 }
+// DILU factorization: only diag (in place, inverted at the end) changes.
+ORC_API void orc_dilu_factor_f32(int64_t n, int32_t T, const int64_t* tileStart,
+                                 const int64_t* rowPtr, const int64_t* col,
+                                 const float* off, float* diag) {
+  ilu0_factor<float, true>(n, T, tileStart, rowPtr, col, const_cast<float*>(off),
+                           diag);
+}
+ORC_API void orc_dilu_factor_f64(int64_t n, int32_t T, const int64_t* tileStart,
+                                 const int64_t* rowPtr, const int64_t* col,
+                                 const double* off, double* diag) {
+  ilu0_factor<double, true>(n, T, tileStart, rowPtr, col, const_cast<double*>(off),
+                            diag);
+}
 
 // ILU(0) substitution per tile. lg/matrix/solver/ilu/Solver.cpp:380-442 (row
 // bodies) and :526-548 (sequential driver); codelet ilu-solve.cpp:30-104.
@@ -551,7 +572,9 @@ ORC_API void orc_ilu0_factor_f64(int64_t n, int32_t T, const int64_t* tileStart,
 // backward, i descending: t = 0; for a descending: break if j < i; skip halo;
 //                        t -= U_a * x_j;  x_i = (x_i + t) * invD_i
 // Arithmetic in the vector type X (factors promoted).
-template <typename F, typename X>
+// DILU (diagonalBased, :405-441): lu = A's off-diagonals, forward
+// x_i = t * invD_i, backward x_i = x_i + t * invD_i.
+template <typename F, typename X, bool DILU = false>
 static void ilu0_apply(int32_t T, const int64_t* tileStart,
                        const int64_t* rowPtr, const int64_t* col, const F* lu,
                        const F* invD, const X* b, X* x) {
@@ -565,7 +588,7 @@ static void ilu0_apply(int32_t T, const int64_t* tileStart,
         if (j < t0 || j >= t1) break;
         s = s - (X)lu[a] * x[j];
       }
-      x[i] = s;
+      x[i] = DILU ? s * (X)invD[i] : s;
     }
     for (int64_t i = t1 - 1; i >= t0; i--) {
       X s = 0;
@@ -577,7 +600,7 @@ static void ilu0_apply(int32_t T, const int64_t* tileStart,
         if (j < i) break;
         s = s - (X)lu[a] * x[j];
       }
-      x[i] = (x[i] + s) * (X)invD[i];
+      x[i] = DILU ? x[i] + s * (X)invD[i] : (x[i] + s) * (X)invD[i];
     }
   }
 }
@@ -592,6 +615,18 @@ ORC_API void orc_ilu0_apply_f64(int32_t T, const int64_t* tileStart,
                         const double* lu, const double* invD, const double* b,
                         double* x) {
   ilu0_apply<double, double>(T, tileStart, rowPtr, col, lu, invD, b, x);
+}
+ORC_API void orc_dilu_apply_f32(int32_t T, const int64_t* tileStart,
+                                const int64_t* rowPtr, const int64_t* col,
+                                const float* off, const float* invD,
+                                const float* b, float* x) {
+  ilu0_apply<float, float, true>(T, tileStart, rowPtr, col, off, invD, b, x);
+}
+ORC_API void orc_dilu_apply_f64(int32_t T, const int64_t* tileStart,
+                                const int64_t* rowPtr, const int64_t* col,
+                                const double* off, const double* invD,
+                                const double* b, double* x) {
+  ilu0_apply<double, double, true>(T, tileStart, rowPtr, col, off, invD, b, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -733,6 +768,9 @@ struct OrcSystem {
   bool hasF32 = false, hasF64 = false;
   std::vector<float> lu32, invD32;
   std::vector<double> lu64, invD64;
+  // lazily built DILU diagonals (off-diagonals are A's)
+  std::vector<float> diluInvD32;
+  std::vector<double> dilu64, diluInvD64;
 };
 
 ORC_API void* orc_system_create(int64_t n, int32_t T, const int64_t* tileStart,
@@ -757,7 +795,7 @@ struct OrcPBiCGStabCfg {
   float absTolerance, relTolerance, relResidual;
   int32_t maxIterations, minIterations;
   int32_t working64;      // 1: working type float64
-  int32_t precond;        // 0 none, 1 ILU0, 2 Gauss-Seidel
+  int32_t precond;        // 0 none, 1 ILU0, 2 Gauss-Seidel, 3 DILU
   int32_t factor64;       // ILU factors computed/stored in float64
   int32_t gsSweeps;       // numFixedIterations of the GS preconditioner
   int32_t gsSymmetric;    // forward + backward per sweep
@@ -787,6 +825,20 @@ static void ensure_factors(OrcSystem* s, bool f64) {
   (void)nnz;
 }
 
+static void ensure_dilu(OrcSystem* s, bool f64) {
+  if (f64 && s->diluInvD64.empty()) {
+    s->dilu64.assign(s->off.begin(), s->off.end());
+    s->diluInvD64.assign(s->diag.begin(), s->diag.end());
+    ilu0_factor<double, true>(s->n, s->T, s->tileStart.data(), s->rowPtr.data(),
+                              s->col.data(), s->dilu64.data(), s->diluInvD64.data());
+  }
+  if (!f64 && s->diluInvD32.empty()) {
+    s->diluInvD32.assign(s->diag.begin(), s->diag.end());
+    ilu0_factor<float, true>(s->n, s->T, s->tileStart.data(), s->rowPtr.data(),
+                             s->col.data(), s->off.data(), s->diluInvD32.data());
+  }
+}
+
 template <typename X>
 static void precond_apply(OrcSystem* s, const OrcPBiCGStabCfg* c, const X* b,
                           X* y) {
@@ -804,6 +856,18 @@ static void precond_apply(OrcSystem* s, const OrcPBiCGStabCfg* c, const X* b,
       ilu0_apply<float, X>(s->T, s->tileStart.data(), s->rowPtr.data(),
                            s->col.data(), s->lu32.data(), s->invD32.data(), b,
                            y);
+    }
+  } else if (c->precond == 3) {
+    if (c->factor64) {
+      ensure_dilu(s, true);
+      ilu0_apply<double, X, true>(s->T, s->tileStart.data(), s->rowPtr.data(),
+                                  s->col.data(), s->dilu64.data(),
+                                  s->diluInvD64.data(), b, y);
+    } else {
+      ensure_dilu(s, false);
+      ilu0_apply<float, X, true>(s->T, s->tileStart.data(), s->rowPtr.data(),
+                                 s->col.data(), s->off.data(),
+                                 s->diluInvD32.data(), b, y);
     }
   } else {
     // Gauss-Seidel uses the initial guess: y = 0 (pbicgstab/Solver.cpp:107-109)

@@ -113,6 +113,35 @@ def test_ilu0_apply_bit_exact(case, ftype):
     assert np.array_equal(x.numpy(), ref), name
 
 
+@pytest.mark.parametrize("ftype", ["float32", "float64"])
+def test_dilu_apply_bit_exact(case, ftype):
+    """DILU (diagonalBased) factorization + substitution equal the oracle's."""
+    name, h, L, A = case
+    dt = np.float32 if ftype == "float32" else np.float64
+    S = g.Solver(A, {"type": "ILU", "factorType": ftype, "diagonalBased": True})
+    assert S.name() == "DILU"
+    b = o.uniform_pm1(27, h.nrows).astype(dt)
+    x = dev(np.zeros(h.nrows, dt))
+    S.solve(x, dev(b))
+    lu, d = o.ilu0_factor(L, dt, diagonalBased=True)
+    ref = o.ilu0_apply(L, lu, d, b, diagonalBased=True)
+    assert np.array_equal(x.numpy(), ref), name
+
+
+def test_ilu0_fp16_factors_bit_exact(case):
+    """FP16-stored factors: the FP32 factors rounded to FP16, applied in FP32."""
+    name, h, L, A = case
+    S = g.Solver(A, {"type": "ILU", "factorType": "float16"})
+    b = o.uniform_pm1(28, h.nrows).astype(np.float32)
+    x = dev(np.zeros(h.nrows, np.float32))
+    S.solve(x, dev(b))
+    lu, d = o.ilu0_factor(L, np.float32)
+    lu16 = lu.astype(np.float16).astype(np.float32)
+    d16 = d.astype(np.float16).astype(np.float32)
+    ref = o.ilu0_apply(L, lu16, d16, b)
+    assert np.array_equal(x.numpy(), ref), name
+
+
 @pytest.mark.parametrize("symmetric", [False, True])
 def test_gauss_seidel_bit_exact(case, symmetric):
     name, h, L, A = case
@@ -138,12 +167,16 @@ def _pbicgstab_cfg(rel, precond="ILU", working="float32", maxit=1000):
     elif precond == "GS":
         c["preconditioner"] = {"type": "GaussSeidel", "numFixedIterations": 1,
                                "symmetric": True}
+    elif precond == "DILU":
+        c["preconditioner"] = {"type": "ILU", "factorType": working,
+                               "diagonalBased": True}
     return c
 
 
 @pytest.mark.parametrize("precond,working", [("ILU", "float32"), ("ILU", "float64"),
                                               ("none", "float64"), ("GS", "float64"),
-                                              ("none", "float32"), ("GS", "float32")])
+                                              ("none", "float32"), ("GS", "float32"),
+                                              ("DILU", "float32"), ("DILU", "float64")])
 def test_pbicgstab_iterations_match_oracle(case, precond, working):
     """Iteration counts within 5% of the oracle (the reduction order of the
     dot products differs, so the iterates are not bit-identical). FP32 runs
@@ -157,11 +190,11 @@ def test_pbicgstab_iterations_match_oracle(case, precond, working):
     st = g.Solver(A, _pbicgstab_cfg(rel, precond, working)).solve(x, dev(b.astype(dt)))
     S = o.System(L)
     ocfg = S.pbicgstab_cfg(rel, 1000, precond={"ILU": "ilu", "none": "none",
-                                                 "GS": "gs"}[precond],
+                                                 "GS": "gs", "DILU": "dilu"}[precond],
                            working64=(working == "float64"), gsSweeps=1, gsSymmetric=True)
     xo, sto = S.pbicgstab(np.zeros(h.nrows, dt), b.astype(dt), ocfg)
     assert st.converged and sto.converged, (name, st)
-    if precond != "none" and (precond == "ILU" or working == "float64"):
+    if precond != "none" and (precond in ("ILU", "DILU") or working == "float64"):
         assert abs(st.iterations - sto.iterations) <= max(1, 0.05 * sto.iterations), \
             (name, st.iterations, sto.iterations)
     elif working == "float64":

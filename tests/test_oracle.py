@@ -247,6 +247,72 @@ def test_ilu0_apply_vs_triangular_solves():
     assert np.allclose(o.ilu0_apply(L, lu, invD, b), x, rtol=1e-12, atol=1e-13)
 
 
+def _dilu_dense_blocks(L, invD, tiled_off):
+    """Dense (D+L) D^-1 (D+U) restricted to each tile's block (FP64)."""
+    n = len(invD)
+    ts = L["tile_start"]
+    rows = np.repeat(np.arange(n), np.diff(L["row_ptr"]))
+    M = np.zeros((n, n))
+    for t in range(len(ts) - 1):
+        a, b = ts[t], ts[t + 1]
+        sel = (rows >= a) & (rows < b) & (L["col"] >= a) & (L["col"] < b)
+        r, c, v = rows[sel], L["col"][sel], tiled_off[sel].astype(np.float64)
+        D = np.diag(1.0 / invD[a:b])
+        Lo = np.zeros((b - a, b - a))
+        Up = np.zeros((b - a, b - a))
+        lo = c < r
+        Lo[r[lo] - a, c[lo] - a] = v[lo]
+        Up[r[~lo] - a, c[~lo] - a] = v[~lo]
+        M[a:b, a:b] = (D + Lo) @ np.diag(invD[a:b]) @ (D + Up)
+    return M
+
+
+@pytest.mark.parametrize("T,block", [(1, None), (8, (3, 3, 3))])
+def test_dilu_diagonal_identity_and_apply(T, block):
+    """DILU (ilu/Solver.cpp:191-216, :405-441): M = (D+L) D^-1 (D+U) with
+    diag(M) = diag(A) per tile block; the apply solves M y = b."""
+    import graphene_amd as g
+    m = 6
+    r, c, v = g.generateConvectionDiffusion27Triplets(m, m, m, 10.0, 3)
+    n = m ** 3
+    r2t = None
+    if block is not None:
+        r2t, T = g.partition_grid3d(m, m, m, *block)
+    L = oracle_layout(n, r, c, v, T, r2t)
+    lu, invD = o.ilu0_factor(L, np.float64, diagonalBased=True)
+    assert np.array_equal(lu, L["off"].astype(np.float64))  # off-diagonals untouched
+    M = _dilu_dense_blocks(L, invD, L["off"])
+    assert np.allclose(np.diag(M), L["diag"].astype(np.float64), rtol=1e-12, atol=1e-12)
+    b = o.uniform_pm1(9, n).astype(np.float64)
+    y = o.ilu0_apply(L, lu, invD, b, diagonalBased=True)
+    assert np.allclose(M @ y, b, rtol=1e-11, atol=1e-11)
+    # FP32 factors/apply stay close to the FP64 ones
+    lu32, invD32 = o.ilu0_factor(L, np.float32, diagonalBased=True)
+    assert np.allclose(invD32, invD, rtol=1e-5)
+    y32 = o.ilu0_apply(L, lu32, invD32, b.astype(np.float32), diagonalBased=True)
+    assert np.allclose(y32, y, rtol=1e-4, atol=1e-5)
+
+
+def test_dilu_dense_brute_force():
+    """Textbook DILU recursion d_i = a_ii - sum_{k<i} a_ik a_ki / d_k."""
+    rng = np.random.default_rng(11)
+    n = 14
+    M = (rng.random((n, n)) < 0.3) * rng.standard_normal((n, n))
+    np.fill_diagonal(M, 6 + rng.random(n))
+    pattern = M != 0
+    rows, cols = np.nonzero(pattern)
+    L = oracle_layout(n, rows.astype(np.int64), cols.astype(np.int64),
+                      M[rows, cols].astype(np.float32))
+    Mf = M.astype(np.float32).astype(np.float64)
+    d = np.diag(Mf).copy()
+    for i in range(n):
+        for k in range(i):
+            if pattern[i, k]:
+                d[i] -= Mf[i, k] * Mf[k, i] / d[k]
+    _, invD = o.ilu0_factor(L, np.float64, diagonalBased=True)
+    assert np.allclose(1.0 / invD, d, rtol=1e-13)
+
+
 # --- Gauss-Seidel ---------------------------------------------------------------
 def test_gs_dense_textbook_and_fixed_point():
     L = poisson_layout(5, 4, 3)
