@@ -33,13 +33,15 @@ static void packDirection(const host::HostMatrix& m,
   const int32_t G = (int32_t)groups.size() - 1;
   d.slabOff.assign((size_t)G + 1, 0);
   d.lvlOff.assign((size_t)G + 1, 0);
-  d.rowInfo.assign((size_t)m.nrows * 2, 0);
+  d.rowLoc.assign((size_t)m.nrows, 0);
   const std::vector<int32_t>& rows = backward ? m.upperLevelRows : m.colorSortAddr;
   const std::vector<int64_t>& starts = backward ? m.upperColorStart : m.colorStart;
   const std::vector<int64_t>& startOff =
       backward ? m.upperColorStartOffset : m.colorStartOffset;
   int64_t slab = 0;
-  std::vector<int32_t> ents;
+  std::vector<int64_t> lvRows;   // rows of the level (matrix ids)
+  std::vector<int32_t> lvEnts;   // their entries (CRS positions), row after row
+  std::vector<int64_t> lvOff;    // R+1 offsets into lvEnts
   for (int32_t gi = 0; gi < G; gi++) {
     const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
     const int64_t g0 = m.tileStart[(size_t)ta];
@@ -58,8 +60,9 @@ static void packDirection(const host::HostMatrix& m,
     int64_t groupSlab = 0;
     int64_t slot = 0;  // row slot within the group, level order
     for (int32_t L = 0; L < nL; L++) {
-      d.rowStart.push_back((int32_t)slot);
-      int64_t e = 0;  // entries of this level
+      lvRows.clear();
+      lvEnts.clear();
+      lvOff.assign(1, 0);
       for (int32_t t = ta; t < tb; t++) {
         const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
         const int64_t* ls = starts.data() + startOff[(size_t)t];
@@ -67,39 +70,48 @@ static void packDirection(const host::HostMatrix& m,
         if (L >= tL) continue;
         for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
           const int64_t i = t0 + rows[(size_t)(t0 + q)];
-          ents.clear();
           const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
           if (!backward) {
             for (int64_t a = a0; a < a1; a++) {
               const int64_t j = m.col[(size_t)a];
               if (i < j || j < t0 || j >= t1) break;
-              ents.push_back((int32_t)a);
+              lvEnts.push_back((int32_t)a);
             }
           } else {
             for (int64_t a = a1 - 1; a >= a0; a--) {
               const int64_t j = m.col[(size_t)a];
               if (j < t0 || j >= t1) continue;
               if (j < i) break;
-              ents.push_back((int32_t)a);
+              lvEnts.push_back((int32_t)a);
             }
           }
-          const int64_t len = ents.empty() ? 1 : (int64_t)ents.size();
-          const size_t rs = (size_t)(g0 + slot) * 2;
-          d.rowInfo[rs] = (uint32_t)(i - g0) | ((uint32_t)len << 16);
-          d.rowInfo[rs + 1] = (uint32_t)e;
-          slot++;
-          if (ents.empty()) {
-            d.col.push_back((uint16_t)ng);
-            d.src.push_back(-1);
-          } else {
-            for (int32_t a : ents) {
-              d.col.push_back((uint16_t)(m.col[(size_t)a] - g0));
-              d.src.push_back(a);
-            }
-          }
-          e += len;
+          lvRows.push_back(i);
+          lvOff.push_back((int64_t)lvEnts.size());
         }
       }
+      const int64_t R = (int64_t)lvRows.size();
+      int64_t W = 0;
+      for (int64_t r = 0; r < R; r++)
+        W = std::max<int64_t>(W, lvOff[(size_t)r + 1] - lvOff[(size_t)r]);
+      d.rowStart.push_back((int32_t)slot);
+      d.width.push_back((int32_t)W);
+      for (int64_t r = 0; r < R; r++)
+        d.rowLoc[(size_t)(g0 + slot + r)] = (uint16_t)(lvRows[(size_t)r] - g0);
+      slot += R;
+      // entry j of row r at position j*R + r; short rows padded with zero
+      // entries (value 0 at the zero slot ng), which subtract +0
+      for (int64_t j = 0; j < W; j++)
+        for (int64_t r = 0; r < R; r++) {
+          if (j < lvOff[(size_t)r + 1] - lvOff[(size_t)r]) {
+            const int32_t a = lvEnts[(size_t)(lvOff[(size_t)r] + j)];
+            d.col.push_back((uint16_t)(m.col[(size_t)a] - g0));
+            d.src.push_back(a);
+          } else {
+            d.col.push_back((uint16_t)ng);
+            d.src.push_back(-1);
+          }
+        }
+      const int64_t e = W * R;
       d.maxLevelEntries = std::max<int32_t>(d.maxLevelEntries, (int32_t)e);
       const int64_t ns = (e + kPackSlab - 1) / kPackSlab;
       for (int64_t p = e; p < ns * kPackSlab; p++) {  // pad the level
@@ -114,7 +126,8 @@ static void packDirection(const host::HostMatrix& m,
       return;
     }
     d.rowStart.push_back((int32_t)ng);
-    d.slabEnd.push_back(INT32_MAX);  // sentinel: both tables hold nL+1 entries
+    d.width.push_back(0);
+    d.slabEnd.push_back(INT32_MAX);  // sentinel: the tables hold nL+1 entries
     if (groupSlab % 2) {  // streams start on an even slab (pairSlabs)
       for (int64_t p = 0; p < kPackSlab; p++) {
         d.col.push_back((uint16_t)ng);

@@ -8,10 +8,11 @@
 // descending column order, as ilu/Solver.cpp:380-442 visits them), the
 // entries are grouped by level and stored level after level; every level
 // starts on a 64-entry "slab" boundary so one wave loads a slab with one
-// coalesced instruction. A row without entries in a direction gets one
-// zero entry that reads a zero slot, so every row has at least one entry
-// and the substitution keeps the reference's arithmetic exactly.
-// Per row: (local id, entry count, first entry within its level).
+// coalesced instruction. Within a level of R rows and width W (the longest
+// row), entry j of row r sits at j*R + r; shorter rows are padded with zero
+// entries that read a zero slot. Subtracting their +0 products leaves every
+// partial sum unchanged, so the substitution keeps the reference's
+// arithmetic exactly. Per row slot: the row's local id.
 #pragma once
 
 #include <cstdint>
@@ -31,7 +32,8 @@ struct PackedDirection {
   std::vector<int32_t> lvlOff;    // T+1: offset of each tile's level table
   std::vector<int32_t> slabEnd;   // per tile: numLevels (cumulative) + sentinel
   std::vector<int32_t> rowStart;  // per tile: numLevels+1 entries
-  std::vector<uint32_t> rowInfo;  // 2 words per row: local | len<<16, start
+  std::vector<int32_t> width;     // per tile: row width of each level (+ 0)
+  std::vector<uint16_t> rowLoc;   // per row slot (level order): local row id
   int32_t maxLevels = 0;
   int32_t maxLevelEntries = 0;
 };

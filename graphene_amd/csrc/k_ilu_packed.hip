@@ -13,6 +13,9 @@
 //   * per level, lane k multiplies entry k with the x value it references
 //     (LDS) into a product buffer, then one lane per row subtracts its
 //     products in stored order - exactly the reference's sequential row sum.
+//     The level's entries are stored column-major (entry j of row r at
+//     j*R + r, short rows padded with +0 products), so the row sums read the
+//     buffer with a uniform trip count and without bank conflicts.
 #include <climits>
 
 #include "kernels.hpp"
@@ -22,9 +25,12 @@ namespace gph {
 static constexpr int kDepth = 16;  // must equal kPackDepth of ilu_pack.hpp
 
 struct PackedLds {
-  size_t xs, invd, prod, rows, slabEnd, rowStart, total;
+  size_t xs, invd, prod, tab[2], rows[2], total;
 };
 
+// LDS: x (+ zero slot), 1/diag, level products, and per direction the level
+// tables (slab end, first row slot, width: 3 x (maxLevels+1) ints) and the
+// local id of every row slot (u16).
 __host__ __device__ inline PackedLds packedLayout(const PackedArgs& p,
                                                   size_t xsize) {
   PackedLds L;
@@ -35,13 +41,15 @@ __host__ __device__ inline PackedLds packedLayout(const PackedArgs& p,
   o += (size_t)p.maxRows * xsize;
   L.prod = o;
   o += (size_t)(p.maxLevelEntries + 64) * xsize;
-  o = (o + 7) & ~(size_t)7;
-  L.rows = o;
-  o += (size_t)p.maxRows * 8;
-  L.slabEnd = o;
-  o += (size_t)(p.maxLevels + 1) * 4;
-  L.rowStart = o;
-  o += (size_t)(p.maxLevels + 1) * 4;
+  o = (o + 3) & ~(size_t)3;
+  for (int d = 0; d < 2; d++) {
+    L.tab[d] = o;
+    o += (size_t)3 * (p.maxLevels + 1) * 4;
+  }
+  for (int d = 0; d < 2; d++) {
+    L.rows[d] = o;
+    o += (size_t)p.maxRows * 2;
+  }
   L.total = (o + 15) & ~(size_t)15;
   return L;
 }
@@ -50,22 +58,20 @@ size_t packedSharedBytes(const PackedArgs& p, DType xt) {
   return packedLayout(p, dtypeSize(xt)).total;
 }
 
+// Setup loads are issued kStage per lane before any of them is used, so a
+// tile's staging costs a few memory latencies rather than one per row block.
+static constexpr int kStage = 8;
+
 template <typename FT, typename X, bool BACKWARD>
 __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
-                                            int t0, int nt, X* xs,
-                                            const X* invd, X* prod,
-                                            uint2* rows, int* slabEnd,
-                                            int* rowStart, int dilu) {
+                                            int nL, X* xs, const X* invd,
+                                            X* prod, const uint16_t* rows,
+                                            const int* tab, int tabStride,
+                                            int dilu) {
   const int lane = threadIdx.x;
-  const int lo = d.lvlOff[tile];
-  const int nL = d.lvlOff[tile + 1] - lo - 1;
-  for (int q = lane; q <= nL; q += kPackedThreads) {
-    slabEnd[q] = d.slabEnd[lo + q];
-    rowStart[q] = d.rowStart[lo + q];
-  }
-  const uint2* ri = reinterpret_cast<const uint2*>(d.rowInfo);
-  for (int q = lane; q < nt; q += kPackedThreads) rows[q] = ri[t0 + q];
-  __syncthreads();
+  const int* slabEnd = tab;
+  const int* rowStart = tab + tabStride;
+  const int* width = tab + 2 * tabStride;
   const int S = nL > 0 ? slabEnd[nL - 1] : 0;
   const int Stot = ((S + kDepth - 1) / kDepth) * kDepth;
   // 16-bit slots are stored slab-pair interleaved (the stream starts on an
@@ -85,7 +91,7 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
   uint32_t cw[kDepth / 2] = {};
   uint32_t hw[kHalfVals ? kDepth / 2 : 1] = {};
   FT v[kHalfVals ? 1 : kDepth] = {};
-  int lvl = 0, lvlFirst = 0, lvlEnd = slabEnd[0];
+  int lvl = 0, lvlFirst = 0, lvlEnd = slabEnd[0], r0 = 0;
   for (int sb = -kDepth; sb < Stot; sb += kDepth) {
 #pragma unroll
     for (int u = 0; u < kDepth; u++) {
@@ -111,22 +117,23 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
       if constexpr (!kHalfVals) v[u] = vp[(size_t)(s + kDepth) * kPackedThreads];
       if (s + 1 == lvlEnd) {
         __syncthreads();
-        const int r1 = rowStart[lvl + 1];
-        for (int r = rowStart[lvl] + lane; r < r1; r += kPackedThreads) {
-          const uint2 info = rows[r];
-          const int loc = (int)(info.x & 0xffffu);
-          const int len = (int)(info.x >> 16);
-          const X* pp = prod + info.y;
+        // level tables of this level and the next, read together
+        const int r1 = rowStart[lvl + 1], W = width[lvl], nextEnd = slabEnd[lvl + 1];
+        const int R = r1 - r0;
+        for (int r = lane; r < R; r += kPackedThreads) {
+          const int loc = rows[r0 + r];
+          const X* pp = prod + r;
           X acc = BACKWARD ? (X)0 : xs[loc];
-          // issue the LDS reads of 8 products at once, then subtract in order
-          for (int a0 = 0; a0 < len; a0 += 8) {
+          int j = 0;
+          // issue 8 reads, then subtract in stored order
+          for (; j + 8 <= W; j += 8) {
             X pv[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) pv[j] = (a0 + j < len) ? pp[a0 + j] : (X)0;
+            for (int q = 0; q < 8; q++) pv[q] = pp[(j + q) * R];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-              if (a0 + j < len) acc = acc - pv[j];
+            for (int q = 0; q < 8; q++) acc = acc - pv[q];
           }
+          for (; j < W; j++) acc = acc - pp[j * R];
           if (BACKWARD)
             xs[loc] = dilu ? xs[loc] + acc * invd[loc] : (xs[loc] + acc) * invd[loc];
           else
@@ -134,13 +141,57 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
         }
         __syncthreads();
         lvl++;
+        r0 = r1;
         lvlFirst = lvlEnd;
-        lvlEnd = slabEnd[lvl];
+        lvlEnd = nextEnd;
       }
     }
   }
   __syncthreads();
 }
+
+// Stages a tile's rows into LDS: right-hand side value f(i) (written back
+// to `wb` when given: the fused p- or s-update), 1/diag, row slots.
+template <typename FT, typename X>
+struct PackedStage {
+  int t0, nt;
+  const FT* invD;
+  const uint16_t* locF;
+  const uint16_t* locB;
+  X* xs;
+  X* invd;
+  uint16_t* rowsF;
+  uint16_t* rowsB;
+  template <typename F>
+  __device__ __forceinline__ void run(F f, X* wb) const {
+    const int lane = threadIdx.x;
+    for (int q0 = 0; q0 < nt; q0 += kStage * kPackedThreads) {
+      X a[kStage];
+      FT dv[kStage];
+      uint16_t rf[kStage], rb[kStage];
+#pragma unroll
+      for (int k = 0; k < kStage; k++) {
+        const int q = q0 + k * kPackedThreads + lane;
+        const int i = t0 + (q < nt ? q : nt - 1);  // clamped: loads unconditional
+        a[k] = f(i);
+        dv[k] = invD[i];
+        rf[k] = locF[i];
+        rb[k] = locB[i];
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; k++) {
+        const int q = q0 + k * kPackedThreads + lane;
+        if (q < nt) {
+          if (wb) wb[t0 + q] = a[k];
+          xs[q] = a[k];
+          invd[q] = cvt<X>(dv[k]);
+          rowsF[q] = rf[k];
+          rowsB[q] = rb[k];
+        }
+      }
+    }
+  }
+};
 
 template <typename FT, typename X>
 __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
@@ -152,49 +203,74 @@ __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
   X* xs = reinterpret_cast<X*>(smem + L.xs);
   X* invd = reinterpret_cast<X*>(smem + L.invd);
   X* prod = reinterpret_cast<X*>(smem + L.prod);
-  uint2* rows = reinterpret_cast<uint2*>(smem + L.rows);
-  int* slabEnd = reinterpret_cast<int*>(smem + L.slabEnd);
-  int* rowStart = reinterpret_cast<int*>(smem + L.rowStart);
+  int* tabF = reinterpret_cast<int*>(smem + L.tab[0]);
+  int* tabB = reinterpret_cast<int*>(smem + L.tab[1]);
+  uint16_t* rowsF = reinterpret_cast<uint16_t*>(smem + L.rows[0]);
+  uint16_t* rowsB = reinterpret_cast<uint16_t*>(smem + L.rows[1]);
+  const int ts = p.maxLevels + 1;  // table stride
   const int tile = blockIdx.x, lane = threadIdx.x;
   const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
+  const int loF = p.fwd.lvlOff[tile], nLF = p.fwd.lvlOff[tile + 1] - loF - 1;
+  const int loB = p.bwd.lvlOff[tile], nLB = p.bwd.lvlOff[tile + 1] - loB - 1;
 
-  // right-hand side (with the fused PBiCGStab update) and 1/diag into LDS
-  X beta = 0, omega = 0, alpha = 0;
-  bool pupd = false;
-  if (pre.kind == kPreP) {
-    pupd = st->iter != 0;
-    beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
-    omega = (X)st->omega;
-  } else if (pre.kind == kPreS) {
-    alpha = (X)st->alpha;
+  // level tables of both directions
+  const int nTab = (nLF > nLB ? nLF : nLB) + 1;
+  for (int q0 = 0; q0 < nTab; q0 += kStage * kPackedThreads) {
+    int a[kStage][6];
+    // loads are unconditional (clamped index) so they issue back to back
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const int q = q0 + k * kPackedThreads + lane;
+      const int qf = loF + (q < nLF ? q : nLF), qb = loB + (q < nLB ? q : nLB);
+      a[k][0] = p.fwd.slabEnd[qf];
+      a[k][1] = p.fwd.rowStart[qf];
+      a[k][2] = p.fwd.width[qf];
+      a[k][3] = p.bwd.slabEnd[qb];
+      a[k][4] = p.bwd.rowStart[qb];
+      a[k][5] = p.bwd.width[qb];
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const int q = q0 + k * kPackedThreads + lane;
+      if (q <= nLF) {
+        tabF[q] = a[k][0];
+        tabF[ts + q] = a[k][1];
+        tabF[2 * ts + q] = a[k][2];
+      }
+      if (q <= nLB) {
+        tabB[q] = a[k][3];
+        tabB[ts + q] = a[k][4];
+        tabB[2 * ts + q] = a[k][5];
+      }
+    }
   }
+
+  // right-hand side (with the fused PBiCGStab update), 1/diag and the row
+  // slots of both directions
   const X* r = static_cast<const X*>(pre.r);
   X* pv = static_cast<X*>(pre.p);
   const X* vv = static_cast<const X*>(pre.v);
-  for (int q = lane; q < nt; q += kPackedThreads) {
-    const int i = t0 + q;
-    X s;
-    if (pre.kind == kPreP) {
-      s = pv[i];
-      if (pupd) {
-        s = r[i] + beta * (s - omega * vv[i]);
-        pv[i] = s;
-      }
-    } else if (pre.kind == kPreS) {
-      s = r[i] - alpha * vv[i];
-      pv[i] = s;
+  const PackedStage<FT, X> stg{t0, nt, invD, p.fwd.rowLoc, p.bwd.rowLoc, xs, invd, rowsF, rowsB};
+  if (pre.kind == kPreP) {
+    if (st->iter != 0) {
+      const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+      const X omega = (X)st->omega;
+      stg.run([&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); }, pv);
     } else {
-      s = rhs[i];
+      stg.run([&](int i) { return pv[i]; }, (X*)nullptr);
     }
-    xs[q] = s;
-    invd[q] = cvt<X>(invD[i]);
+  } else if (pre.kind == kPreS) {
+    const X alpha = (X)st->alpha;
+    stg.run([&](int i) { return r[i] - alpha * vv[i]; }, pv);
+  } else {
+    stg.run([&](int i) { return rhs[i]; }, (X*)nullptr);
   }
   if (lane == 0) xs[nt] = (X)0;
   __syncthreads();
-  packedSweep<FT, X, false>(p.fwd, tile, t0, nt, xs, invd, prod, rows, slabEnd,
-                            rowStart, p.dilu);
-  packedSweep<FT, X, true>(p.bwd, tile, t0, nt, xs, invd, prod, rows, slabEnd,
-                           rowStart, p.dilu);
+  packedSweep<FT, X, false>(p.fwd, tile, nLF, xs, invd, prod, rowsF, tabF, ts,
+                            p.dilu);
+  packedSweep<FT, X, true>(p.bwd, tile, nLB, xs, invd, prod, rowsB, tabB, ts,
+                           p.dilu);
   for (int q = lane; q < nt; q += kPackedThreads) out[t0 + q] = xs[q];
 }
 

@@ -141,7 +141,8 @@ struct PackedDirArgs {
   const int32_t* lvlOff = nullptr;   // T+1
   const int32_t* slabEnd = nullptr;  // per tile nL+1 (last = sentinel)
   const int32_t* rowStart = nullptr; // per tile nL+1
-  const uint32_t* rowInfo = nullptr; // 2 words per row, tile-major
+  const int32_t* width = nullptr;    // per tile nL+1: entries per row slot
+  const uint16_t* rowLoc = nullptr;  // per row slot: local row id
 };
 struct PackedArgs {
   int32_t T = 0;

@@ -150,7 +150,7 @@ class IluSolver : public Solver {
     const void* src = ft_ == DType::F16 ? lu16_.get() : lu_.get();
     const size_t es = gph::dtypeSize(ft_);
     auto dir = [&](const PackedDirection& d, DevBuf& col, DevBuf& val, DevBuf& so,
-                   DevBuf& lo, DevBuf& se, DevBuf& rs, DevBuf& ri,
+                   DevBuf& lo, DevBuf& se, DevBuf& rs, DevBuf& ri, DevBuf& wd,
                    gph::PackedDirArgs& a) {
       if ((d.col.size() / kPackSlab) % 2 != 0)
         fail(gph::kRuntime, "packed ILU stream has an odd number of slabs");
@@ -167,17 +167,19 @@ class IluSolver : public Solver {
       lo.upload(d.lvlOff.data(), d.lvlOff.size(), s);
       se.upload(d.slabEnd.data(), d.slabEnd.size(), s);
       rs.upload(d.rowStart.data(), d.rowStart.size(), s);
-      ri.upload(d.rowInfo.data(), d.rowInfo.size(), s);
+      ri.upload(d.rowLoc.data(), d.rowLoc.size(), s);
+      wd.upload(d.width.data(), d.width.size(), s);
       a.col = col.as<uint16_t>();
       a.val = val.get();
       a.slabOff = so.as<int32_t>();
       a.lvlOff = lo.as<int32_t>();
       a.slabEnd = se.as<int32_t>();
       a.rowStart = rs.as<int32_t>();
-      a.rowInfo = ri.as<uint32_t>();
+      a.rowLoc = ri.as<uint16_t>();
+      a.width = wd.as<int32_t>();
     };
-    dir(P.fwd, fcol_, fval_, fso_, flo_, fse_, frs_, fri_, pargs_.fwd);
-    dir(P.bwd, bcol_, bval_, bso_, blo_, bse_, brs_, bri_, pargs_.bwd);
+    dir(P.fwd, fcol_, fval_, fso_, flo_, fse_, frs_, fri_, fwd_, pargs_.fwd);
+    dir(P.bwd, bcol_, bval_, bso_, blo_, bse_, brs_, bri_, bwd_, pargs_.bwd);
     pargs_.T = (int32_t)P.groupStart.size() - 1;
     pargs_.tileStart = groupStart_.as<int32_t>();
     pargs_.maxRows = P.maxRows;
@@ -194,8 +196,8 @@ class IluSolver : public Solver {
   DevBuf lu_, invD_, lu16_, invD16_;
   bool packed_ = false;
   gph::PackedArgs pargs_;
-  DevBuf fcol_, fval_, fso_, flo_, fse_, frs_, fri_;
-  DevBuf bcol_, bval_, bso_, blo_, bse_, brs_, bri_;
+  DevBuf fcol_, fval_, fso_, flo_, fse_, frs_, fri_, fwd_;
+  DevBuf bcol_, bval_, bso_, blo_, bse_, brs_, bri_, bwd_;
 };
 
 // ---------------------------------------------------------------------------
