@@ -29,8 +29,9 @@ struct PackedLds {
 };
 
 // LDS: x (+ zero slot), 1/diag, level products, and per direction the level
-// tables (slab end, first row slot, width: 3 x (maxLevels+1) ints) and the
-// local id of every row slot (u16).
+// tables (slab end, first row slot, width: 3 x (maxLevels+2) ints, the last
+// entry a spare that read-ahead may touch) and the local id of every row
+// slot (u16).
 __host__ __device__ inline PackedLds packedLayout(const PackedArgs& p,
                                                   size_t xsize) {
   PackedLds L;
@@ -44,7 +45,7 @@ __host__ __device__ inline PackedLds packedLayout(const PackedArgs& p,
   o = (o + 3) & ~(size_t)3;
   for (int d = 0; d < 2; d++) {
     L.tab[d] = o;
-    o += (size_t)3 * (p.maxLevels + 1) * 4;
+    o += (size_t)3 * (p.maxLevels + 2) * 4;
   }
   for (int d = 0; d < 2; d++) {
     L.rows[d] = o;
@@ -91,8 +92,20 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
   uint32_t cw[kDepth / 2] = {};
   uint32_t hw[kHalfVals ? kDepth / 2 : 1] = {};
   FT v[kHalfVals ? 1 : kDepth] = {};
-  int lvl = 0, lvlFirst = 0, lvlEnd = slabEnd[0], r0 = 0;
+
+  // The level's tables, and for its first 64 rows the local id, the current
+  // x value and 1/diag, are read one level ahead so that the row sums only
+  // wait for the products. Row r0 + lane of a level is held by lane `lane`.
+  int lvl = 0, lvlFirst = 0, r0 = 0;
+  int lvlEnd = slabEnd[0], r1 = rowStart[1], W = width[0];
+  const int ntm = rowStart[nL] > 0 ? rowStart[nL] - 1 : 0;  // last row slot (index clamp)
+  int pLoc = rows[lane < ntm ? lane : ntm];
+  X pX = xs[pLoc], pD = invd[pLoc];
   for (int sb = -kDepth; sb < Stot; sb += kDepth) {
+    // refill sources of this pass: slab sb + kDepth + u at constant offsets
+    const uint32_t* __restrict__ cpn = cp + (size_t)((sb + kDepth) >> 1) * kPackedThreads;
+    const uint32_t* __restrict__ hpn = hp + (size_t)((sb + kDepth) >> 1) * kPackedThreads;
+    const FT* __restrict__ vpn = vp + (size_t)(sb + kDepth) * kPackedThreads;
 #pragma unroll
     for (int u = 0; u < kDepth; u++) {
       const int s = sb + u;
@@ -110,20 +123,17 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
       const int pi = (s >= 0 && s < S) ? (s - lvlFirst) * kPackedThreads + lane : lane;
       prod[pi] = a * xs[c];
       if (u & 1) {  // both slots of the word consumed: refill it
-        const size_t nw = (size_t)((s + kDepth) >> 1) * kPackedThreads;
-        cw[u >> 1] = cp[nw];
-        if constexpr (kHalfVals) hw[u >> 1] = hp[nw];
+        cw[u >> 1] = cpn[(u >> 1) * kPackedThreads];
+        if constexpr (kHalfVals) hw[u >> 1] = hpn[(u >> 1) * kPackedThreads];
       }
-      if constexpr (!kHalfVals) v[u] = vp[(size_t)(s + kDepth) * kPackedThreads];
+      if constexpr (!kHalfVals) v[u] = vpn[u * kPackedThreads];
       if (s + 1 == lvlEnd) {
         __syncthreads();
-        // level tables of this level and the next, read together
-        const int r1 = rowStart[lvl + 1], W = width[lvl], nextEnd = slabEnd[lvl + 1];
         const int R = r1 - r0;
-        for (int r = lane; r < R; r += kPackedThreads) {
-          const int loc = rows[r0 + r];
-          const X* pp = prod + r;
-          X acc = BACKWARD ? (X)0 : xs[loc];
+        // rows r0 .. r0+63 from the prefetched registers
+        if (lane < R) {
+          const X* pp = prod + lane;
+          X acc = BACKWARD ? (X)0 : pX;
           int j = 0;
           // issue 8 reads, then subtract in stored order
           for (; j + 8 <= W; j += 8) {
@@ -135,15 +145,33 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
           }
           for (; j < W; j++) acc = acc - pp[j * R];
           if (BACKWARD)
-            xs[loc] = dilu ? xs[loc] + acc * invd[loc] : (xs[loc] + acc) * invd[loc];
+            xs[pLoc] = dilu ? pX + acc * pD : (pX + acc) * pD;
           else
-            xs[loc] = dilu ? acc * invd[loc] : acc;
+            xs[pLoc] = dilu ? acc * pD : acc;
+        }
+        // further rows of wide levels
+        for (int r = lane + kPackedThreads; r < R; r += kPackedThreads) {
+          const int loc = rows[r0 + r];
+          const X* pp = prod + r;
+          const X x0 = xs[loc], d0 = invd[loc];
+          X acc = BACKWARD ? (X)0 : x0;
+          for (int j = 0; j < W; j++) acc = acc - pp[j * R];
+          if (BACKWARD)
+            xs[loc] = dilu ? x0 + acc * d0 : (x0 + acc) * d0;
+          else
+            xs[loc] = dilu ? acc * d0 : acc;
         }
         __syncthreads();
+        // next level: tables and the first 64 rows' values
         lvl++;
         r0 = r1;
         lvlFirst = lvlEnd;
-        lvlEnd = nextEnd;
+        lvlEnd = slabEnd[lvl];
+        r1 = rowStart[lvl + 1];
+        W = width[lvl];
+        pLoc = rows[r0 + lane < ntm ? r0 + lane : ntm];
+        pX = xs[pLoc];
+        pD = invd[pLoc];
       }
     }
   }
@@ -207,7 +235,7 @@ __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
   int* tabB = reinterpret_cast<int*>(smem + L.tab[1]);
   uint16_t* rowsF = reinterpret_cast<uint16_t*>(smem + L.rows[0]);
   uint16_t* rowsB = reinterpret_cast<uint16_t*>(smem + L.rows[1]);
-  const int ts = p.maxLevels + 1;  // table stride
+  const int ts = p.maxLevels + 2;  // table stride (one spare entry)
   const int tile = blockIdx.x, lane = threadIdx.x;
   const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
   const int loF = p.fwd.lvlOff[tile], nLF = p.fwd.lvlOff[tile + 1] - loF - 1;
