@@ -41,7 +41,11 @@ __host__ __device__ inline PackedLds packedLayout(const PackedArgs& p,
   L.invd = o;
   o += (size_t)p.maxRows * xsize;
   L.prod = o;
-  o += (size_t)(p.maxLevelEntries + 64) * xsize;
+  // a level's products, or the kDepth*64 entries written by passes outside
+  // the levels (ring fill, ring overrun)
+  const int prodEntries =
+      p.maxLevelEntries + 64 > kDepth * 64 ? p.maxLevelEntries + 64 : kDepth * 64;
+  o += (size_t)prodEntries * xsize;
   o = (o + 3) & ~(size_t)3;
   for (int d = 0; d < 2; d++) {
     L.tab[d] = o;
@@ -92,36 +96,43 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
   uint32_t cw[kDepth / 2] = {};
   uint32_t hw[kHalfVals ? kDepth / 2 : 1] = {};
   FT v[kHalfVals ? 1 : kDepth] = {};
+  // refill sources: slab sb + kDepth + u of pass sb (advanced per pass)
+  const uint32_t* __restrict__ cpn = cp;
+  const uint32_t* __restrict__ hpn = hp;
+  const FT* __restrict__ vpn = vp;
 
   // The level's tables, and for its first 64 rows the local id, the current
   // x value and 1/diag, are read one level ahead so that the row sums only
   // wait for the products. Row r0 + lane of a level is held by lane `lane`.
-  int lvl = 0, lvlFirst = 0, r0 = 0;
+  // Products of slab s go to prod[(s - lvlFirst)*64 + lane]; the slabs of
+  // the first pass (s < 0, lvlFirst = -kDepth) and those past the last level
+  // (lvlFirst = S) land in the first kDepth*64 entries, which no row sum of
+  // theirs reads.
+  int lvl = 0, lvlFirst = -kDepth, r0 = 0;
   int lvlEnd = slabEnd[0], r1 = rowStart[1], W = width[0];
   const int ntm = rowStart[nL] > 0 ? rowStart[nL] - 1 : 0;  // last row slot (index clamp)
   int pLoc = rows[lane < ntm ? lane : ntm];
   X pX = xs[pLoc], pD = invd[pLoc];
+  X xNext = xs[0];  // x at the column of the next slot, read one slot ahead
   for (int sb = -kDepth; sb < Stot; sb += kDepth) {
-    // refill sources of this pass: slab sb + kDepth + u at constant offsets
-    const uint32_t* __restrict__ cpn = cp + (size_t)((sb + kDepth) >> 1) * kPackedThreads;
-    const uint32_t* __restrict__ hpn = hp + (size_t)((sb + kDepth) >> 1) * kPackedThreads;
-    const FT* __restrict__ vpn = vp + (size_t)(sb + kDepth) * kPackedThreads;
+    int pb = (sb - lvlFirst) * kPackedThreads + lane;  // pb + u*64 >= 0
 #pragma unroll
     for (int u = 0; u < kDepth; u++) {
       const int s = sb + u;
-      const uint32_t c = (u & 1) ? cw[u >> 1] >> 16 : cw[u >> 1] & 0xffffu;
       X a;
       if constexpr (kHalfVals)
         a = cvt<X>(__ushort_as_half(
             (unsigned short)((u & 1) ? hw[u >> 1] >> 16 : hw[u >> 1] & 0xffffu)));
       else
         a = cvt<X>(v[u]);
-      // slabs before 0 (empty ring) and past S (ring overrun into the next
-      // group's stream or the padding, after the last level's row sums)
-      // write their product to a dead slot; every ring word is consumed on
-      // every path, so no refill has to wait for an unused load
-      const int pi = (s >= 0 && s < S) ? (s - lvlFirst) * kPackedThreads + lane : lane;
-      prod[pi] = a * xs[c];
+      const X xu = xNext;
+      // column of slot u+1 (the next pass's slot 0 sits in cw[0], refilled
+      // at slot 1 of this pass); its x is read before this slot's product
+      // is stored, and read again if a level ends in between
+      const int un = (u + 1) % kDepth;  // constant after unrolling
+      const uint32_t cn = (un & 1) ? cw[un >> 1] >> 16 : cw[un >> 1] & 0xffffu;
+      xNext = xs[cn];
+      prod[pb + u * kPackedThreads] = a * xu;
       if (u & 1) {  // both slots of the word consumed: refill it
         cw[u >> 1] = cpn[(u >> 1) * kPackedThreads];
         if constexpr (kHalfVals) hw[u >> 1] = hpn[(u >> 1) * kPackedThreads];
@@ -172,8 +183,13 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
         pLoc = rows[r0 + lane < ntm ? r0 + lane : ntm];
         pX = xs[pLoc];
         pD = invd[pLoc];
+        pb = (sb - lvlFirst) * kPackedThreads + lane;
+        xNext = xs[cn];
       }
     }
+    cpn += (kDepth / 2) * kPackedThreads;
+    hpn += (kDepth / 2) * kPackedThreads;
+    vpn += kDepth * kPackedThreads;
   }
   __syncthreads();
 }
