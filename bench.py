@@ -23,6 +23,12 @@ HBM traffic per launch comes from the FETCH_SIZE / WRITE_SIZE counter summary
 under profiles/ (scaled by the same counters on a plain copy kernel), null when
 that summary is absent.
 
+Also in the line: "powerlaw" = the unstructured case (4,194,304 power-law
+rows: CRS SpMV throughput, PBiCGStab + symmetric Gauss-Seidel (2 sweeps) to
+1e-8) and "serving" = continuous batching of MPIR solve requests over a pool
+of resident systems at 50, 200 and 800 requests/s (p50/p99 latency,
+solves/s).
+
 usage: python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--n 128]
        python bench.py --kernels-only   # SpMV, ILU apply and copy kernels only
 """
@@ -83,6 +89,101 @@ def copy_calibration(g, n=1 << 28, iters=5):
             "launches": iters, "host_ms_per_launch": dt * 1e3}
 
 
+def bench_powerlaw(g, n=4194304, tiles=4096):
+    """Unstructured SpMV + solve: power-law rows (lengths 5..64, exponent
+    1.85, 90 % of columns within +-2000), seed 2, contiguous tiles; CRS SpMV
+    throughput, then PBiCGStab with symmetric Gauss-Seidel (2 sweeps) to
+    relative residual 1e-8 in FP32."""
+    t0 = time.time()
+    h = g.generatePowerLawMatrix(n, seed=2, numTiles=tiles)
+    A = h.copyToTile(g.CRS)
+    log(f"power-law {n} rows, {h.nnz_off} off-diagonals, {time.time()-t0:.1f}s")
+    x = g.DeviceVector.from_numpy(g.uniform_pm1(5, n).astype(np.float32))
+    y = g.DeviceVector(n, np.float32)
+    ms = A.bench_spmv(x, y, iters=30)
+    byt = 8 * h.nnz_off + 16 * n + 4
+    xt = h.to_local(g.uniform_pm1(2, n)).astype(np.float64)
+    b = g.DeviceVector.from_numpy(A.spmv(g.DeviceVector.from_numpy(xt)).numpy().astype(np.float32))
+    bnorm = float(np.linalg.norm(b.numpy().astype(np.float64)))
+    cfg = {"type": "PBiCGStab", "relResidual": 1e-8, "maxIterations": 2000,
+           "workingType": "float32",
+           "preconditioner": {"type": "GaussSeidel", "numFixedIterations": 2,
+                              "symmetric": True}}
+    S = g.Solver(A, cfg)
+    xs = g.DeviceVector(n, np.float32)
+    times = []
+    for _ in range(2):
+        xs.fill(0)
+        t = time.perf_counter()
+        st = S.solve(xs, b)
+        times.append(time.perf_counter() - t)
+    r = A.residual(g.DeviceVector.from_numpy(xs.numpy().astype(np.float64)), b,
+                   destType=g.FLOAT64, intermediateType=g.FLOAT64)
+    rel = float(np.linalg.norm(r.numpy())) / bnorm
+    out = {"rows": n, "nnz_off": h.nnz_off, "tiles": tiles,
+           "spmv_us": ms * 1e3, "spmv_GBs": byt / ms / 1e6,
+           "spmv_frac": byt / ms / 1e6 / HBM_PEAK_GBS, "spmv_algorithmic_bytes": byt,
+           "solve_ms": 1e3 * min(times), "iterations": st.iterations,
+           "converged": st.converged, "true_rel_residual": rel}
+    log(f"power-law SpMV {ms*1e3:.1f} us ({out['spmv_GBs']:.0f} GB/s), "
+        f"PBiCGStab+SGS(2): {out['solve_ms']:.1f} ms, {st}, true rel. residual {rel:.2e}")
+    return out
+
+
+def bench_serving(g, n_req, rates=(50, 200, 800), workers=8, budget=64 << 30):
+    """Continuous batching: a pool of resident systems mixing the Poisson-7,
+    27-point convection-diffusion and power-law generators with sizes
+    log-uniform in [1e4, 2e6] rows; Poisson arrivals at each rate; 1-8
+    right-hand sides per request; MPIR (FP64 outer, FP32 PBiCGStab+ILU(0)
+    inner) to relative residual 1e-10. Latency = finish - arrival."""
+    rng = np.random.default_rng(4)
+    t0 = time.time()
+    pool, keep = [], []
+    for k in range(9):
+        rows = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        kind = k % 3
+        if kind < 2:
+            m = max(8, int(round(rows ** (1 / 3))))
+            r2t, T = g.partition_grid3d(m, m, m, 16, 8, 8)
+            h = (g.generate3DPoissonMatrix(m, m, m, numTiles=T, rowToTile=r2t) if kind == 0
+                 else g.generateConvectionDiffusion27Matrix(m, m, m, 10.0, 10 + k,
+                                                            numTiles=T, rowToTile=r2t))
+            A = h.copyToTile(g.LDU if kind == 1 else g.CRS)
+        else:
+            nr = int(rows)
+            h = g.generatePowerLawMatrix(nr, seed=10 + k, numTiles=max(1, nr // 1024))
+            A = h.copyToTile(g.CRS)
+        keep.append(A)
+        pool.append((["poisson7", "convdiff27", "powerlaw"][kind], h.nrows))
+    server = g.Server(workers, budget)
+    cfg = mpir_cfg(1e-3, 100)
+    ids = [server.add_system(A, cfg) for A in keep]
+    log(f"serving pool: {pool}, setup {time.time()-t0:.1f}s")
+    out = {"systems": [{"kind": k, "rows": r} for k, r in pool], "workers": workers,
+           "hbm_budget_bytes": budget, "requests_per_rate": n_req, "rates": []}
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, n_req)
+        arrival = np.cumsum(gaps) - gaps[0]
+        sysv = rng.integers(0, len(ids), n_req)
+        rhs = rng.integers(1, 9, n_req)
+        seeds = rng.integers(1, 1 << 40, n_req)
+        res, _ = server.run([ids[i] for i in sysv], rhs, arrival, seeds)
+        lat = res["latency"]
+        makespan = float(np.max(arrival + lat))
+        rec = {"rate_req_s": rate, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+               "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+               "mean_wait_ms": 1e3 * float(np.mean(res["wait"])),
+               "mean_service_ms": 1e3 * float(np.mean(res["service"])),
+               "solves_per_s": float(np.sum(rhs)) / makespan,
+               "requests_per_s": n_req / makespan,
+               "all_converged": bool(np.all(res["converged"] == 1)),
+               "max_rel_residual": float(np.max(res["rel_residual"]))}
+        out["rates"].append(rec)
+        log(f"serving at {rate} req/s: p50 {rec['p50_ms']:.1f} ms, p99 {rec['p99_ms']:.1f} ms, "
+            f"{rec['solves_per_s']:.1f} solves/s, max rel. residual {rec['max_rel_residual']:.1e}")
+    return out
+
+
 def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=1024):
     return {
         "type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 50,
@@ -115,6 +216,9 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="SpMV, ILU apply and copy-calibration kernels only")
+    ap.add_argument("--skip-powerlaw", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -258,6 +362,9 @@ def main():
                          f"MPIR solve"}
         log(f"CPU oracle: {per_it:.2f} s/iteration -> {est:.1f} s per solve (estimate)")
 
+    powerlaw = None if args.skip_powerlaw else bench_powerlaw(g)
+    serving = None if args.skip_serve else bench_serving(g, args.serve_requests)
+
     # the kernel with the larger per-launch time (both run twice per inner
     # iteration) is the dominant one
     dominant, other = (rl_ilu, rl_spmv) if ilu_ms >= sp["ms"] else (rl_spmv, rl_ilu)
@@ -277,6 +384,8 @@ def main():
         "roofline_other": other,
         "spmv": spmv,
         "ilu_apply": ilu,
+        "powerlaw": powerlaw,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     out.update(extra)

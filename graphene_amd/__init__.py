@@ -52,6 +52,12 @@ class _Stats(C.Structure):
                 ("seconds", C.c_double), ("name", C.c_char * 96)]
 
 
+class _ReqResult(C.Structure):
+    _fields_ = [("latency", C.c_double), ("wait", C.c_double), ("service", C.c_double),
+                ("iterations", C.c_int64), ("inner_iterations", C.c_int64),
+                ("rel_residual", C.c_double), ("converged", C.c_int32), ("pad", C.c_int32)]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -100,6 +106,11 @@ _SIGS = {
     "gph_solver_name": ([_P, C.c_char_p, C.c_size_t], C.c_int),
     "gph_solver_uses_initial_guess": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "gph_config_check": ([C.c_char_p, C.c_char_p, C.c_size_t], C.c_int),
+    "gph_server_create": ([C.c_int32, C.c_size_t, C.POINTER(_P)], C.c_int),
+    "gph_server_destroy": ([_P], None),
+    "gph_server_add_system": ([_P, _P, C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
+    "gph_server_footprint": ([_P, C.c_int32, C.POINTER(C.c_size_t)], C.c_int),
+    "gph_server_run": ([_P, C.c_int64, _P, _P, _P, _P, _P, C.POINTER(_ReqResult)], C.c_int),
 }
 
 
@@ -497,3 +508,57 @@ class Matrix:
     def solve(self, x, b, config):
         """Matrix::solve (lg/matrix/Matrix.cpp:26-30)."""
         return Solver(self, config).solve(x, b)
+
+
+class Server:
+    """Request server with continuous batching (include/graphene_amd.h,
+    gph_server_*): resident systems, requests admitted by HBM footprint,
+    kernels of concurrent requests overlapping on the device."""
+
+    def __init__(self, workers=4, hbm_budget_bytes=64 << 30):
+        h = _P()
+        _check(lib().gph_server_create(int(workers), int(hbm_budget_bytes), C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gph_server_destroy(self._h)
+            self._h = None
+
+    def add_system(self, matrix, config):
+        sid = C.c_int32()
+        _check(lib().gph_server_add_system(self._h, matrix._A, _cfg_text(config).encode(),
+                                           C.byref(sid)))
+        self._keep.append(matrix)
+        return sid.value
+
+    def footprint(self, sid):
+        b = C.c_size_t()
+        _check(lib().gph_server_footprint(self._h, int(sid), C.byref(b)))
+        return b.value
+
+    def run(self, system, n_rhs, arrival, seed, keep_solution=None):
+        """Runs the requests; returns (results dict of arrays, solutions).
+        keep_solution: optional list of (request index, n_rows) whose first
+        right-hand side's solution is returned (tile-major order)."""
+        n = len(system)
+        sysa = np.ascontiguousarray(system, np.int32)
+        rhsa = np.ascontiguousarray(n_rhs, np.int32)
+        arra = np.ascontiguousarray(arrival, np.float64)
+        seeda = np.ascontiguousarray(seed, np.uint64)
+        res = (_ReqResult * max(n, 1))()
+        sols = {}
+        xo = None
+        if keep_solution:
+            xo = (C.c_void_p * n)()
+            for i, rows in keep_solution:
+                sols[i] = np.zeros(rows, np.float64)
+                xo[i] = sols[i].ctypes.data
+        _check(lib().gph_server_run(self._h, n, _ptr(sysa), _ptr(rhsa), _ptr(arra),
+                                    _ptr(seeda), C.cast(xo, _P) if xo is not None else None,
+                                    res))
+        out = {k: np.array([getattr(res[i], k) for i in range(n)])
+               for k in ("latency", "wait", "service", "iterations", "inner_iterations",
+                         "rel_residual", "converged")}
+        return out, sols

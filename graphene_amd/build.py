@@ -21,6 +21,7 @@ SOURCES = [
     "device_matrix.cpp",
     "solvers.cpp",
     "capi.cpp",
+    "server.cpp",
     "ilu_pack.cpp",
     "k_spmv.hip",
     "k_vec.hip",

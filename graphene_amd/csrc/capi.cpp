@@ -7,6 +7,7 @@
 #include "config.hpp"
 #include "device_matrix.hpp"
 #include "host_matrix.hpp"
+#include "server.hpp"
 #include "solvers.hpp"
 
 using namespace graphene;
@@ -18,6 +19,9 @@ struct gph_host_matrix {
 };
 struct gph_matrix {
   std::unique_ptr<DeviceMatrix> A;
+};
+struct gph_server {
+  std::unique_ptr<graphene::serve::Server> S;
 };
 struct gph_solver {
   const gph_matrix* A;
@@ -577,3 +581,63 @@ int gph_config_check(const char* json, char* name_buf, size_t len) {
 }
 
 }  // extern "C"
+
+int gph_server_create(int32_t workers, size_t hbm_budget_bytes, gph_server** out) {
+  return guard([&] {
+    need(out, "out");
+    if (workers < 1) gph::fail(gph::kInvalid, "workers must be >= 1");
+    auto s = std::make_unique<gph_server>();
+    s->S = std::make_unique<graphene::serve::Server>(workers, hbm_budget_bytes);
+    *out = s.release();
+  });
+}
+void gph_server_destroy(gph_server* s) { delete s; }
+int gph_server_add_system(gph_server* s, const gph_matrix* A, const char* json,
+                          int32_t* system_id) {
+  return guard([&] {
+    need(s, "server");
+    need(A, "matrix");
+    need(json, "json");
+    need(system_id, "system_id");
+    *system_id = s->S->addSystem(*A->A, json);
+  });
+}
+int gph_server_footprint(const gph_server* s, int32_t system_id, size_t* bytes) {
+  return guard([&] {
+    need(s, "server");
+    need(bytes, "bytes");
+    *bytes = s->S->footprint(system_id);
+  });
+}
+int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
+                   const int32_t* n_rhs, const double* arrival_s,
+                   const uint64_t* seed, double* const* x0_out,
+                   gph_request_result* results) {
+  return guard([&] {
+    need(s, "server");
+    if (n < 0) gph::fail(gph::kInvalid, "negative request count");
+    if (n == 0) return;
+    need(system, "system");
+    need(n_rhs, "n_rhs");
+    need(arrival_s, "arrival_s");
+    need(seed, "seed");
+    need(results, "results");
+    std::vector<double*> xo;
+    if (x0_out) xo.assign(x0_out, x0_out + n);
+    auto r = s->S->run(std::vector<int32_t>(system, system + n),
+                       std::vector<int32_t>(n_rhs, n_rhs + n),
+                       std::vector<double>(arrival_s, arrival_s + n),
+                       std::vector<uint64_t>(seed, seed + n), xo);
+    for (int64_t i = 0; i < n; i++) {
+      gph_request_result& o = results[i];
+      o.latency = r[(size_t)i].latency;
+      o.wait = r[(size_t)i].wait;
+      o.service = r[(size_t)i].service;
+      o.iterations = r[(size_t)i].iterations;
+      o.inner_iterations = r[(size_t)i].innerIterations;
+      o.rel_residual = r[(size_t)i].relResidual;
+      o.converged = r[(size_t)i].converged;
+      o.pad = 0;
+    }
+  });
+}

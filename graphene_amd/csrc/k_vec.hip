@@ -221,6 +221,25 @@ void launchFill(void* x, DType t, int64_t n, double value, hipStream_t s) {
   GPH_HIP(hipGetLastError());
 }
 
+// x_i = U[-1,1) from a counter-based hash of (seed, i): splitmix64 of
+// seed + (i+1)*golden, mapped as ((u >> 40) * 2^-24) * 2 - 1.
+__global__ void uniformKernel(double* x, int64_t n, uint64_t seed) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    x[i] = (double)((float)(z >> 40) * 5.9604644775390625e-8f) * 2.0 - 1.0;
+  }
+}
+
+void launchUniform(double* x, int64_t n, uint64_t seed, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(uniformKernel, dim3(elemGrid(n)), dim3(256), 0, s, x, n, seed);
+  GPH_HIP(hipGetLastError());
+}
+
 void launchCast(const void* in, DType it, void* out, DType ot, int64_t n,
                 hipStream_t s) {
   if (n <= 0) return;

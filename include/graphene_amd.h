@@ -228,6 +228,40 @@ int gph_solver_uses_initial_guess(const gph_solver* S, int32_t* out);
 /* Validates a configuration without a matrix (CPU-only). */
 int gph_config_check(const char* json, char* name_buf, size_t len);
 
+/* ---- request server (continuous batching) ---------------------------------
+ * No reference counterpart: the serving extension of the solve path. Systems
+ * (a device matrix + a PBiCGStab or IterativeRefinement configuration) are
+ * registered once and stay resident with their factors and workspace;
+ * requests (system, right-hand sides, seed) arrive on a schedule and are
+ * admitted by `workers` host threads when their system is idle and the
+ * footprint of the requests in flight stays within `hbm_budget_bytes`; their
+ * kernels overlap on the device. Each right-hand side is b = fl(A x_true),
+ * x_true ~ U[-1,1) from the seed, solved from x = 0. */
+typedef struct gph_server gph_server;
+typedef struct {
+  double latency;      /* finish - arrival, seconds */
+  double wait;         /* admission - arrival */
+  double service;      /* finish - admission */
+  int64_t iterations;  /* summed over the right-hand sides */
+  int64_t inner_iterations;
+  double rel_residual; /* largest final ||r|| / ||b|| */
+  int32_t converged;   /* all right-hand sides converged */
+  int32_t pad;
+} gph_request_result;
+int gph_server_create(int32_t workers, size_t hbm_budget_bytes, gph_server** out);
+void gph_server_destroy(gph_server* s);
+/* The matrix must outlive the server. */
+int gph_server_add_system(gph_server* s, const gph_matrix* A, const char* json,
+                          int32_t* system_id);
+int gph_server_footprint(const gph_server* s, int32_t system_id, size_t* bytes);
+/* Runs n requests; arrival_s[i] = seconds after the start; x0_out (nullable,
+ * or an array of n nullable pointers to n_rows doubles) receives the solution
+ * of request i's first right-hand side in the matrix's tile-major order. */
+int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
+                   const int32_t* n_rhs, const double* arrival_s,
+                   const uint64_t* seed, double* const* x0_out,
+                   gph_request_result* results);
+
 #ifdef __cplusplus
 }
 #endif
