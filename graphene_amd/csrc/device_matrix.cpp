@@ -96,7 +96,7 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
 
   scratchState_.alloc(sizeof(gph::DevState));
   GPH_HIP(hipMemsetAsync(scratchState_.get(), 0, sizeof(gph::DevState), s));
-  int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096, gph::kSellGrid});
+  int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096, gph::kSellMaxGrid});
   partialsBytes_ = (size_t)maxBlocks * 2 * sizeof(double);
   scratchPartials_.alloc(partialsBytes_);
   GPH_HIP(hipStreamSynchronize(s));
@@ -138,7 +138,7 @@ void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
 }
 
 // Sliced ELL: 64 rows per slice, slot (k, lane) = sliceOff[s] + 64k + lane.
-// Rows keep their stored entry order. When row lengths vary strongly (max >
+// Rows keep their stored entry order; padding slots hold column 0 and value 0. When row lengths vary strongly (max >
 // 2x mean), rows are sorted by length inside windows of 4096 rows so that a
 // slice holds rows of similar length (perm gives each lane's row).
 void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,

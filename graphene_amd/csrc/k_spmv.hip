@@ -29,24 +29,49 @@ struct RedAcc {
   }
 };
 
-// Row result: residual, stores, dot-product contributions.
-template <typename XT, typename BT, typename OT, typename AT, int MODE, int RED>
+// Row inputs of the epilogue: b[row] (residual) and w[row] (dot partner).
+template <typename XT, typename BT, int MODE, int RED>
+struct RowIn {
+  BT b{};
+  XT w{};
+  __device__ __forceinline__ void load(const SpmvArgs& a, int row, bool has) {
+    if constexpr (MODE == kModeResidual) {
+      if (has) b = static_cast<const BT*>(a.b)[row];
+    }
+    if constexpr (RED == kRedW || RED == kRedSelfW) {
+      if (has) w = static_cast<const XT*>(a.w)[row];
+    }
+  }
+};
+
+template <typename T, bool NT>
+__device__ __forceinline__ void storeOut(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
+// Row result: residual, stores (non-temporal with NT), dot-product
+// contributions.
+template <typename XT, typename BT, typename OT, typename AT, int MODE, int RED,
+          bool NT = false>
 __device__ __forceinline__ void spmvRowOut(const SpmvArgs& a, int row, bool has,
-                                           AT sum, RedAcc<AT, RED>& acc) {
+                                           AT sum, const RowIn<XT, BT, MODE, RED>& in,
+                                           RedAcc<AT, RED>& acc) {
   AT o = sum;
   if constexpr (MODE == kModeResidual) {
-    if (has) o = (AT)static_cast<const BT*>(a.b)[row] - sum;
+    if (has) o = (AT)in.b - sum;
   }
   if (has) {
     const OT ov = (OT)o;
-    static_cast<OT*>(a.out)[row] = ov;
-    if (a.out2) static_cast<OT*>(a.out2)[row] = ov;
-    if (a.out3) static_cast<OT*>(a.out3)[row] = ov;
-    if (a.out64) a.out64[row] = (double)o;
+    storeOut<OT, NT>(static_cast<OT*>(a.out) + row, ov);
+    if (a.out2) storeOut<OT, NT>(static_cast<OT*>(a.out2) + row, ov);
+    if (a.out3) storeOut<OT, NT>(static_cast<OT*>(a.out3) + row, ov);
+    if (a.out64) storeOut<double, NT>(a.out64 + row, (double)o);
     if constexpr (RED != kRedNone) {
       AT wv = 0;
-      if constexpr (RED == kRedW || RED == kRedSelfW)
-        wv = (AT) static_cast<const XT*>(a.w)[row];
+      if constexpr (RED == kRedW || RED == kRedSelfW) wv = (AT)in.w;
       if constexpr (RED == kRedSelf) acc.v[0] = acc.v[0] + o * o;
       if constexpr (RED == kRedW) acc.v[0] = acc.v[0] + o * wv;
       if constexpr (RED == kRedSelfW) {
@@ -129,7 +154,9 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
   }
 
   RedAcc<AT, RED> acc;
-  spmvRowOut<XT, BT, OT, AT, MODE, RED>(a, row, has, sum, acc);
+  RowIn<XT, BT, MODE, RED> in;
+  in.load(a, row, has);
+  spmvRowOut<XT, BT, OT, AT, MODE, RED>(a, row, has, sum, in, acc);
   spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);
 }
 
@@ -138,6 +165,15 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
 // sliceOff[s] + k*64 + l, so every wave-level load is one contiguous
 // 256-byte (4-byte types) segment. Per row the order is the stored order,
 // as in the row-block kernel.
+//
+// A wave takes its slices in chunks of kSellChunk and stores a chunk's
+// results after the chunk (rows, sums and epilogue inputs stay in
+// registers): a store counts against the wave's memory counter until it is
+// acknowledged, so a store issued at the end of every slice would delay
+// the next slice's first waits. Results are stored non-temporally (the
+// output is not read again by this kernel).
+constexpr int kSellChunk = 4;
+
 template <typename VT, typename XT, typename BT, typename OT, typename AT,
           int MODE, int RED>
 __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
@@ -153,45 +189,72 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
   const int per = (groups + gridDim.x - 1) / gridDim.x;
   const int g0 = lb * per, g1 = min(groups, g0 + per);
   const XT* __restrict__ x = static_cast<const XT*>(a.x);
+  using In = RowIn<XT, BT, MODE, RED>;
   RedAcc<AT, RED> acc;
-  for (int g = g0; g < g1; g++) {
-    const int slice = g * W + wave;
-    const bool sliceOk = slice < a.numSlices;
-    const int slot = slice * 64 + lane;
-    int row = 0, len = 0;
-    if (sliceOk) {
-      row = a.perm ? a.perm[slot] : slot;
-      len = a.rowLen[slot];
-    }
-    const bool has = sliceOk && row < a.n;
-    AT sum = 0;
-    if (has) {
-      sum = cvt<AT>(static_cast<const VT*>(a.diag)[row]) * (AT)x[row];
-      const int base = a.sliceOff[slice] + lane;
-      const int32_t* __restrict__ col = a.scol + base;
-      const VT* __restrict__ val = static_cast<const VT*>(a.sval) + base;
-      // 8 entries per step: 16 independent stream loads, then 8 gathers,
-      // then the adds in stored order
-      int k = 0;
-      for (; k + 8 <= len; k += 8) {
-        int c[8];
-        VT v[8];
+  for (int c0 = g0; c0 < g1; c0 += kSellChunk) {
+    int rows[kSellChunk];
+    bool hass[kSellChunk];
+    AT sums[kSellChunk];
+    In ins[kSellChunk];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-          c[j] = col[(k + j) * 64];
-          v[j] = val[(k + j) * 64];
-        }
-        XT xv[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
-#pragma unroll
-        for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
+    for (int i = 0; i < kSellChunk; i++) {
+      const int g = c0 + i;
+      const int slice = g * W + wave;
+      const bool sliceOk = g < g1 && slice < a.numSlices;
+      const int slot = slice * 64 + lane;
+      int row = 0, len = 0;
+      if (sliceOk) {
+        row = a.perm ? a.perm[slot] : slot;
+        len = a.rowLen[slot];
       }
-      for (; k < len; k++) sum = sum + cvt<AT>(val[k * 64]) * (AT)x[col[k * 64]];
+      const bool has = sliceOk && row < a.n;
+      ins[i].load(a, row, has);
+      AT sum = 0;
+      if (has) {
+        sum = cvt<AT>(static_cast<const VT*>(a.diag)[row]) * (AT)x[row];
+        const int base = a.sliceOff[slice] + lane;
+        const int32_t* __restrict__ col = a.scol + base;
+        const VT* __restrict__ val = static_cast<const VT*>(a.sval) + base;
+        // 8 entries per step: 16 independent stream loads, then 8 gathers,
+        // then the adds in stored order
+        int k = 0;
+        for (; k + 8 <= len; k += 8) {
+          int c[8];
+          VT v[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            c[j] = col[(k + j) * 64];
+            v[j] = val[(k + j) * 64];
+          }
+          XT xv[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
+#pragma unroll
+          for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
+        }
+        for (; k < len; k++) sum = sum + cvt<AT>(val[k * 64]) * (AT)x[col[k * 64]];
+      }
+      rows[i] = row;
+      hass[i] = has;
+      sums[i] = sum;
     }
-    spmvRowOut<XT, BT, OT, AT, MODE, RED>(a, row, has, sum, acc);
+#pragma unroll
+    for (int i = 0; i < kSellChunk; i++)
+      spmvRowOut<XT, BT, OT, AT, MODE, RED, true>(a, rows[i], hass[i], sums[i], ins[i], acc);
   }
   spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);
+}
+
+// Blocks of a kernel that are resident at once on the current device (the
+// sliced kernel's grid: one round of blocks, each with an equal share).
+template <typename K>
+static int residentBlocks(K kernel) {
+  int dev = 0, cus = 0, per = 0;
+  GPH_HIP(hipGetDevice(&dev));
+  GPH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  GPH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kSpmvThreads, 0));
+  const int n = cus * per;
+  return n < 1 ? 1 : (n > kSellMaxGrid ? kSellMaxGrid : n);
 }
 
 template <int S, typename VT, typename XT, typename BT, typename OT,
@@ -199,8 +262,9 @@ template <int S, typename VT, typename XT, typename BT, typename OT,
 static void launchOne(const SpmvArgs& a, hipStream_t s) {
   if (S == 0) {
     if (a.numSlices <= 0) return;
+    static const int grid = residentBlocks(sellKernel<VT, XT, BT, OT, AT, MODE, RED>);
     const int groups = (a.numSlices + kSpmvThreads / 64 - 1) / (kSpmvThreads / 64);
-    const int nb = groups < kSellGrid ? groups : kSellGrid;
+    const int nb = groups < grid ? groups : grid;
     hipLaunchKernelGGL((sellKernel<VT, XT, BT, OT, AT, MODE, RED>), dim3(nb),
                        dim3(kSpmvThreads), 0, s, a);
   } else {

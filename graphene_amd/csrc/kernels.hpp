@@ -11,8 +11,8 @@
 namespace gph {
 
 constexpr int kSpmvThreads = 256;   // one row per thread, <= 256 rows/block
+constexpr int kSellMaxGrid = 4096;  // bound of the sliced-ELL grid (resident blocks)
 constexpr int kSpmvCap = 2048;      // staged products per chunk
-constexpr int kSellGrid = 2048;     // blocks of the sliced-ELL kernel
 constexpr int kVecThreads = 256;
 constexpr int kTileThreads = 256;
 
