@@ -138,7 +138,12 @@ void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
 }
 
 // Sliced ELL: 64 rows per slice, slot (k, lane) = sliceOff[s] + 64k + lane.
-// Rows keep their stored entry order; padding slots hold column 0 and value 0. When row lengths vary strongly (max >
+// Rows keep their stored entry order; padding slots hold value 0 and a valid
+// column. Column indices are compressed per slice: when a slice's columns
+// span less than 2^16, they are stored as 16-bit offsets from sliceBase[s]
+// (entry (k, lane) at 16-bit word sliceColW[s] + 64k + lane); otherwise as
+// 32-bit columns (sliceBase[s] = -1, entry at word sliceColW[s] + 2(64k +
+// lane)). Structured-grid matrices in tile order have most slices narrow. When row lengths vary strongly (max >
 // 2x mean), rows are sorted by length inside windows of 4096 rows so that a
 // slice holds rows of similar length (perm gives each lane's row).
 void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
@@ -181,22 +186,64 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
     if (slots > INT32_MAX) return;
   }
   sliceOff[(size_t)numSlices] = (int32_t)slots;
-  std::vector<int32_t> c((size_t)std::max<int64_t>(slots, 1), 0);
-  std::vector<float> v((size_t)std::max<int64_t>(slots, 1), 0.0f);
-  for (int64_t sl = 0; sl < numSlices; sl++)
+  // column range per slice -> 16-bit or 32-bit column words
+  std::vector<int32_t> base((size_t)numSlices, -1);
+  std::vector<uint32_t> colW((size_t)numSlices + 1, 0);
+  uint64_t words = 0;
+  for (int64_t sl = 0; sl < numSlices; sl++) {
+    int32_t lo = INT32_MAX, hi = -1;
     for (int l = 0; l < 64; l++) {
       int32_t r = perm[(size_t)(sl * 64 + l)];
       if (r >= n) continue;
-      for (int64_t k = 0; k < rowPtr[(size_t)r + 1] - rowPtr[(size_t)r]; k++) {
-        size_t slot = (size_t)(sliceOff[(size_t)sl] + 64 * k + l);
-        c[slot] = cols[(size_t)(rowPtr[(size_t)r] + k)];
-        v[slot] = vals[(size_t)(rowPtr[(size_t)r] + k)];
+      for (int64_t e = rowPtr[(size_t)r]; e < rowPtr[(size_t)r + 1]; e++) {
+        lo = std::min(lo, cols[(size_t)e]);
+        hi = std::max(hi, cols[(size_t)e]);
       }
     }
+    const int64_t sw = sliceOff[(size_t)sl + 1] - sliceOff[(size_t)sl];
+    if (hi < 0) lo = hi = 0;  // empty slice
+    colW[(size_t)sl] = (uint32_t)words;
+    if ((int64_t)hi - lo < 65536) {
+      base[(size_t)sl] = lo;
+      words += (uint64_t)sw;
+    } else {
+      words += 2 * (uint64_t)sw;
+    }
+    if (words > UINT32_MAX) return;
+  }
+  colW[(size_t)numSlices] = (uint32_t)words;
+  std::vector<uint16_t> c((size_t)std::max<uint64_t>(words, 2), 0);
+  std::vector<float> v((size_t)std::max<int64_t>(slots, 1), 0.0f);
+  for (int64_t sl = 0; sl < numSlices; sl++) {
+    const int32_t b = base[(size_t)sl];
+    for (int l = 0; l < 64; l++) {
+      int32_t r = perm[(size_t)(sl * 64 + l)];
+      const int64_t ln = r < n ? rowPtr[(size_t)r + 1] - rowPtr[(size_t)r] : 0;
+      const int64_t K = (sliceOff[(size_t)sl + 1] - sliceOff[(size_t)sl]) / 64;
+      for (int64_t k = 0; k < K; k++) {
+        const size_t slot = (size_t)(sliceOff[(size_t)sl] + 64 * k + l);
+        // padding: the slice base (narrow) or column 0 (wide)
+        int32_t cc = b >= 0 ? b : 0;
+        if (k < ln) {
+          cc = cols[(size_t)(rowPtr[(size_t)r] + k)];
+          v[slot] = vals[(size_t)(rowPtr[(size_t)r] + k)];
+        }
+        const size_t w = (size_t)colW[(size_t)sl] + (size_t)(64 * k + l) * (b >= 0 ? 1 : 2);
+        if (b >= 0) {
+          c[w] = (uint16_t)(cc - b);
+        } else {
+          c[w] = (uint16_t)((uint32_t)cc & 0xffffu);
+          c[w + 1] = (uint16_t)((uint32_t)cc >> 16);
+        }
+      }
+    }
+  }
   hipStream_t s = stream_;
   out.sliceOff.upload(sliceOff.data(), sliceOff.size(), s);
   out.rowLen.upload(len.data(), len.size(), s);
   if (sorted) out.perm.upload(perm.data(), perm.size(), s);
+  out.sliceBase.upload(base.data(), base.size(), s);
+  out.sliceColW.upload(colW.data(), colW.size(), s);
   out.col.upload(c.data(), c.size(), s);
   out.val32.upload(v.data(), v.size(), s);
   out.slots = slots;
@@ -260,7 +307,9 @@ gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
       a.sliceOff = sl.sliceOff.as<int32_t>();
       a.rowLen = sl.rowLen.as<uint16_t>();
       a.perm = sl.perm ? sl.perm.as<int32_t>() : nullptr;
-      a.scol = sl.col.as<int32_t>();
+      a.sliceBase = sl.sliceBase.as<int32_t>();
+      a.sliceColW = sl.sliceColW.as<uint32_t>();
+      a.scol = sl.col.as<uint16_t>();
       a.sval = sellVal(sl, vt);
       return a;
     }
@@ -362,7 +411,8 @@ size_t DeviceMatrix::deviceBytes() const {
     b += p->bytes();
   for (const Sell* sl : {&sellCrs_, &sellLdu_})
     for (const DevBuf* p : std::initializer_list<const DevBuf*>{
-             &sl->sliceOff, &sl->rowLen, &sl->perm, &sl->col, &sl->val32,
+             &sl->sliceOff, &sl->rowLen, &sl->perm, &sl->sliceBase, &sl->sliceColW,
+             &sl->col, &sl->val32,
              &sl->val64, &sl->val16})
       b += p->bytes();
   return b;

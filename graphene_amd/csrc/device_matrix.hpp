@@ -98,7 +98,9 @@ class DeviceMatrix {
   // sliced-ELL copies of the row streams (CRS order / LDU order)
   struct Sell {
     int32_t numSlices = 0;
-    gph::DevBuf sliceOff, rowLen, perm, col, val32;
+    // col: column words, 16-bit offsets from sliceBase for slices whose
+    // columns span less than 2^16, full 32-bit columns otherwise
+    gph::DevBuf sliceOff, rowLen, perm, sliceBase, sliceColW, col, val32;
     mutable gph::DevBuf val64, val16;
     int64_t slots = 0;
   };

@@ -162,17 +162,24 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
 
 // Sliced ELL: one wave per 64-row slice; lane l owns row perm[s*64+l]
 // (or s*64+l) and walks its entries k = 0..len-1 at slot
-// sliceOff[s] + k*64 + l, so every wave-level load is one contiguous
-// 256-byte (4-byte types) segment. Per row the order is the stored order,
-// as in the row-block kernel.
+// sliceOff[s] + k*64 + l, so every wave-level value load is one contiguous
+// 256-byte (4-byte types) segment. Column indices are 16-bit offsets from a
+// per-slice base where the slice's columns span less than 2^16 (2 bytes per
+// entry instead of 4), 32-bit columns otherwise. Per row the order is the
+// stored order, as in the row-block kernel.
 //
-// A wave takes its slices in chunks of kSellChunk and stores a chunk's
-// results after the chunk (rows, sums and epilogue inputs stay in
+// A wave takes its slices in chunks of CH and stores a chunk's results
+// after the chunk (rows, sums and epilogue inputs stay in
 // registers): a store counts against the wave's memory counter until it is
 // acknowledged, so a store issued at the end of every slice would delay
 // the next slice's first waits. Results are stored non-temporally (the
 // output is not read again by this kernel).
-constexpr int kSellChunk = 4;
+// Chunk length: 4 slices, 2 where the held epilogue state (FP64 sums,
+// residual and dot inputs) would cost occupancy.
+template <typename AT, int MODE, int RED>
+constexpr int sellChunk() {
+  return (sizeof(AT) == 8 || MODE == kModeResidual || RED != kRedNone) ? 2 : 4;
+}
 
 template <typename VT, typename XT, typename BT, typename OT, typename AT,
           int MODE, int RED>
@@ -191,13 +198,14 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
   const XT* __restrict__ x = static_cast<const XT*>(a.x);
   using In = RowIn<XT, BT, MODE, RED>;
   RedAcc<AT, RED> acc;
-  for (int c0 = g0; c0 < g1; c0 += kSellChunk) {
-    int rows[kSellChunk];
-    bool hass[kSellChunk];
-    AT sums[kSellChunk];
-    In ins[kSellChunk];
+  constexpr int CH = sellChunk<AT, MODE, RED>();
+  for (int c0 = g0; c0 < g1; c0 += CH) {
+    int rows[CH];
+    bool hass[CH];
+    AT sums[CH];
+    In ins[CH];
 #pragma unroll
-    for (int i = 0; i < kSellChunk; i++) {
+    for (int i = 0; i < CH; i++) {
       const int g = c0 + i;
       const int slice = g * W + wave;
       const bool sliceOk = g < g1 && slice < a.numSlices;
@@ -212,34 +220,41 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
       AT sum = 0;
       if (has) {
         sum = cvt<AT>(static_cast<const VT*>(a.diag)[row]) * (AT)x[row];
-        const int base = a.sliceOff[slice] + lane;
-        const int32_t* __restrict__ col = a.scol + base;
-        const VT* __restrict__ val = static_cast<const VT*>(a.sval) + base;
+        const VT* __restrict__ val = static_cast<const VT*>(a.sval) + a.sliceOff[slice] + lane;
+        const int cbase = a.sliceBase[slice];
+        const uint16_t* __restrict__ cw = a.scol + a.sliceColW[slice];
         // 8 entries per step: 16 independent stream loads, then 8 gathers,
         // then the adds in stored order
-        int k = 0;
-        for (; k + 8 <= len; k += 8) {
-          int c[8];
-          VT v[8];
+        auto rowSum = [&](auto* __restrict__ col, int cb) {
+          int k = 0;
+          for (; k + 8 <= len; k += 8) {
+            int c[8];
+            VT v[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) {
-            c[j] = col[(k + j) * 64];
-            v[j] = val[(k + j) * 64];
+            for (int j = 0; j < 8; j++) {
+              c[j] = cb + (int)col[(k + j) * 64];
+              v[j] = val[(k + j) * 64];
+            }
+            XT xv[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
           }
-          XT xv[8];
-#pragma unroll
-          for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
-#pragma unroll
-          for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
-        }
-        for (; k < len; k++) sum = sum + cvt<AT>(val[k * 64]) * (AT)x[col[k * 64]];
+          for (; k < len; k++)
+            sum = sum + cvt<AT>(val[k * 64]) * (AT)x[cb + (int)col[k * 64]];
+        };
+        if (cbase >= 0)
+          rowSum(cw + lane, cbase);  // 16-bit offsets from the slice base
+        else
+          rowSum(reinterpret_cast<const int32_t*>(cw) + lane, 0);
       }
       rows[i] = row;
       hass[i] = has;
       sums[i] = sum;
     }
 #pragma unroll
-    for (int i = 0; i < kSellChunk; i++)
+    for (int i = 0; i < CH; i++)
       spmvRowOut<XT, BT, OT, AT, MODE, RED, true>(a, rows[i], hass[i], sums[i], ins[i], acc);
   }
   spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);

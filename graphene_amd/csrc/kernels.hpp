@@ -47,7 +47,9 @@ struct SpmvArgs {
   const int32_t* sliceOff = nullptr;   // numSlices+1, in slots
   const uint16_t* rowLen = nullptr;    // numSlices*64 entries per lane
   const int32_t* perm = nullptr;       // lane -> row (null: slice*64+lane)
-  const int32_t* scol = nullptr;
+  const int32_t* sliceBase = nullptr;  // column base (16-bit slices) or -1
+  const uint32_t* sliceColW = nullptr; // first column word of each slice
+  const uint16_t* scol = nullptr;      // column words
   const void* sval = nullptr;
 };
 

@@ -26,14 +26,16 @@ def poisson_case(nx, ny, nz, block=None):
     return h, L
 
 
-def convdiff_case(nx, block=None, peclet=10.0, seed=1):
-    n = nx ** 3
-    rows, cols, vals = g.generateConvectionDiffusion27Triplets(nx, nx, nx, peclet, seed)
+def convdiff_case(nx, block=None, peclet=10.0, seed=1, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    rows, cols, vals = g.generateConvectionDiffusion27Triplets(nx, ny, nz, peclet, seed)
     if block is None:
         h = g.HostMatrix.from_coo(n, rows, cols, vals)
         L = oracle_layout(n, rows, cols, vals)
     else:
-        r2t, T = g.partition_grid3d(nx, nx, nx, *block)
+        r2t, T = g.partition_grid3d(nx, ny, nz, *block)
         h = g.HostMatrix.from_coo(n, rows, cols, vals, T, r2t)
         L = oracle_layout(n, rows, cols, vals, T, r2t)
     return h, L

@@ -60,6 +60,47 @@ def test_spmv_fp16_values(case):
     assert np.array_equal(y, o.spmv(L16, x)), name
 
 
+def slice_column_spans(L):
+    """Column span (max - min) of every 64-row slice of the tile-major CRS
+    rows (slices of the unsorted sliced layout)."""
+    rp, col, n = L["row_ptr"], L["col"], L["n"]
+    spans = []
+    for r0 in range(0, n, 64):
+        c = col[rp[r0]:rp[min(n, r0 + 64)]]
+        spans.append(int(c.max() - c.min()) if len(c) else 0)
+    return np.array(spans)
+
+
+WIDE_CASES = {
+    # z-neighbour tiles 128 tiles (131072 rows) apart: slices on tile z-faces
+    # need 32-bit columns, the others take 16-bit offsets
+    "convdiff128x128x16_T256": lambda: convdiff_case(128, block=(16, 8, 8), ny=128, nz=16),
+    # long-range columns: every slice 32-bit
+    "powerlaw150000_T8": lambda: powerlaw_case(150000, T=8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(WIDE_CASES))
+def test_spmv_wide_and_narrow_slices(name):
+    """Sliced SpMV with per-slice 16-bit column offsets and 32-bit columns:
+    bit-exact against the oracle (CRS and, for the stencil, LDU; FP32 and
+    FP64 values)."""
+    h, L = WIDE_CASES[name]()
+    if name.startswith("convdiff"):
+        spans = slice_column_spans(L)
+        assert (spans < 65536).any() and (spans >= 65536).any()
+    A = h.copyToTile()
+    x = o.uniform_pm1(31, h.nrows)
+    y = (A * dev(x)).numpy()
+    assert np.array_equal(y.view(np.uint32), o.spmv(L, x).view(np.uint32))
+    x64 = x.astype(np.float64) * 1.0000001
+    assert np.array_equal(A.spmv(dev(x64)).numpy(), o.spmv(L, x64))
+    if name.startswith("convdiff"):
+        A_ldu = h.copyToTile(g.LDU)
+        U = o.ldu_build(L)
+        assert np.array_equal((A_ldu * dev(x)).numpy(), o.ldu_spmv_f32(U, x))
+
+
 def test_spmv_f64(case):
     name, h, L, A = case
     x = o.uniform_pm1(12, h.nrows).astype(np.float64) * 1.0000001
