@@ -27,6 +27,7 @@ SOURCES = [
     "k_vec.hip",
     "k_precond.hip",
     "k_ilu_packed.hip",
+    "k_ilu_level.hip",
 ]
 
 COMMON = [

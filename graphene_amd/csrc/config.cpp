@@ -322,6 +322,12 @@ ilu::Configuration::Configuration(const Json& c) {
   readBool(c, "diagonalBased", diagonalBased);
   readType(c, "factorType", factorType);
   readInt(c, "rowsPerWave", rowsPerWave);
+  if (const Json* j = c.get("substitution")) {
+    substitution = asText(*j);
+    if (substitution != "level" && substitution != "packed" && substitution != "tile")
+      fail(kInvalid, "Invalid value for substitution: " + substitution +
+                         ". Allowed values are \"level\", \"packed\" or \"tile\"");
+  }
 }
 
 gaussseidel::Configuration::Configuration(const Json& c) {

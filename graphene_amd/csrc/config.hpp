@@ -107,8 +107,11 @@ struct Configuration : solver::Configuration {
   bool diagonalBased = false;
   // extension: storage type of the factors (default: matrix type float32)
   std::optional<gph::DType> factorType;
-  // extension: rows per wave group of the packed substitution kernel
+  // extension: rows per wave group of the substitution kernels
   int rowsPerWave = 1024;
+  // extension: substitution kernel: "packed" (product slabs), "level"
+  // (lane per row over level records), "tile" (row blocks)
+  std::string substitution = "packed";
   explicit Configuration(const Json& c);
   std::string solverName() const override {
     return diagonalBased ? "DILU" : "ILU(0)";

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 
 namespace graphene::matrix {
 
@@ -10,8 +11,7 @@ namespace graphene::matrix {
 // rows; a group is one wave's unit of work. Tiles are independent blocks,
 // so level L of a group is the union of its tiles' level-L rows and the
 // group's level count is the largest tile level count.
-static std::vector<int32_t> groupTiles(const host::HostMatrix& m,
-                                       int64_t targetRows) {
+std::vector<int32_t> groupTiles(const host::HostMatrix& m, int64_t targetRows) {
   std::vector<int32_t> g;
   g.push_back(0);
   int64_t rows = 0;
@@ -150,6 +150,177 @@ static void packDirection(const host::HostMatrix& m,
   }
   d.slabOff[(size_t)G] = (int32_t)slab;
   d.lvlOff[(size_t)G] = (int32_t)d.slabEnd.size();
+}
+
+// Rows of level L of a group (matrix ids, level order) and their entries
+// (CRS positions, in substitution order), row after row.
+static void groupLevel(const host::HostMatrix& m, int32_t ta, int32_t tb, int32_t L,
+                       bool backward, std::vector<int64_t>& lvRows,
+                       std::vector<int32_t>& lvEnts, std::vector<int64_t>& lvOff) {
+  const std::vector<int32_t>& rows = backward ? m.upperLevelRows : m.colorSortAddr;
+  const std::vector<int64_t>& starts = backward ? m.upperColorStart : m.colorStart;
+  const std::vector<int64_t>& startOff =
+      backward ? m.upperColorStartOffset : m.colorStartOffset;
+  lvRows.clear();
+  lvEnts.clear();
+  lvOff.assign(1, 0);
+  for (int32_t t = ta; t < tb; t++) {
+    const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
+    const int64_t* ls = starts.data() + startOff[(size_t)t];
+    const int32_t tL = (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1);
+    if (L >= tL) continue;
+    for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
+      const int64_t i = t0 + rows[(size_t)(t0 + q)];
+      const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
+      if (!backward) {
+        for (int64_t a = a0; a < a1; a++) {
+          const int64_t j = m.col[(size_t)a];
+          if (i < j || j < t0 || j >= t1) break;
+          lvEnts.push_back((int32_t)a);
+        }
+      } else {
+        for (int64_t a = a1 - 1; a >= a0; a--) {
+          const int64_t j = m.col[(size_t)a];
+          if (j < t0 || j >= t1) continue;
+          if (j < i) break;
+          lvEnts.push_back((int32_t)a);
+        }
+      }
+      lvRows.push_back(i);
+      lvOff.push_back((int64_t)lvEnts.size());
+    }
+  }
+}
+
+static int32_t groupLevels(const host::HostMatrix& m, int32_t ta, int32_t tb, bool backward) {
+  const std::vector<int64_t>& startOff =
+      backward ? m.upperColorStartOffset : m.colorStartOffset;
+  int32_t nL = 0;
+  for (int32_t t = ta; t < tb; t++)
+    nL = std::max<int32_t>(nL, (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1));
+  return nL;
+}
+
+IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
+                             const void* lu, const void* invD, int es, bool diluFwd,
+                             int64_t maxStepBytes, int64_t ring, int64_t tail) {
+  IluLevelStream o;
+  o.ok = true;
+  const std::vector<int32_t> groups = groupTiles(m, targetRows);
+  const int32_t G = (int32_t)groups.size() - 1;
+  o.groupStart.resize((size_t)G + 1);
+  for (int32_t q = 0; q <= G; q++) {
+    o.groupStart[(size_t)q] = (int32_t)m.tileStart[(size_t)groups[(size_t)q]];
+    if (q > 0)
+      o.maxRows = std::max(o.maxRows, o.groupStart[(size_t)q] - o.groupStart[(size_t)q - 1]);
+  }
+  o.groupOff.assign((size_t)G + 1, 0);
+  o.steps.assign((size_t)2 * G, 0);
+  const uint8_t* luB = static_cast<const uint8_t*>(lu);
+  const uint8_t* invB = static_cast<const uint8_t*>(invD);
+  auto pad16 = [](int64_t b) { return (b + 15) & ~(int64_t)15; };
+  auto put = [&](size_t at, const void* src, size_t n) {
+    std::memcpy(o.bytes.data() + at, src, n);
+  };
+  std::vector<int64_t> lvRows, lvOff;
+  std::vector<int32_t> lvEnts;
+  for (int32_t gi = 0; gi < G; gi++) {
+    const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
+    const int64_t g0 = m.tileStart[(size_t)ta];
+    const int64_t ng = m.tileStart[(size_t)tb] - g0;
+    if (ng > 65536) {
+      o.ok = false;
+      return o;
+    }
+    o.groupOff[(size_t)gi] = (int64_t)o.bytes.size();
+    for (int dir = 0; dir < 2; dir++) {
+      const bool backward = dir == 1;
+      const bool withInv = backward || diluFwd;
+      const int32_t nL = groupLevels(m, ta, tb, backward);
+      int64_t rowsSeen = 0;
+      for (int32_t L = 0; L < nL; L++) {
+        groupLevel(m, ta, tb, L, backward, lvRows, lvEnts, lvOff);
+        const int64_t R = (int64_t)lvRows.size();
+        rowsSeen += R;
+        for (int64_t r0 = 0; r0 < R;) {
+          // step rows: up to 64, fewer if the record would exceed the limit
+          // or cross the next ring boundary
+          auto sizeOf = [&](int64_t rs, int64_t& w) {
+            w = 0;
+            for (int64_t r = r0; r < r0 + rs; r++)
+              w = std::max<int64_t>(w, lvOff[(size_t)r + 1] - lvOff[(size_t)r]);
+            return 16 + pad16(4 * rs) + (withInv ? pad16(es * rs) : 0) + pad16(2 * w * rs) +
+                   pad16(es * w * rs);
+          };
+          int64_t Rs = std::min<int64_t>(64, R - r0);
+          int64_t W = 0, bytes = sizeOf(Rs, W);
+          while (bytes > maxStepBytes && Rs > 1) {
+            Rs = (Rs + 1) / 2;
+            bytes = sizeOf(Rs, W);
+          }
+          if (bytes > maxStepBytes || bytes > ring || W > 65535) {
+            o.ok = false;
+            return o;
+          }
+          const int64_t pos = ((int64_t)o.bytes.size() - o.groupOff[(size_t)gi]) % ring;
+          if (pos + bytes > ring) {
+            while (Rs > 1 && pos + bytes > ring) bytes = sizeOf(--Rs, W);
+            if (pos + bytes > ring) {  // not even one row fits: filler record
+              const size_t at = o.bytes.size();
+              o.bytes.resize(at + (size_t)(ring - pos), 0);
+              const uint32_t fb = (uint32_t)(ring - pos);
+              put(at + 4, &fb, 4);
+              o.steps[(size_t)(2 * gi + dir)]++;
+              continue;
+            }
+          }
+          const size_t at = o.bytes.size();
+          o.bytes.resize(at + (size_t)bytes, 0);
+          const uint16_t hR = (uint16_t)Rs, hW = (uint16_t)W;
+          const uint32_t hB = (uint32_t)bytes, hF = withInv ? 1u : 0u;
+          put(at, &hR, 2);
+          put(at + 2, &hW, 2);
+          put(at + 4, &hB, 4);
+          put(at + 8, &hF, 4);
+          size_t q = at + 16;
+          for (int64_t r = 0; r < Rs; r++) {
+            const uint32_t len = (uint32_t)(lvOff[(size_t)(r0 + r) + 1] - lvOff[(size_t)(r0 + r)]);
+            const uint32_t info = (uint32_t)(lvRows[(size_t)(r0 + r)] - g0) | (len << 16);
+            put(q + 4 * (size_t)r, &info, 4);
+          }
+          q += (size_t)pad16(4 * Rs);
+          if (withInv) {
+            for (int64_t r = 0; r < Rs; r++)
+              put(q + (size_t)(es * r), invB + (size_t)es * (size_t)lvRows[(size_t)(r0 + r)],
+                  (size_t)es);
+            q += (size_t)pad16(es * Rs);
+          }
+          const size_t qc = q, qv = q + (size_t)pad16(2 * W * Rs);
+          for (int64_t j = 0; j < W; j++)
+            for (int64_t r = 0; r < Rs; r++) {
+              const int64_t len = lvOff[(size_t)(r0 + r) + 1] - lvOff[(size_t)(r0 + r)];
+              const size_t e = (size_t)(j * Rs + r);
+              uint16_t c = 0;  // unused slot: column 0, value 0
+              if (j < len) {
+                const int32_t a = lvEnts[(size_t)(lvOff[(size_t)(r0 + r)] + j)];
+                c = (uint16_t)(m.col[(size_t)a] - g0);
+                put(qv + (size_t)es * e, luB + (size_t)es * (size_t)a, (size_t)es);
+              }
+              put(qc + 2 * e, &c, 2);
+            }
+          o.steps[(size_t)(2 * gi + dir)]++;
+          r0 += Rs;
+        }
+      }
+      if (rowsSeen != ng) {
+        o.ok = false;
+        return o;
+      }
+    }
+  }
+  o.groupOff[(size_t)G] = (int64_t)o.bytes.size();
+  o.bytes.resize(o.bytes.size() + (size_t)pad16(tail), 0);
+  return o;
 }
 
 PackedILU packILU(const host::HostMatrix& m, int64_t targetRows) {

@@ -50,6 +50,45 @@ struct PackedILU {
 // even number of slabs.
 PackedILU packILU(const host::HostMatrix& m, int64_t targetRows);
 
+// Consecutive tiles grouped into units of about targetRows rows: G+1 tile
+// boundaries.
+std::vector<int32_t> groupTiles(const host::HostMatrix& m, int64_t targetRows);
+
+// Level-record stream of the ILU(0) factors for the lane-per-row kernel.
+//
+// Per group, the forward then the backward substitution as a sequence of
+// steps; a step is up to 64 rows of one level (rows of a level are
+// independent). A step record is 16-byte aligned:
+//   header (16 B): u16 R, u16 W, u32 bytes (record size), u32 flags
+//                  (bit 0: per-row 1/diag present), u32 0
+//   u32 row info [R]: local row id | entry count << 16  (padded to 16 B)
+//   factor 1/diag [R]           (flag bit 0)   (padded to 16 B)
+//   u16 local column [W*R], entry j of row r at j*R + r (padded to 16 B)
+//   factor value [W*R], same order             (padded to 16 B)
+// where W is the longest row of the step. Entries are in the order the
+// reference's substitution visits them (forward: strictly lower in-tile
+// entries ascending; backward: strictly upper in-tile entries descending,
+// ilu/Solver.cpp:380-442); slots past a row's entry count hold column 0 and
+// value 0 and are not used. A step's record is at most maxStepBytes. No
+// record crosses a multiple of `ring` bytes from its group's start: a step
+// that would takes fewer rows, and where not even one row fits, a filler
+// record (R = 0, W = 0, header only) pads up to the boundary.
+struct IluLevelStream {
+  std::vector<uint8_t> bytes;        // all groups, then a zero tail
+  std::vector<int64_t> groupOff;     // G+1 byte offsets (multiples of 16)
+  std::vector<int32_t> steps;        // 2G: forward, backward step counts
+  std::vector<int32_t> groupStart;   // G+1 row offsets
+  int32_t maxRows = 0;
+  bool ok = false;  // false: 16-bit ids or the record size limit exceeded
+};
+
+// lu: factor off-diagonals in CRS order, invD: 1/diag (or the DILU
+// diagonal), both of element size es (2, 4 or 8 bytes). diluFwd: forward
+// records carry 1/diag too. tail: zero bytes appended (read-ahead).
+IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
+                             const void* lu, const void* invD, int es, bool diluFwd,
+                             int64_t maxStepBytes, int64_t ring, int64_t tail);
+
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,
 // out[p*128 + 2l + (s&1)] = in[s*64 + l].

@@ -161,6 +161,28 @@ size_t packedSharedBytes(const PackedArgs& p, DType xt);
 void launchIluPacked(const PackedArgs& p, const void* invD, DType ft,
                      const void* rhs, void* out, DType xt, const PreOp& pre,
                      DevState* st, int checkDone, hipStream_t s);
+// Level-record ILU(0) substitution (lane per row, record stream through an
+// LDS ring filled by LDS-DMA; record format in ilu_pack.hpp).
+struct LevelArgs {
+  int32_t G = 0;                       // groups (one wave each)
+  const int32_t* groupStart = nullptr;  // G+1 row offsets
+  const int64_t* groupOff = nullptr;    // G+1 byte offsets of the records
+  const int32_t* steps = nullptr;       // 2G: forward, backward step counts
+  const uint8_t* bytes = nullptr;       // record stream (+ zero tail)
+  int32_t maxRows = 0;
+  int32_t dilu = 0;
+};
+constexpr int kLvlChunk = 1024;        // bytes per LDS-DMA wave load (64 x 16 B)
+constexpr int kLvlRing = 16384;        // LDS ring bytes
+constexpr int kLvlLag = 10;            // chunks in flight beyond the current step
+// Largest step record (the ring holds a step, its successor's header and a
+// partial chunk on each side) and the zero tail after the stream.
+constexpr int64_t kLvlMaxStep = kLvlRing - 2 * kLvlChunk;
+constexpr int64_t kLvlTail = kLvlRing + kLvlChunk;
+size_t levelSharedBytes(const LevelArgs& p, DType xt);
+void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
+                    DType xt, const PreOp& pre, DevState* st, int checkDone,
+                    hipStream_t s);
 // dst[k] = idx[k] < 0 ? 0 : src[idx[k]]  (same element type t)
 void launchGather(const void* src, DType t, const int32_t* idx, void* dst,
                   int64_t n, hipStream_t s);

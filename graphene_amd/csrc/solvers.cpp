@@ -107,6 +107,10 @@ class IluSolver : public Solver {
       lu = lu16_.get();
       inv = invD16_.get();
     }
+    if (level_ && gph::levelSharedBytes(largs_, out.type) <= kMaxPackedShared) {
+      gph::launchIluLevel(largs_, ft, rhs.ptr, out.ptr, out.type, pre, st, st ? 1 : 0, s);
+      return;
+    }
     if (packed_ && gph::packedSharedBytes(pargs_, out.type) <= kMaxPackedShared) {
       gph::launchIluPacked(pargs_, inv, ft, rhs.ptr, out.ptr, out.type, pre, st,
                            st ? 1 : 0, s);
@@ -118,7 +122,7 @@ class IluSolver : public Solver {
 
   size_t factorBytes() const {
     return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes() +
-           fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes();
+           fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes();
   }
 
  private:
@@ -138,8 +142,38 @@ class IluSolver : public Solver {
       gph::launchCast(lu_.get(), work, lu16_.get(), DType::F16, nnz, s);
       gph::launchCast(invD_.get(), work, invD16_.get(), DType::F16, n, s);
     }
-    buildPacked(s);
+    if (cfg_->substitution == "level") buildLevel(s);
+    if (!level_ && cfg_->substitution != "tile") buildPacked(s);  // also the level fallback
     GPH_HIP(hipStreamSynchronize(s));
+  }
+
+  // Level records of the factors for the lane-per-row kernel (built on the
+  // host from the device factors).
+  void buildLevel(hipStream_t s) {
+    const int64_t nnz = A_.nnzOff(), n = A_.n();
+    const size_t es = gph::dtypeSize(ft_);
+    std::vector<uint8_t> lu((size_t)std::max<int64_t>(nnz, 1) * es), inv((size_t)n * es);
+    const void* luD = ft_ == DType::F16 ? lu16_.get() : lu_.get();
+    const void* invDv = ft_ == DType::F16 ? invD16_.get() : invD_.get();
+    GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipMemcpyAsync(inv.data(), invDv, (size_t)n * es, hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipStreamSynchronize(s));
+    IluLevelStream L = packIluLevels(A_.hostMatrix(), std::max(1, cfg_->rowsPerWave), lu.data(),
+                                     inv.data(), (int)es, cfg_->diagonalBased, gph::kLvlMaxStep,
+                                     gph::kLvlRing, gph::kLvlTail);
+    if (!L.ok) return;
+    lbytes_.upload(L.bytes.data(), L.bytes.size(), s);
+    lgoff_.upload(L.groupOff.data(), L.groupOff.size(), s);
+    lsteps_.upload(L.steps.data(), L.steps.size(), s);
+    lgstart_.upload(L.groupStart.data(), L.groupStart.size(), s);
+    largs_.G = (int32_t)L.groupStart.size() - 1;
+    largs_.groupStart = lgstart_.as<int32_t>();
+    largs_.groupOff = lgoff_.as<int64_t>();
+    largs_.steps = lsteps_.as<int32_t>();
+    largs_.bytes = lbytes_.as<uint8_t>();
+    largs_.maxRows = L.maxRows;
+    largs_.dilu = cfg_->diagonalBased ? 1 : 0;
+    level_ = true;
   }
 
   // Level-major packed copy of the factors for the one-wave-per-tile kernel.
@@ -194,8 +228,10 @@ class IluSolver : public Solver {
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_, invD16_;
-  bool packed_ = false;
+  bool packed_ = false, level_ = false;
   gph::PackedArgs pargs_;
+  gph::LevelArgs largs_;
+  DevBuf lbytes_, lgoff_, lsteps_, lgstart_;
   DevBuf fcol_, fval_, fso_, flo_, fse_, frs_, fri_, fwd_;
   DevBuf bcol_, bval_, bso_, blo_, bse_, brs_, bri_, bwd_;
 };

@@ -140,11 +140,15 @@ def test_ldu_spmv_bit_exact(case, sliced):
     assert np.array_equal(r.numpy(), o.ldu_residual_mixed(U, x64, b)[0])
 
 
+@pytest.mark.parametrize("sub", ["packed", "level", "tile"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
-def test_ilu0_apply_bit_exact(case, ftype):
+def test_ilu0_apply_bit_exact(case, ftype, sub):
+    """ILU(0) factorization + substitution equal the oracle's, with each
+    substitution kernel (product slabs, lane-per-row level records, row
+    blocks)."""
     name, h, L, A = case
     dt = np.float32 if ftype == "float32" else np.float64
-    S = g.Solver(A, {"type": "ILU", "factorType": ftype})
+    S = g.Solver(A, {"type": "ILU", "factorType": ftype, "substitution": sub})
     b = o.uniform_pm1(17, h.nrows).astype(dt)
     x = dev(np.zeros(h.nrows, dt))
     st = S.solve(x, dev(b))
@@ -154,12 +158,14 @@ def test_ilu0_apply_bit_exact(case, ftype):
     assert np.array_equal(x.numpy(), ref), name
 
 
+@pytest.mark.parametrize("sub", ["packed", "level"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
-def test_dilu_apply_bit_exact(case, ftype):
+def test_dilu_apply_bit_exact(case, ftype, sub):
     """DILU (diagonalBased) factorization + substitution equal the oracle's."""
     name, h, L, A = case
     dt = np.float32 if ftype == "float32" else np.float64
-    S = g.Solver(A, {"type": "ILU", "factorType": ftype, "diagonalBased": True})
+    S = g.Solver(A, {"type": "ILU", "factorType": ftype, "diagonalBased": True,
+                     "substitution": sub})
     assert S.name() == "DILU"
     b = o.uniform_pm1(27, h.nrows).astype(dt)
     x = dev(np.zeros(h.nrows, dt))
@@ -169,10 +175,11 @@ def test_dilu_apply_bit_exact(case, ftype):
     assert np.array_equal(x.numpy(), ref), name
 
 
-def test_ilu0_fp16_factors_bit_exact(case):
+@pytest.mark.parametrize("sub", ["packed", "level"])
+def test_ilu0_fp16_factors_bit_exact(case, sub):
     """FP16-stored factors: the FP32 factors rounded to FP16, applied in FP32."""
     name, h, L, A = case
-    S = g.Solver(A, {"type": "ILU", "factorType": "float16"})
+    S = g.Solver(A, {"type": "ILU", "factorType": "float16", "substitution": sub})
     b = o.uniform_pm1(28, h.nrows).astype(np.float32)
     x = dev(np.zeros(h.nrows, np.float32))
     S.solve(x, dev(b))
@@ -274,6 +281,24 @@ def test_pbicgstab_f64_and_mpir(case):
     if st.iterations == sto.iterations:
         assert abs(st.innerIterations - inner) <= max(3, 0.1 * inner), \
             (st.innerIterations, inner)
+
+
+@pytest.mark.parametrize("ftype", ["float32", "float16"])
+def test_pbicgstab_level_substitution_identical(case, ftype):
+    """PBiCGStab with the lane-per-row ILU(0) kernel (its fused p- and
+    s-updates included) gives the same iterates, bit for bit, as with the
+    product-slab kernel."""
+    name, h, L, A = case
+    xt, b = rhs_for(h, L, seed=5)
+    outs = []
+    for sub in ("packed", "level"):
+        cfg = _pbicgstab_cfg(1e-6)
+        cfg["preconditioner"] = {"type": "ILU", "factorType": ftype, "substitution": sub}
+        x = dev(np.zeros(h.nrows, np.float32))
+        st = g.Solver(A, cfg).solve(x, dev(b))
+        outs.append((st.iterations, x.numpy()))
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
 
 
 def test_solver_deterministic(case):
