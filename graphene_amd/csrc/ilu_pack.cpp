@@ -203,122 +203,157 @@ static int32_t groupLevels(const host::HostMatrix& m, int32_t ta, int32_t tb, bo
 
 IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
                              const void* lu, const void* invD, int es, bool diluFwd,
-                             int64_t maxStepBytes, int64_t ring, int64_t tail) {
+                             int64_t maxStepBytes, int64_t ring, int64_t tail, int64_t wAlign) {
   IluLevelStream o;
   o.ok = true;
   const std::vector<int32_t> groups = groupTiles(m, targetRows);
   const int32_t G = (int32_t)groups.size() - 1;
+  const int32_t P = (G + 1) / 2;
   o.groupStart.resize((size_t)G + 1);
   for (int32_t q = 0; q <= G; q++) {
     o.groupStart[(size_t)q] = (int32_t)m.tileStart[(size_t)groups[(size_t)q]];
     if (q > 0)
       o.maxRows = std::max(o.maxRows, o.groupStart[(size_t)q] - o.groupStart[(size_t)q - 1]);
   }
-  o.groupOff.assign((size_t)G + 1, 0);
-  o.steps.assign((size_t)2 * G, 0);
+  if (o.maxRows + 1 > 65535) {
+    o.ok = false;
+    return o;
+  }
+  const uint16_t zero = (uint16_t)o.maxRows;  // zero slot of every half
+  o.pairOff.assign((size_t)P + 1, 0);
+  o.steps.assign((size_t)2 * P, 0);
   const uint8_t* luB = static_cast<const uint8_t*>(lu);
   const uint8_t* invB = static_cast<const uint8_t*>(invD);
   auto pad16 = [](int64_t b) { return (b + 15) & ~(int64_t)15; };
   auto put = [&](size_t at, const void* src, size_t n) {
     std::memcpy(o.bytes.data() + at, src, n);
   };
-  std::vector<int64_t> lvRows, lvOff;
-  std::vector<int32_t> lvEnts;
-  for (int32_t gi = 0; gi < G; gi++) {
-    const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
-    const int64_t g0 = m.tileStart[(size_t)ta];
-    const int64_t ng = m.tileStart[(size_t)tb] - g0;
-    if (ng > 65536) {
-      o.ok = false;
-      return o;
-    }
-    o.groupOff[(size_t)gi] = (int64_t)o.bytes.size();
+  // one half's current level: rows (matrix ids) and entries, and the next
+  // row of it to place
+  struct Half {
+    int32_t ta = 0, tb = 0;
+    int64_t g0 = 0;
+    int32_t nL = 0, L = -1;
+    int64_t r0 = 0;
+    std::vector<int64_t> rows, off;
+    std::vector<int32_t> ents;
+    int64_t R() const { return (int64_t)rows.size(); }
+    int64_t len(int64_t r) const { return off[(size_t)r + 1] - off[(size_t)r]; }
+  };
+  for (int32_t pi = 0; pi < P; pi++) {
+    o.pairOff[(size_t)pi] = (int64_t)o.bytes.size();
     for (int dir = 0; dir < 2; dir++) {
       const bool backward = dir == 1;
       const bool withInv = backward || diluFwd;
-      const int32_t nL = groupLevels(m, ta, tb, backward);
-      int64_t rowsSeen = 0;
-      for (int32_t L = 0; L < nL; L++) {
-        groupLevel(m, ta, tb, L, backward, lvRows, lvEnts, lvOff);
-        const int64_t R = (int64_t)lvRows.size();
-        rowsSeen += R;
-        for (int64_t r0 = 0; r0 < R;) {
-          // step rows: up to 64, fewer if the record would exceed the limit
-          // or cross the next ring boundary
-          auto sizeOf = [&](int64_t rs, int64_t& w) {
-            w = 0;
-            for (int64_t r = r0; r < r0 + rs; r++)
-              w = std::max<int64_t>(w, lvOff[(size_t)r + 1] - lvOff[(size_t)r]);
-            return 16 + pad16(4 * rs) + (withInv ? pad16(es * rs) : 0) + pad16(2 * w * rs) +
-                   pad16(es * w * rs);
-          };
-          int64_t Rs = std::min<int64_t>(64, R - r0);
-          int64_t W = 0, bytes = sizeOf(Rs, W);
-          while (bytes > maxStepBytes && Rs > 1) {
-            Rs = (Rs + 1) / 2;
-            bytes = sizeOf(Rs, W);
-          }
-          if (bytes > maxStepBytes || bytes > ring || W > 65535) {
-            o.ok = false;
-            return o;
-          }
-          const int64_t pos = ((int64_t)o.bytes.size() - o.groupOff[(size_t)gi]) % ring;
-          if (pos + bytes > ring) {
-            while (Rs > 1 && pos + bytes > ring) bytes = sizeOf(--Rs, W);
-            if (pos + bytes > ring) {  // not even one row fits: filler record
-              const size_t at = o.bytes.size();
-              o.bytes.resize(at + (size_t)(ring - pos), 0);
-              const uint32_t fb = (uint32_t)(ring - pos);
-              put(at + 4, &fb, 4);
-              o.steps[(size_t)(2 * gi + dir)]++;
-              continue;
-            }
-          }
-          const size_t at = o.bytes.size();
-          o.bytes.resize(at + (size_t)bytes, 0);
-          const uint16_t hR = (uint16_t)Rs, hW = (uint16_t)W;
-          const uint32_t hB = (uint32_t)bytes, hF = withInv ? 1u : 0u;
-          put(at, &hR, 2);
-          put(at + 2, &hW, 2);
-          put(at + 4, &hB, 4);
-          put(at + 8, &hF, 4);
-          size_t q = at + 16;
-          for (int64_t r = 0; r < Rs; r++) {
-            const uint32_t len = (uint32_t)(lvOff[(size_t)(r0 + r) + 1] - lvOff[(size_t)(r0 + r)]);
-            const uint32_t info = (uint32_t)(lvRows[(size_t)(r0 + r)] - g0) | (len << 16);
-            put(q + 4 * (size_t)r, &info, 4);
-          }
-          q += (size_t)pad16(4 * Rs);
-          if (withInv) {
-            for (int64_t r = 0; r < Rs; r++)
-              put(q + (size_t)(es * r), invB + (size_t)es * (size_t)lvRows[(size_t)(r0 + r)],
-                  (size_t)es);
-            q += (size_t)pad16(es * Rs);
-          }
-          const size_t qc = q, qv = q + (size_t)pad16(2 * W * Rs);
-          for (int64_t j = 0; j < W; j++)
-            for (int64_t r = 0; r < Rs; r++) {
-              const int64_t len = lvOff[(size_t)(r0 + r) + 1] - lvOff[(size_t)(r0 + r)];
-              const size_t e = (size_t)(j * Rs + r);
-              uint16_t c = 0;  // unused slot: column 0, value 0
-              if (j < len) {
-                const int32_t a = lvEnts[(size_t)(lvOff[(size_t)(r0 + r)] + j)];
-                c = (uint16_t)(m.col[(size_t)a] - g0);
-                put(qv + (size_t)es * e, luB + (size_t)es * (size_t)a, (size_t)es);
-              }
-              put(qc + 2 * e, &c, 2);
-            }
-          o.steps[(size_t)(2 * gi + dir)]++;
-          r0 += Rs;
-        }
+      Half H[2];
+      for (int h = 0; h < 2; h++) {
+        const int32_t gi = 2 * pi + h;
+        if (gi >= G) continue;
+        H[h].ta = groups[(size_t)gi];
+        H[h].tb = groups[(size_t)gi + 1];
+        H[h].g0 = m.tileStart[(size_t)H[h].ta];
+        H[h].nL = groupLevels(m, H[h].ta, H[h].tb, backward);
       }
-      if (rowsSeen != ng) {
-        o.ok = false;
-        return o;
+      auto advance = [&](Half& x) {  // next level with rows left, if any
+        while (x.r0 >= x.R() && x.L + 1 < x.nL) {
+          x.L++;
+          x.r0 = 0;
+          groupLevel(m, x.ta, x.tb, x.L, backward, x.rows, x.ents, x.off);
+        }
+        return x.r0 < x.R();
+      };
+      int64_t placed[2] = {0, 0};
+      for (;;) {
+        const bool has0 = advance(H[0]), has1 = advance(H[1]);
+        if (!has0 && !has1) break;
+        // rows of this step per half (up to 32 each), the same count slot
+        // range for both halves; W = longest row of both, rounded up to even
+        int64_t Rs = 32;
+        int64_t cnt[2], W = 0, bytes = 0;
+        auto sizeOf = [&](int64_t rs) {
+          W = 0;
+          for (int h = 0; h < 2; h++) {
+            cnt[h] = (h == 0 ? has0 : has1) ? std::min<int64_t>(rs, H[h].R() - H[h].r0) : 0;
+            for (int64_t r = 0; r < cnt[h]; r++) W = std::max(W, H[h].len(H[h].r0 + r));
+          }
+          W = (W + wAlign - 1) / wAlign * wAlign;
+          const int64_t rm = std::max(cnt[0], cnt[1]);
+          return 16 + pad16(2 * 2 * rm) + (withInv ? pad16(es * 2 * rm) : 0) +
+                 pad16(2 * 2 * rm * W) + pad16(es * 2 * rm * W);
+        };
+        bytes = sizeOf(Rs);
+        while (bytes > maxStepBytes && Rs > 1) bytes = sizeOf(Rs = (Rs + 1) / 2);
+        if (bytes > maxStepBytes || bytes > ring || W > 65534) {
+          o.ok = false;
+          return o;
+        }
+        const int64_t pos = ((int64_t)o.bytes.size() - o.pairOff[(size_t)pi]) % ring;
+        if (pos + bytes > ring) {
+          while (Rs > 1 && pos + bytes > ring) bytes = sizeOf(--Rs);
+          if (pos + bytes > ring) {  // not even one row fits: filler record
+            const size_t at = o.bytes.size();
+            o.bytes.resize(at + (size_t)(ring - pos), 0);
+            const uint32_t fb = (uint32_t)(ring - pos);
+            put(at + 4, &fb, 4);
+            o.steps[(size_t)(2 * pi + dir)]++;
+            continue;
+          }
+        }
+        const int64_t rm = std::max(cnt[0], cnt[1]);
+        const size_t at = o.bytes.size();
+        o.bytes.resize(at + (size_t)bytes, 0);
+        const uint16_t hR = (uint16_t)rm, hW = (uint16_t)W;
+        const uint32_t hB = (uint32_t)bytes, hF = withInv ? 1u : 0u;
+        put(at, &hR, 2);
+        put(at + 2, &hW, 2);
+        put(at + 4, &hB, 4);
+        put(at + 8, &hF, 4);
+        const size_t qLoc = at + 16;
+        const size_t qInv = qLoc + (size_t)pad16(4 * rm);
+        const size_t qc = qInv + (withInv ? (size_t)pad16(es * 2 * rm) : 0);
+        const size_t qv = qc + (size_t)pad16(4 * rm * W);
+        for (int h = 0; h < 2; h++)
+          for (int64_t r = 0; r < rm; r++) {
+            const size_t slot = (size_t)(h * rm + r);
+            uint16_t loc = zero;  // empty slot: zero row, zero entries
+            if (r < cnt[h]) {
+              const Half& x = H[h];
+              const int64_t row = x.r0 + r;
+              loc = (uint16_t)(x.rows[(size_t)row] - x.g0);
+              if (withInv)
+                put(qInv + (size_t)es * slot, invB + (size_t)es * (size_t)x.rows[(size_t)row],
+                    (size_t)es);
+              for (int64_t j = 0; j < x.len(row); j++) {
+                const int32_t a = x.ents[(size_t)(x.off[(size_t)row] + j)];
+                const uint16_t c = (uint16_t)(m.col[(size_t)a] - x.g0);
+                put(qc + 2 * (slot * (size_t)W + (size_t)j), &c, 2);
+                put(qv + (size_t)es * (slot * (size_t)W + (size_t)j), luB + (size_t)es * (size_t)a,
+                    (size_t)es);
+              }
+              for (int64_t j = x.len(row); j < W; j++)  // zero products
+                put(qc + 2 * (slot * (size_t)W + (size_t)j), &zero, 2);
+            } else {
+              for (int64_t j = 0; j < W; j++) put(qc + 2 * (slot * (size_t)W + (size_t)j), &zero, 2);
+            }
+            put(qLoc + 2 * slot, &loc, 2);
+          }
+        for (int h = 0; h < 2; h++) {
+          H[h].r0 += cnt[h];
+          placed[h] += cnt[h];
+        }
+        o.steps[(size_t)(2 * pi + dir)]++;
+      }
+      for (int h = 0; h < 2; h++) {
+        const int32_t gi = 2 * pi + h;
+        const int64_t ng = gi < G ? o.groupStart[(size_t)gi + 1] - o.groupStart[(size_t)gi] : 0;
+        if (placed[h] != ng) {
+          o.ok = false;
+          return o;
+        }
       }
     }
   }
-  o.groupOff[(size_t)G] = (int64_t)o.bytes.size();
+  o.pairOff[(size_t)P] = (int64_t)o.bytes.size();
   o.bytes.resize(o.bytes.size() + (size_t)pad16(tail), 0);
   return o;
 }

@@ -56,29 +56,32 @@ std::vector<int32_t> groupTiles(const host::HostMatrix& m, int64_t targetRows);
 
 // Level-record stream of the ILU(0) factors for the lane-per-row kernel.
 //
-// Per group, the forward then the backward substitution as a sequence of
-// steps; a step is up to 64 rows of one level (rows of a level are
-// independent). A step record is 16-byte aligned:
-//   header (16 B): u16 R, u16 W, u32 bytes (record size), u32 flags
-//                  (bit 0: per-row 1/diag present), u32 0
-//   u32 row info [R]: local row id | entry count << 16  (padded to 16 B)
-//   factor 1/diag [R]           (flag bit 0)   (padded to 16 B)
-//   u16 local column [W*R], entry j of row r at j*R + r (padded to 16 B)
-//   factor value [W*R], same order             (padded to 16 B)
-// where W is the longest row of the step. Entries are in the order the
-// reference's substitution visits them (forward: strictly lower in-tile
+// Groups are taken in pairs (A = group 2p, B = group 2p+1); a wave runs both
+// with lanes 0-31 on A and 32-63 on B. Per pair, the forward then the
+// backward substitution is a sequence of steps; a step holds up to 32 rows
+// of the current level of each half (rows of a level are independent). A
+// step record is 16-byte aligned:
+//   header (16 B): u16 Rm (row slots per half), u16 W (entries per row
+//                  slot, even), u32 bytes (record size), u32 flags (bit 0:
+//                  per-row 1/diag present), u32 0
+//   u16 local row id [2][Rm]                     (padded to 16 B)
+//   factor 1/diag [2][Rm]        (flag bit 0)    (padded to 16 B)
+//   u16 local column [2][Rm][W]  (row-major)     (padded to 16 B)
+//   factor value [2][Rm][W]      (row-major)     (padded to 16 B)
+// Slot q = h*Rm + r is row r of half h. Entries of a row are in the order
+// the reference's substitution visits them (forward: strictly lower in-tile
 // entries ascending; backward: strictly upper in-tile entries descending,
-// ilu/Solver.cpp:380-442); slots past a row's entry count hold column 0 and
-// value 0 and are not used. A step's record is at most maxStepBytes. No
-// record crosses a multiple of `ring` bytes from its group's start: a step
-// that would takes fewer rows, and where not even one row fits, a filler
-// record (R = 0, W = 0, header only) pads up to the boundary.
+// ilu/Solver.cpp:380-442). Unused entries and empty slots refer to the zero
+// slot (local id maxRows, where x stays +0) with value 0: their +0 products
+// leave every partial sum unchanged. A record is at most maxStepBytes and
+// does not cross a multiple of `ring` bytes from its pair's start (the step
+// takes fewer rows, or a filler record with Rm = 0 pads to the boundary).
 struct IluLevelStream {
-  std::vector<uint8_t> bytes;        // all groups, then a zero tail
-  std::vector<int64_t> groupOff;     // G+1 byte offsets (multiples of 16)
-  std::vector<int32_t> steps;        // 2G: forward, backward step counts
+  std::vector<uint8_t> bytes;        // all pairs, then a zero tail
+  std::vector<int64_t> pairOff;      // P+1 byte offsets (multiples of 16)
+  std::vector<int32_t> steps;        // 2P: forward, backward step counts
   std::vector<int32_t> groupStart;   // G+1 row offsets
-  int32_t maxRows = 0;
+  int32_t maxRows = 0;               // largest group (the zero slot's id)
   bool ok = false;  // false: 16-bit ids or the record size limit exceeded
 };
 
@@ -87,7 +90,8 @@ struct IluLevelStream {
 // records carry 1/diag too. tail: zero bytes appended (read-ahead).
 IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
                              const void* lu, const void* invD, int es, bool diluFwd,
-                             int64_t maxStepBytes, int64_t ring, int64_t tail);
+                             int64_t maxStepBytes, int64_t ring, int64_t tail,
+                             int64_t wAlign);
 
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,

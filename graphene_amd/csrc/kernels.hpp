@@ -161,15 +161,17 @@ size_t packedSharedBytes(const PackedArgs& p, DType xt);
 void launchIluPacked(const PackedArgs& p, const void* invD, DType ft,
                      const void* rhs, void* out, DType xt, const PreOp& pre,
                      DevState* st, int checkDone, hipStream_t s);
-// Level-record ILU(0) substitution (lane per row, record stream through an
-// LDS ring filled by LDS-DMA; record format in ilu_pack.hpp).
+// Level-record ILU(0) substitution (lane per row, two groups per wave,
+// record stream through an LDS ring filled by LDS-DMA; record format in
+// ilu_pack.hpp).
 struct LevelArgs {
-  int32_t G = 0;                       // groups (one wave each)
+  int32_t G = 0;                        // groups
+  int32_t P = 0;                        // group pairs (one wave each)
   const int32_t* groupStart = nullptr;  // G+1 row offsets
-  const int64_t* groupOff = nullptr;    // G+1 byte offsets of the records
-  const int32_t* steps = nullptr;       // 2G: forward, backward step counts
+  const int64_t* pairOff = nullptr;     // P+1 byte offsets of the records
+  const int32_t* steps = nullptr;       // 2P: forward, backward step counts
   const uint8_t* bytes = nullptr;       // record stream (+ zero tail)
-  int32_t maxRows = 0;
+  int32_t maxRows = 0;                  // also the zero slot's local id
   int32_t dilu = 0;
 };
 constexpr int kLvlChunk = 1024;        // bytes per LDS-DMA wave load (64 x 16 B)

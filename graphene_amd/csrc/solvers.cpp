@@ -160,15 +160,16 @@ class IluSolver : public Solver {
     GPH_HIP(hipStreamSynchronize(s));
     IluLevelStream L = packIluLevels(A_.hostMatrix(), std::max(1, cfg_->rowsPerWave), lu.data(),
                                      inv.data(), (int)es, cfg_->diagonalBased, gph::kLvlMaxStep,
-                                     gph::kLvlRing, gph::kLvlTail);
+                                     gph::kLvlRing, gph::kLvlTail, 2);
     if (!L.ok) return;
     lbytes_.upload(L.bytes.data(), L.bytes.size(), s);
-    lgoff_.upload(L.groupOff.data(), L.groupOff.size(), s);
+    lgoff_.upload(L.pairOff.data(), L.pairOff.size(), s);
     lsteps_.upload(L.steps.data(), L.steps.size(), s);
     lgstart_.upload(L.groupStart.data(), L.groupStart.size(), s);
     largs_.G = (int32_t)L.groupStart.size() - 1;
+    largs_.P = (int32_t)L.pairOff.size() - 1;
     largs_.groupStart = lgstart_.as<int32_t>();
-    largs_.groupOff = lgoff_.as<int64_t>();
+    largs_.pairOff = lgoff_.as<int64_t>();
     largs_.steps = lsteps_.as<int32_t>();
     largs_.bytes = lbytes_.as<uint8_t>();
     largs_.maxRows = L.maxRows;
