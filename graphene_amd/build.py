@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgraphene_amd.so")
+CLI = os.path.join(HERE, "benchmark")           # drop-in benchmark tool
+CLI_SOURCE = "benchmark_main.cpp"
 
 SOURCES = [
     "host_matrix.cpp",
@@ -81,6 +83,15 @@ def build(jobs=None, verbose=False):
     if p.returncode != 0:
         raise RuntimeError("link failed\n" + p.stdout + p.stderr)
     os.replace(tmp, LIB)
+    # the drop-in `benchmark` command-line tool, linked against the library
+    obj, _ = _compile(CLI_SOURCE)
+    tmp = CLI + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-o", tmp, obj, "-L", HERE, "-lgraphene_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("link failed (benchmark)\n" + p.stdout + p.stderr)
+    os.replace(tmp, CLI)
     return LIB
 
 
