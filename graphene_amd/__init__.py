@@ -107,6 +107,11 @@ _SIGS = {
     "gph_solver_uses_initial_guess": ([_P, C.POINTER(C.c_int32)], C.c_int),
     "gph_config_check": ([C.c_char_p, C.c_char_p, C.c_size_t], C.c_int),
     "gph_server_create": ([C.c_int32, C.c_size_t, C.POINTER(_P)], C.c_int),
+    "gph_batch_create": ([_P, C.c_int32, _P, C.c_char_p, C.POINTER(_P)], C.c_int),
+    "gph_batch_destroy": ([_P], None),
+    "gph_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                        C.POINTER(C.c_int32)], C.c_int),
+    "gph_batch_solve": ([_P, C.c_int, _P, _P, _P, C.POINTER(C.c_double)], C.c_int),
     "gph_server_destroy": ([_P], None),
     "gph_server_add_system": ([_P, _P, C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "gph_server_footprint": ([_P, C.c_int32, C.POINTER(C.c_size_t)], C.c_int),
@@ -508,6 +513,59 @@ class Matrix:
     def solve(self, x, b, config):
         """Matrix::solve (lg/matrix/Matrix.cpp:26-30)."""
         return Solver(self, config).solve(x, b)
+
+
+class Batch:
+    """Batched solves (include/graphene_amd.h, gph_batch_*): the systems are
+    consecutive tile ranges of one block-diagonal device matrix
+    (system_tile_start, S+1 entries), solved together from x = 0."""
+
+    def __init__(self, matrix, system_tile_start, config):
+        st = np.ascontiguousarray(system_tile_start, np.int32)
+        h = _P()
+        _check(lib().gph_batch_create(matrix._A, len(st) - 1, _ptr(st),
+                                      _cfg_text(config).encode(), C.byref(h)))
+        self._h = h
+        self._A = matrix
+        n, ref, rpw = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().gph_batch_info(h, C.byref(n), C.byref(ref), C.byref(rpw)))
+        self.num_systems = n.value
+        self.refinement = bool(ref.value)
+        self.rows_per_wave = rpw.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gph_batch_destroy(self._h)
+            self._h = None
+
+    def solve(self, x, b):
+        """x: float64 (refinement) or float32 DeviceVector, overwritten; b:
+        float32. Returns (per-system SolverStats list, seconds)."""
+        stats = (_Stats * self.num_systems)()
+        sec = C.c_double()
+        _check(lib().gph_batch_solve(self._h, x.type, x.ptr, b.ptr, stats, C.byref(sec)))
+        return [SolverStats(stats[k]) for k in range(self.num_systems)], sec.value
+
+
+def batch_union(systems, block=(32, 32)):
+    """Block-diagonal union of 2D systems for Batch: systems is a list of
+    (nx, ny, rows, cols, vals) triplets (natural order per system); every
+    system is tiled into bx x by blocks. Returns (HostMatrix,
+    system_tile_start)."""
+    bx, by = block
+    rows_l, cols_l, vals_l, r2t_l, tile_start = [], [], [], [], [0]
+    off = 0
+    for nx, ny, r, c, v in systems:
+        r2t, T = partition_grid3d(nx, ny, 1, bx, by, 1)
+        rows_l.append(np.asarray(r, np.int64) + off)
+        cols_l.append(np.asarray(c, np.int64) + off)
+        vals_l.append(np.asarray(v, np.float32))
+        r2t_l.append(r2t + tile_start[-1])
+        tile_start.append(tile_start[-1] + T)
+        off += nx * ny
+    h = HostMatrix.from_coo(off, np.concatenate(rows_l), np.concatenate(cols_l),
+                            np.concatenate(vals_l), tile_start[-1], np.concatenate(r2t_l))
+    return h, np.asarray(tile_start, np.int32)
 
 
 class Server:

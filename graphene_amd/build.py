@@ -24,12 +24,14 @@ SOURCES = [
     "solvers.cpp",
     "capi.cpp",
     "server.cpp",
+    "batch.cpp",
     "ilu_pack.cpp",
     "k_spmv.hip",
     "k_vec.hip",
     "k_precond.hip",
     "k_ilu_packed.hip",
     "k_ilu_level.hip",
+    "k_batch.hip",
 ]
 
 COMMON = [

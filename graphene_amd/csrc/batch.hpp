@@ -1,0 +1,81 @@
+// Batched solves: many independent systems held as one block-diagonal
+// matrix, solved together.
+//
+// No reference counterpart (the reference solves one system per program);
+// each system follows the single-system algorithm: MPIR with an FP32
+// PBiCGStab + ILU(0) inner solve and FP64 residual and update
+// (iterative-refinement/Solver.cpp:48-123, pbicgstab/Solver.cpp:48-194), or
+// FP32 PBiCGStab + ILU(0) alone, from a zero initial guess.
+//
+// The systems are consecutive tile ranges of one device matrix, with no
+// entry coupling two systems (checked). SpMV and the block-Jacobi ILU(0)
+// run once per step on the union; per system are the Krylov scalars, the
+// convergence state, the (segmented, deterministic) dot products and the
+// masked vector updates. The inner iteration is captured once in a HIP
+// graph of several iterations and replayed until every system has
+// finished; finished systems are skipped and a finished batch makes every
+// kernel return at once. Setup tunes the ILU kernel's rows per wave on the
+// union (a few timed applications per candidate).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_matrix.hpp"
+#include "solvers.hpp"
+
+namespace graphene::batch {
+
+struct SystemStats {
+  int64_t iterations = 0;       // outer (MPIR) or PBiCGStab iterations
+  int64_t innerIterations = 0;  // PBiCGStab iterations over all outer steps
+  double initialResidual = 0, finalResidual = 0;
+  bool converged = false, singular = false;
+};
+
+class BatchSolver {
+ public:
+  // systemTileStart: S+1 tile offsets; config: IterativeRefinement with a
+  // PBiCGStab inner solver, or PBiCGStab, each with an ILU preconditioner.
+  BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_t>& systemTileStart,
+              const std::string& configJson);
+  ~BatchSolver();
+
+  int32_t numSystems() const { return S_; }
+  int64_t systemRowStart(int32_t s) const { return sysRow_[(size_t)s]; }
+  bool refinement() const { return mpir_; }
+  int32_t tunedRowsPerWave() const { return rowsPerWave_; }
+
+  // MPIR: x float64, b float32. PBiCGStab: x and b float32. Tile-major
+  // order; x is overwritten (the solve starts from zero).
+  std::vector<SystemStats> solve(const matrix::Vec& x, const matrix::Vec& b);
+  double lastSeconds() const { return seconds_; }
+  int64_t lastGraphLaunches() const { return launches_; }
+
+ private:
+  void innerSolve(const void* rhs, void* x, std::vector<int64_t>& innerIts);
+  void enqueueIteration(hipStream_t s);
+  void captureGraph(hipStream_t s);
+
+  const matrix::DeviceMatrix& A_;
+  int32_t S_ = 0, C_ = 0;
+  std::vector<int64_t> sysRow_;
+  bool mpir_ = false;
+  int32_t rowsPerWave_ = 0;
+  gph::DevState innerInit_{}, outerInit_{};
+  std::unique_ptr<matrix::solver::Solver> ilu_;
+  gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_;
+  gph::DevBuf r_, r0_, p_, v_, y_, s_, z_, t_, d_, rw_, r64_;
+  gph::BatchArgs args_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  gph::DevState* pinned_ = nullptr;  // 2 host copies of the batch flag
+  float* xInner_ = nullptr;          // current inner solution (captured)
+  const float* rhsInner_ = nullptr;
+  double seconds_ = 0;
+  int64_t launches_ = 0;
+};
+
+}  // namespace graphene::batch

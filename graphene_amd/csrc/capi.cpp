@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/graphene_amd.h"
+#include "batch.hpp"
 #include "config.hpp"
 #include "device_matrix.hpp"
 #include "host_matrix.hpp"
@@ -22,6 +23,9 @@ struct gph_matrix {
 };
 struct gph_server {
   std::unique_ptr<graphene::serve::Server> S;
+};
+struct gph_batch {
+  std::unique_ptr<graphene::batch::BatchSolver> B;
 };
 struct gph_solver {
   const gph_matrix* A;
@@ -577,6 +581,63 @@ int gph_config_check(const char* json, char* name_buf, size_t len) {
       std::strncpy(name_buf, n.c_str(), len - 1);
       name_buf[len - 1] = 0;
     }
+  });
+}
+
+int gph_batch_create(const gph_matrix* A, int32_t n_systems,
+                     const int32_t* system_tile_start, const char* json,
+                     gph_batch** out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(system_tile_start, "system_tile_start");
+    need(json, "json");
+    need(out, "out");
+    if (n_systems < 1) gph::fail(gph::kInvalid, "n_systems must be >= 1");
+    std::vector<int32_t> st(system_tile_start, system_tile_start + n_systems + 1);
+    auto* b = new gph_batch;
+    try {
+      b->B = std::make_unique<graphene::batch::BatchSolver>(*A->A, st, json);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+void gph_batch_destroy(gph_batch* b) { delete b; }
+int gph_batch_info(const gph_batch* b, int32_t* n_systems, int32_t* refinement,
+                   int32_t* rows_per_wave) {
+  return guard([&] {
+    need(b, "batch");
+    if (n_systems) *n_systems = b->B->numSystems();
+    if (refinement) *refinement = b->B->refinement() ? 1 : 0;
+    if (rows_per_wave) *rows_per_wave = b->B->tunedRowsPerWave();
+  });
+}
+int gph_batch_solve(gph_batch* b, gph_type x_type, void* x, const void* b_f32,
+                    gph_solver_stats* stats, double* seconds) {
+  return guard([&] {
+    need(b, "batch");
+    need(x, "x");
+    need(b_f32, "b");
+    auto r = b->B->solve(Vec{x, dt(x_type)}, Vec{const_cast<void*>(b_f32), gph::DType::F32});
+    if (stats) {
+      for (size_t k = 0; k < r.size(); k++) {
+        gph_solver_stats& o = stats[k];
+        std::memset(&o, 0, sizeof(o));
+        o.initial_residual = r[k].initialResidual;
+        o.final_residual = r[k].finalResidual;
+        o.iterations = r[k].iterations;
+        o.inner_iterations = r[k].innerIterations;
+        o.converged = r[k].converged ? 1 : 0;
+        o.singular = r[k].singular ? 1 : 0;
+        o.seconds = b->B->lastSeconds();
+        std::strncpy(o.name, b->B->refinement() ? "batched IterativeRefinement"
+                                                : "batched PBiCGStab",
+                     sizeof(o.name) - 1);
+      }
+    }
+    if (seconds) *seconds = b->B->lastSeconds();
   });
 }
 

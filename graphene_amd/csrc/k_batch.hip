@@ -1,0 +1,284 @@
+// Batched PBiCGStab / MPIR kernels: many independent systems stored as one
+// block-diagonal matrix (each system a contiguous range of rows and tiles).
+//
+// SpMV and the ILU(0) substitution run on the union unchanged (no entry
+// couples two systems, the tiles of the block-Jacobi preconditioner never
+// span two systems). What is per system: the Krylov scalars and the
+// convergence state (one DevState each, updated by the scalar epilogue of
+// the single-system solver, pbicgstab/Solver.cpp:86-180), the dot products
+// (segmented: every chunk of rows lies in one system), and the vector
+// updates (masked by the system's `done` flag). Reductions are
+// deterministic: each chunk's partial sum is a fixed-order block sum, and a
+// system adds its chunks' partials in chunk order.
+//
+// Every kernel returns at once when the batch is finished (glob->done), so
+// a captured graph of several iterations can be replayed past convergence.
+#include "kernels.hpp"
+
+namespace gph {
+
+// Rows of chunk c: [chunkRow[c], chunkRow[c+1]), system chunkSys[c].
+struct ChunkRange {
+  int32_t sys, r0, r1;
+};
+__device__ __forceinline__ ChunkRange chunkOf(const BatchArgs& b, int c) {
+  return ChunkRange{b.chunkSys[c], b.chunkRow[c], b.chunkRow[c + 1]};
+}
+
+// Partial sums of K dot products a_k . b_k over one chunk (one block per
+// chunk); chunks of finished systems are skipped (their partials unused).
+template <typename X, int K>
+__global__ __launch_bounds__(kBatchThreads) void batchDotKernel(BatchArgs b, BatchDot<K> d) {
+  __shared__ X lds[K * kBatchThreads / 64];
+  if (b.glob->done) return;
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  if (b.st[ch.sys].done) return;
+  X acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = (X)0;
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) {
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      acc[k] = acc[k] + static_cast<const X*>(d.a[k])[i] * static_cast<const X*>(d.b[k])[i];
+  }
+  blockSum<X, K, kBatchThreads>(acc, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) static_cast<X*>(b.partials)[k * b.C + blockIdx.x] = acc[k];
+  }
+}
+
+// One thread per system: adds the system's chunk partials in chunk order
+// and applies the scalar epilogue `op` (the single-system finalize). Then
+// the block counts the systems still running and sets glob->done when none
+// is (one block; the counts are integers, so the result is order-free).
+template <typename X>
+__global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int K) {
+  __shared__ int nActive;
+  if (op != kFinInit && b.glob->done) return;
+  if (threadIdx.x == 0) nActive = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x) {
+    DevState* st = b.st + s;
+    if (st->done && op != kFinInit) continue;
+    double sum[2] = {0.0, 0.0};
+    for (int k = 0; k < K; k++) {
+      X acc = (X)0;
+      for (int c = b.sysChunk[s]; c < b.sysChunk[s + 1]; c++)
+        acc = acc + static_cast<const X*>(b.partials)[k * b.C + c];
+      sum[k] = (double)acc;
+    }
+    if (op == kFinInit && st->done) {
+      // a system excluded from this solve (its outer iteration has
+      // converged) stays finished
+    } else {
+      finalize<X>(st, op, sum[0], sum[1]);
+    }
+    mine += st->done ? 0 : 1;
+  }
+  if (op == kFinInit || op == kFinUpdate || op == kFinOmega) {
+    atomicAdd(&nActive, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) b.glob->done = nActive == 0 ? 1 : 0;
+  }
+}
+
+// p = r + beta (p - omega v) per active system (iteration > 0), as the
+// single-system p-update (pbicgstab/Solver.cpp:98-101).
+template <typename X>
+__global__ __launch_bounds__(kBatchThreads) void batchUpdatePKernel(BatchArgs b, const X* r,
+                                                                    X* p, const X* v) {
+  if (b.glob->done) return;
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  const DevState* st = b.st + ch.sys;
+  if (st->done || st->iter == 0) return;
+  const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+  const X omega = (X)st->omega;
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads)
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// s = r - alpha v per active system (pbicgstab/Solver.cpp:127-129).
+template <typename X>
+__global__ __launch_bounds__(kBatchThreads) void batchUpdateSKernel(BatchArgs b, const X* r,
+                                                                    X* s, const X* v) {
+  if (b.glob->done) return;
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  const DevState* st = b.st + ch.sys;
+  if (st->done) return;
+  const X alpha = (X)st->alpha;
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) s[i] = r[i] - alpha * v[i];
+}
+
+// x += alpha y + omega z ; r = s - omega t per active system, with the
+// chunk partials of r.r and r0.r (pbicgstab/Solver.cpp:162-172, :91).
+template <typename X>
+__global__ __launch_bounds__(kBatchThreads) void batchUpdateXRKernel(
+    BatchArgs b, X* x, X* r, const X* y, const X* z, const X* s, const X* t, const X* r0) {
+  __shared__ X lds[2 * kBatchThreads / 64];
+  if (b.glob->done) return;
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  const DevState* st = b.st + ch.sys;
+  if (st->done) return;
+  const X alpha = (X)st->alpha, omega = (X)st->omega;
+  X acc[2] = {(X)0, (X)0};
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) {
+    x[i] = (x[i] + alpha * y[i]) + omega * z[i];
+    const X ri = s[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] = acc[0] + ri * ri;
+    acc[1] = acc[1] + r0[i] * ri;
+  }
+  blockSum<X, 2, kBatchThreads>(acc, lds);
+  if (threadIdx.x == 0) {
+    static_cast<X*>(b.partials)[blockIdx.x] = acc[0];
+    static_cast<X*>(b.partials)[b.C + blockIdx.x] = acc[1];
+  }
+}
+
+// Start of a solve per active system: x = 0, r = r0 = p = rhs
+// (pbicgstab/Solver.cpp:66-83 with a zero initial guess), and the chunk
+// partials of r.r.
+template <typename X>
+__global__ __launch_bounds__(kBatchThreads) void batchStartKernel(BatchArgs b, const X* rhs, X* x,
+                                                                  X* r, X* r0, X* p) {
+  __shared__ X lds[kBatchThreads / 64];
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  if (b.st[ch.sys].done) return;
+  X acc[1] = {(X)0};
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) {
+    const X v = rhs[i];
+    x[i] = (X)0;
+    r[i] = v;
+    r0[i] = v;
+    p[i] = v;
+    acc[0] = acc[0] + v * v;
+  }
+  blockSum<X, 1, kBatchThreads>(acc, lds);
+  if (threadIdx.x == 0) static_cast<X*>(b.partials)[blockIdx.x] = acc[0];
+}
+
+// phi += d per system that is still refining (iterative-refinement/Solver.cpp:103).
+__global__ __launch_bounds__(kBatchThreads) void batchAccumulateKernel(BatchArgs b,
+                                                                       const DevState* outer,
+                                                                       double* phi,
+                                                                       const float* d) {
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  if (outer[ch.sys].done) return;
+  for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads)
+    phi[i] = phi[i] + (double)d[i];
+}
+
+// Outer (MPIR) residual norms, per system: the norm of the unrounded FP64
+// residual; on the first call the initial norm (r0n), afterwards one
+// refinement step is counted and convergence tested against it, with the
+// loop condition of iterative-refinement/Solver.cpp:89-112. The inner state
+// of a system that has finished refining is marked finished, so the next
+// inner solve skips it.
+__global__ __launch_bounds__(1024) void batchOuterKernel(BatchArgs b, DevState* outer,
+                                                         int first) {
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x) {
+    DevState* o = outer + s;
+    double acc = 0.0;
+    for (int c = b.sysChunk[s]; c < b.sysChunk[s + 1]; c++)
+      acc = acc + static_cast<const double*>(b.partials)[c];
+    const double rF = sqrt(acc);
+    if (first) {
+      o->r0n = rF;
+      o->rF = rF;
+      o->iter = 0;
+      o->converged = 0;
+      o->done = o->maxIt <= 0 ? 1 : 0;
+    } else if (!o->done) {
+      o->iter = o->iter + 1;
+      o->rF = rF;
+      o->converged = convergedTest(o, rF) ? 1 : 0;
+      o->done = (o->converged && o->iter >= o->minIt) || o->iter >= o->maxIt ? 1 : 0;
+    }
+    b.st[s].done = o->done;
+  }
+}
+
+// Inner states of systems that have finished refining are marked finished.
+__global__ __launch_bounds__(1024) void batchOuterMaskKernel(BatchArgs b, const DevState* outer) {
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x)
+    if (outer[s].done) b.st[s].done = 1;
+}
+
+// FP64 chunk partials of the squared unrounded residual (outer norm).
+__global__ __launch_bounds__(kBatchThreads) void batchNorm64Kernel(BatchArgs b,
+                                                                   const DevState* outer,
+                                                                   const double* r64) {
+  __shared__ double lds[kBatchThreads / 64];
+  const ChunkRange ch = chunkOf(b, blockIdx.x);
+  double acc[1] = {0.0};
+  if (!outer[ch.sys].done)
+    for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) acc[0] = acc[0] + r64[i] * r64[i];
+  blockSum<double, 1, kBatchThreads>(acc, lds);
+  if (threadIdx.x == 0) static_cast<double*>(b.partials)[blockIdx.x] = acc[0];
+}
+
+// ---------------------------------------------------------------------------
+void launchBatchDot1(const BatchArgs& b, const void* a0, const void* b0, hipStream_t s) {
+  BatchDot<1> d;
+  d.a[0] = a0;
+  d.b[0] = b0;
+  hipLaunchKernelGGL((batchDotKernel<float, 1>), dim3(b.C), dim3(kBatchThreads), 0, s, b, d);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchDot2(const BatchArgs& b, const void* a0, const void* b0, const void* a1,
+                     const void* b1, hipStream_t s) {
+  BatchDot<2> d;
+  d.a[0] = a0;
+  d.b[0] = b0;
+  d.a[1] = a1;
+  d.b[1] = b1;
+  hipLaunchKernelGGL((batchDotKernel<float, 2>), dim3(b.C), dim3(kBatchThreads), 0, s, b, d);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchFin(const BatchArgs& b, int op, int k, hipStream_t s) {
+  hipLaunchKernelGGL(batchFinKernel<float>, dim3(1), dim3(1024), 0, s, b, op, k);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchUpdateP(const BatchArgs& b, const float* r, float* p, const float* v,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(batchUpdatePKernel<float>, dim3(b.C), dim3(kBatchThreads), 0, s, b, r, p, v);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchUpdateS(const BatchArgs& b, const float* r, float* sv, const float* v,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(batchUpdateSKernel<float>, dim3(b.C), dim3(kBatchThreads), 0, s, b, r, sv, v);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchUpdateXR(const BatchArgs& b, float* x, float* r, const float* y, const float* z,
+                         const float* sv, const float* t, const float* r0, hipStream_t s) {
+  hipLaunchKernelGGL(batchUpdateXRKernel<float>, dim3(b.C), dim3(kBatchThreads), 0, s, b, x, r, y,
+                     z, sv, t, r0);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchStart(const BatchArgs& b, const float* rhs, float* x, float* r, float* r0,
+                      float* p, hipStream_t s) {
+  hipLaunchKernelGGL(batchStartKernel<float>, dim3(b.C), dim3(kBatchThreads), 0, s, b, rhs, x, r,
+                     r0, p);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchAccumulate(const BatchArgs& b, const DevState* outer, double* phi,
+                           const float* d, hipStream_t s) {
+  hipLaunchKernelGGL(batchAccumulateKernel, dim3(b.C), dim3(kBatchThreads), 0, s, b, outer, phi,
+                     d);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchOuter(const BatchArgs& b, DevState* outer, const double* r64, int first,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(batchNorm64Kernel, dim3(b.C), dim3(kBatchThreads), 0, s, b, outer, r64);
+  hipLaunchKernelGGL(batchOuterKernel, dim3(1), dim3(1024), 0, s, b, outer, first);
+  GPH_HIP(hipGetLastError());
+}
+
+void launchBatchOuterMask(const BatchArgs& b, const DevState* outer, hipStream_t s) {
+  hipLaunchKernelGGL(batchOuterMaskKernel, dim3(1), dim3(1024), 0, s, b, outer);
+  GPH_HIP(hipGetLastError());
+}
+
+}  // namespace gph

@@ -185,6 +185,42 @@ size_t levelSharedBytes(const LevelArgs& p, DType xt);
 void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
                     DType xt, const PreOp& pre, DevState* st, int checkDone,
                     hipStream_t s);
+// Batched solves over a block-diagonal union of systems (k_batch.hip):
+// chunks of rows, each inside one system; per-system solver states.
+constexpr int kBatchThreads = 256;
+constexpr int kBatchChunk = 2048;  // rows per chunk (at most)
+struct BatchArgs {
+  int32_t S = 0;                      // systems
+  int32_t C = 0;                      // chunks
+  const int32_t* chunkSys = nullptr;  // C: system of each chunk
+  const int32_t* chunkRow = nullptr;  // C+1: first row of each chunk
+  const int32_t* sysChunk = nullptr;  // S+1: first chunk of each system
+  DevState* st = nullptr;             // S inner-solver states
+  DevState* glob = nullptr;           // done = no system running
+  void* partials = nullptr;           // 2*C sums
+};
+template <int K>
+struct BatchDot {
+  const void* a[K];
+  const void* b[K];
+};
+void launchBatchDot1(const BatchArgs& b, const void* a0, const void* b0, hipStream_t s);
+void launchBatchDot2(const BatchArgs& b, const void* a0, const void* b0, const void* a1,
+                     const void* b1, hipStream_t s);
+void launchBatchFin(const BatchArgs& b, int op, int k, hipStream_t s);
+void launchBatchUpdateP(const BatchArgs& b, const float* r, float* p, const float* v,
+                        hipStream_t s);
+void launchBatchUpdateS(const BatchArgs& b, const float* r, float* sv, const float* v,
+                        hipStream_t s);
+void launchBatchUpdateXR(const BatchArgs& b, float* x, float* r, const float* y, const float* z,
+                         const float* sv, const float* t, const float* r0, hipStream_t s);
+void launchBatchStart(const BatchArgs& b, const float* rhs, float* x, float* r, float* r0,
+                      float* p, hipStream_t s);
+void launchBatchAccumulate(const BatchArgs& b, const DevState* outer, double* phi,
+                           const float* d, hipStream_t s);
+void launchBatchOuter(const BatchArgs& b, DevState* outer, const double* r64, int first,
+                      hipStream_t s);
+void launchBatchOuterMask(const BatchArgs& b, const DevState* outer, hipStream_t s);
 // dst[k] = idx[k] < 0 ? 0 : src[idx[k]]  (same element type t)
 void launchGather(const void* src, DType t, const int32_t* idx, void* dst,
                   int64_t n, hipStream_t s);

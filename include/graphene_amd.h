@@ -262,6 +262,32 @@ int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
                    const uint64_t* seed, double* const* x0_out,
                    gph_request_result* results);
 
+/* ---- batched solves ---------------------------------------------------------
+ * No reference counterpart: many independent systems solved together. The
+ * systems are consecutive tile ranges of one device matrix (tiles
+ * system_tile_start[k] .. system_tile_start[k+1]-1 form system k) with no
+ * entry coupling two systems. json: IterativeRefinement (float64 extended,
+ * float32 working type) around PBiCGStab, or PBiCGStab alone, each with an
+ * ILU preconditioner; every system follows the single-system algorithm from
+ * x = 0 with its own scalars and convergence. The inner iterations run as a
+ * captured HIP graph; the ILU kernel's rows per wave is tuned at creation
+ * unless the ILU configuration sets "rowsPerWave". */
+typedef struct gph_batch gph_batch;
+int gph_batch_create(const gph_matrix* A, int32_t n_systems,
+                     const int32_t* system_tile_start, const char* json,
+                     gph_batch** out);
+void gph_batch_destroy(gph_batch* b);
+/* refinement: 1 for IterativeRefinement (x float64), 0 for PBiCGStab (x
+ * float32); rows_per_wave: the ILU kernel's setting chosen at creation. */
+int gph_batch_info(const gph_batch* b, int32_t* n_systems, int32_t* refinement,
+                   int32_t* rows_per_wave);
+/* x (overwritten) and b (float32) in the matrix's tile-major order; stats
+ * (nullable) receives n_systems entries (iterations = outer iterations for
+ * refinement, inner_iterations = all PBiCGStab iterations); seconds
+ * (nullable) the wall time of the solve. */
+int gph_batch_solve(gph_batch* b, gph_type x_type, void* x, const void* b_f32,
+                    gph_solver_stats* stats, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif
