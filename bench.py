@@ -89,6 +89,66 @@ def copy_calibration(g, n=1 << 28, iters=5):
             "launches": iters, "host_ms_per_launch": dt * 1e3}
 
 
+def bench_batched(g, S=512, seed=3):
+    """Batched solves: S independent 2D 9-point convection-diffusion systems
+    with sizes log-uniform in [1e3, 2e5] rows (square grids of side
+    round(sqrt(n))), diffusion U[0.5, 2] and Peclet U[0, 20] per system from
+    the seed, 32x32-cell tiles; b = fl32(A x_true), x_true ~ U[-1,1); MPIR
+    (FP64 outer, FP32 PBiCGStab + ILU(0) inner, 1e-3) to 1e-10 for every
+    system, all systems in one block-diagonal matrix (gph_batch_*)."""
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), S))
+    systems = []
+    for k, nrow in enumerate(sizes):
+        side = max(2, int(round(np.sqrt(nrow))))
+        D, Pe = rng.uniform(0.5, 2.0), rng.uniform(0.0, 20.0)
+        r, c, v = g.generateConvectionDiffusion9Triplets(side, side, D, Pe, 1000 + k)
+        systems.append((side, side, r, c, v))
+    h, ts = g.batch_union(systems, (32, 32))
+    del systems
+    A = h.copyToTile(g.CRS)
+    n = h.nrows
+    t_host = time.time() - t0
+    xt = h.to_local(g.uniform_pm1(11, n).astype(np.float64))
+    b = g.DeviceVector.from_numpy(A.spmv(g.DeviceVector.from_numpy(xt)).numpy()
+                                  .astype(np.float32))
+    cfg = {"type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 20,
+           "extendedPrecisionType": "float64", "workingType": "float32",
+           "innerSolver": {"type": "PBiCGStab", "relResidual": 1e-3, "maxIterations": 200,
+                           "preconditioner": {"type": "ILU"}}}
+    t1 = time.time()
+    bt = g.Batch(A, ts, cfg)
+    t_setup = time.time() - t1
+    x = g.DeviceVector(n, np.float64)
+    times = []
+    for _ in range(2):  # the first solve captures the graph
+        stats, sec = bt.solve(x, b)
+        times.append(sec)
+    # true residual per system (tile-major rows of a system are contiguous)
+    r = A.residual(x, b, destType=g.FLOAT64, intermediateType=g.FLOAT64).numpy()
+    bb = b.numpy().astype(np.float64)
+    rs = h.export()["tile_start"][ts]
+    rel = [float(np.linalg.norm(r[rs[k]:rs[k + 1]]) / np.linalg.norm(bb[rs[k]:rs[k + 1]]))
+           for k in range(S)]
+    outer = np.array([s.iterations for s in stats])
+    inner = np.array([s.innerIterations for s in stats])
+    out = {"systems": S, "rows": n, "nnz_off": h.nnz_off, "tiles": int(ts[-1]),
+           "host_setup_s": t_host, "batch_setup_s": t_setup,
+           "rows_per_wave": bt.rows_per_wave,
+           "solve_ms": 1e3 * times[-1], "first_solve_ms": 1e3 * times[0],
+           "systems_per_s": S / times[-1], "rows_per_s": n / times[-1],
+           "all_converged": all(s.converged for s in stats),
+           "max_true_rel_residual": max(rel),
+           "outer_iterations_max": int(outer.max()), "inner_iterations_max": int(inner.max()),
+           "inner_iterations_mean": float(inner.mean())}
+    log(f"batched: {S} systems, {n} rows, {h.nnz_off} off-diagonals, {int(ts[-1])} tiles "
+        f"(host {t_host:.1f} s, batch setup {t_setup:.1f} s, rows/wave {bt.rows_per_wave}); "
+        f"solve {1e3*times[-1]:.1f} ms ({S/times[-1]:.0f} systems/s), outer <= "
+        f"{outer.max()}, inner <= {inner.max()}, max true rel. residual {max(rel):.2e}")
+    return out
+
+
 def bench_powerlaw(g, n=4194304, tiles=4096):
     """Unstructured SpMV + solve: power-law rows (lengths 5..64, exponent
     1.85, 90 % of columns within +-2000), seed 2, contiguous tiles; CRS SpMV
@@ -217,6 +277,7 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="SpMV, ILU apply and copy-calibration kernels only")
     ap.add_argument("--skip-powerlaw", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=64)
     args = ap.parse_args()
@@ -363,6 +424,7 @@ def main():
         log(f"CPU oracle: {per_it:.2f} s/iteration -> {est:.1f} s per solve (estimate)")
 
     powerlaw = None if args.skip_powerlaw else bench_powerlaw(g)
+    batched = None if args.skip_batched else bench_batched(g)
     serving = None if args.skip_serve else bench_serving(g, args.serve_requests)
 
     # the kernel with the larger per-launch time (both run twice per inner
@@ -385,6 +447,7 @@ def main():
         "spmv": spmv,
         "ilu_apply": ilu,
         "powerlaw": powerlaw,
+        "batched": batched,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -5,6 +5,7 @@
 #include <chrono>
 
 #include "config.hpp"
+#include "ilu_pack.hpp"
 #include "kernels.hpp"
 
 namespace graphene::batch {
@@ -117,8 +118,20 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   // rows-per-wave of the ILU kernel: the configured value, or the fastest of
   // a few candidates on this union (three timed applications each)
   const bool tune = configJson.find("rowsPerWave") == std::string::npos;
-  std::vector<int32_t> cands = tune ? std::vector<int32_t>{512, 1024, 2048, 4096}
-                                    : std::vector<int32_t>{ilu->rowsPerWave};
+  std::vector<int32_t> cands;
+  if (tune) {
+    // candidates that group the tiles differently (equal groupings are tried once)
+    std::vector<std::vector<int32_t>> seen;
+    for (int32_t c : {512, 1024, 2048, 4096}) {
+      std::vector<int32_t> gr = matrix::groupTiles(h, c);
+      if (std::find(seen.begin(), seen.end(), gr) == seen.end()) {
+        seen.push_back(std::move(gr));
+        cands.push_back(c);
+      }
+    }
+  } else {
+    cands.push_back(ilu->rowsPerWave);
+  }
   double best = 1e30;
   for (int32_t c : cands) {
     auto ic = std::make_shared<solver::ilu::Configuration>(*ilu);
