@@ -151,6 +151,30 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
       ilu_ = std::move(S);
     }
   }
+  // work of finished systems is skipped: ILU wave groups and SpMV slices
+  // whose systems have all finished return at once
+  ilu_->setSystemSkip(args_.st, sysRow_);
+  {
+    const gph::SpmvArgs a = A.spmvArgs(DType::F32);
+    if (a.numSlices > 0) {
+      auto sysOf = [&](int64_t r) {
+        return (int32_t)(std::upper_bound(sysRow_.begin(), sysRow_.end(), r) - sysRow_.begin() - 1);
+      };
+      std::vector<int32_t> ss(2 * (size_t)a.numSlices);
+      for (int64_t sl = 0; sl < a.numSlices; sl++) {
+        // rows of the slice: 64 consecutive rows, or (rows sorted by length
+        // inside 4096-row windows) the slice's window
+        int64_t r0 = sl * 64, r1 = std::min<int64_t>(n, r0 + 64);
+        if (a.perm) {
+          r0 = (r0 / 4096) * 4096;
+          r1 = std::min<int64_t>(n, r0 + 4096);
+        }
+        ss[2 * (size_t)sl] = sysOf(std::min<int64_t>(r0, n - 1));
+        ss[2 * (size_t)sl + 1] = sysOf(std::max<int64_t>(r0, r1 - 1));
+      }
+      sliceSys_.upload(ss.data(), ss.size(), st);
+    }
+  }
   // warm-up of the SpMV launch path outside any capture
   A.spmv(Vec{y_.get(), DType::F32}, Vec{v_.get(), DType::F32}, DType::F32, DType::F32, st);
   GPH_HIP(hipStreamSynchronize(st));
@@ -173,6 +197,10 @@ void BatchSolver::enqueueIteration(hipStream_t s) {
     a.out = y.ptr;
     a.state = glob;
     a.checkDone = 1;
+    if (sliceSys_) {
+      a.sysState = args_.st;
+      a.sliceSys = sliceSys_.as<int32_t>();
+    }
     gph::launchSpmv(a, DType::F32, DType::F32, DType::F32, DType::F32, DType::F32,
                     gph::kModeSpmv, gph::kRedNone, s);
   };

@@ -66,7 +66,7 @@ class BatchSolver {
   int32_t rowsPerWave_ = 0;
   gph::DevState innerInit_{}, outerInit_{};
   std::unique_ptr<matrix::solver::Solver> ilu_;
-  gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_;
+  gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_, sliceSys_;
   gph::DevBuf r_, r0_, p_, v_, y_, s_, z_, t_, d_, rw_, r64_;
   gph::BatchArgs args_;
   hipGraph_t graph_ = nullptr;

@@ -208,7 +208,13 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
     for (int i = 0; i < CH; i++) {
       const int g = c0 + i;
       const int slice = g * W + wave;
-      const bool sliceOk = g < g1 && slice < a.numSlices;
+      bool sliceOk = g < g1 && slice < a.numSlices;
+      if (sliceOk && a.sliceSys) {  // batched: skip slices of finished systems
+        bool done = true;
+        for (int k = a.sliceSys[2 * slice]; k <= a.sliceSys[2 * slice + 1]; k++)
+          done = done && a.sysState[k].done;
+        sliceOk = !done;
+      }
       const int slot = slice * 64 + lane;
       int row = 0, len = 0;
       if (sliceOk) {

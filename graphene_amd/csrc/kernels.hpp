@@ -51,6 +51,10 @@ struct SpmvArgs {
   const uint32_t* sliceColW = nullptr; // first column word of each slice
   const uint16_t* scol = nullptr;      // column words
   const void* sval = nullptr;
+  // optional (batched solves): slice s is skipped when the systems
+  // sliceSys[2s] .. sliceSys[2s+1] are all finished (sysState[k].done)
+  const DevState* sysState = nullptr;
+  const int32_t* sliceSys = nullptr;
 };
 
 enum SpmvMode : int32_t { kModeSpmv = 0, kModeResidual = 1 };
@@ -154,6 +158,10 @@ struct PackedArgs {
   int32_t maxRows = 0, maxLevels = 0, maxLevelEntries = 0;
   int32_t dilu = 0;  // diagonal-based ILU: x_i = t*invD_i / x_i + t*invD_i
   PackedDirArgs fwd, bwd;
+  // optional (batched solves): group g is skipped when the systems
+  // groupSys[2g] .. groupSys[2g+1] are all finished (sysState[k].done)
+  const DevState* sysState = nullptr;
+  const int32_t* groupSys = nullptr;
 };
 constexpr int kPackedThreads = 64;  // one wave per tile
 size_t packedSharedBytes(const PackedArgs& p, DType xt);

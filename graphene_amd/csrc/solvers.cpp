@@ -120,6 +120,27 @@ class IluSolver : public Solver {
                         pre, st, st ? 1 : 0, cfg_->diagonalBased, s);
   }
 
+  void setSystemSkip(const DevState* states,
+                     const std::vector<int64_t>& systemRowStart) override {
+    if (!packed_) return;
+    // systems of each wave group (groups and systems are row ranges)
+    const size_t G = groupRowStart_.size() - 1;
+    std::vector<int32_t> gs(2 * G);
+    size_t k = 0;
+    for (size_t g = 0; g < G; g++) {
+      while (k + 1 < systemRowStart.size() - 1 && systemRowStart[k + 1] <= groupRowStart_[g]) k++;
+      size_t h = k;
+      while (h + 1 < systemRowStart.size() - 1 && systemRowStart[h + 1] < groupRowStart_[g + 1])
+        h++;
+      gs[2 * g] = (int32_t)k;
+      gs[2 * g + 1] = (int32_t)h;
+    }
+    groupSys_.upload(gs.data(), gs.size(), A_.stream());
+    GPH_HIP(hipStreamSynchronize(A_.stream()));
+    pargs_.groupSys = groupSys_.as<int32_t>();
+    pargs_.sysState = states;
+  }
+
   size_t factorBytes() const {
     return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes() +
            fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes();
@@ -182,6 +203,7 @@ class IluSolver : public Solver {
     PackedILU P = packILU(A_.hostMatrix(), std::max(1, cfg_->rowsPerWave));
     if (!P.ok) return;
     groupStart_.upload(P.groupStart.data(), P.groupStart.size(), s);
+    groupRowStart_.assign(P.groupStart.begin(), P.groupStart.end());
     const void* src = ft_ == DType::F16 ? lu16_.get() : lu_.get();
     const size_t es = gph::dtypeSize(ft_);
     auto dir = [&](const PackedDirection& d, DevBuf& col, DevBuf& val, DevBuf& so,
@@ -225,7 +247,8 @@ class IluSolver : public Solver {
   }
 
   static constexpr size_t kMaxPackedShared = 156 * 1024;
-  DevBuf groupStart_;
+  DevBuf groupStart_, groupSys_;
+  std::vector<int64_t> groupRowStart_;
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_, invD16_;

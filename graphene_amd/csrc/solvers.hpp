@@ -46,6 +46,14 @@ class Solver {
   // right-hand side is formed on the fly by `pre` (p- or s-update of
   // PBiCGStab). Kernels skip their work once st->done is set.
   virtual bool supportsDeviceApply() const { return false; }
+  // Batched solves: the matrix is block-diagonal with system k on rows
+  // systemRowStart[k] .. systemRowStart[k+1]-1; work of systems whose
+  // states[k].done is set may be skipped by applyDevice (optional).
+  virtual void setSystemSkip(const gph::DevState* states,
+                             const std::vector<int64_t>& systemRowStart) {
+    (void)states;
+    (void)systemRowStart;
+  }
   virtual void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,
                            gph::DevState* st, hipStream_t s);
 
