@@ -71,6 +71,7 @@ _SIGS = {
     "gph_generate_powerlaw": ([C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_uint64, _I64P, _P, _P, _P], C.c_int),
     "gph_uniform_pm1": ([C.c_uint64, C.c_int64, _P], C.c_int),
     "gph_partition_grid3d": ([C.c_int64] * 6 + [_P, C.POINTER(C.c_int32)], C.c_int),
+    "gph_partition_graph": ([C.c_int64, C.c_int64, _P, _P, C.c_int32, _P], C.c_int),
     "gph_host_matrix_from_coo": ([C.c_int64, C.c_int64, _P, _P, _P, C.c_int32, _P, C.POINTER(_P)], C.c_int),
     "gph_host_matrix_poisson3d": ([C.c_int64] * 3 + [C.c_int32, _P, C.POINTER(_P)], C.c_int),
     "gph_host_matrix_convdiff3d27": ([C.c_int64] * 3 + [C.c_double, C.c_uint64, C.c_int32, _P, C.POINTER(_P)], C.c_int),
@@ -205,6 +206,17 @@ def partition_grid3d(nx, ny, nz, bx, by, bz):
     T = C.c_int32(0)
     _check(lib().gph_partition_grid3d(nx, ny, nz, bx, by, bz, _ptr(r2t), C.byref(T)))
     return r2t, T.value
+
+
+def partition_graph(nrows, rows, cols, num_tiles):
+    """Deterministic graph partition (recursive BFS bisection of the
+    symmetrized pattern, weights row length + 1): the row_to_tile array."""
+    r = np.ascontiguousarray(rows, np.int64)
+    c = np.ascontiguousarray(cols, np.int64)
+    out = np.empty(nrows, np.int32)
+    _check(lib().gph_partition_graph(int(nrows), len(r), _ptr(r), _ptr(c), int(num_tiles),
+                                     _ptr(out)))
+    return out
 
 
 # ----------------------------------------------------------------------------

@@ -6,10 +6,12 @@
 // and the same config keys (benchmark.cpp:163-183): "x" / "b" (number, or a
 // MatrixMarket file in global row order), "x0" (number, default 0),
 // "benchmark" ("solve" or "spmv"), "solver" (the solver configuration tree).
-// Differences: the matrix is split into N tiles as contiguous row ranges
-// (METIS is not available); --emulate is accepted and ignored (the run is on
-// the GPU); --profile writes profile.json (timings and solver statistics)
-// instead of Poplar profiling data.
+// Differences: the matrix is split into N tiles by the library's graph
+// partitioner (recursive BFS bisection; METIS is not available), or by
+// --partition contiguous|FILE (FILE: one tile id per row, global order);
+// --emulate is accepted and ignored (the run is on the GPU); --profile writes
+// profile.json (timings and solver statistics) instead of Poplar profiling
+// data.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -39,7 +41,7 @@ namespace host = graphene::matrix::host;
 namespace solver = graphene::matrix::solver;
 
 struct Cli {
-  std::string config, matrix, poisson, profile;
+  std::string config, matrix, poisson, profile, partition = "graph";
   long tiles = -1;
   bool emulate = false;
 };
@@ -47,7 +49,8 @@ struct Cli {
 [[noreturn]] void usage(const std::string& msg) {
   std::fprintf(stderr,
                "%s\nUsage: benchmark <config> -t,--tiles N [-e,--emulate B] "
-               "(-m,--matrix FILE | -p,--poisson nx,ny,nz) [-d,--profile DIR]\n",
+               "(-m,--matrix FILE | -p,--poisson nx,ny,nz) [-d,--profile DIR] "
+               "[--partition graph|contiguous|FILE]\n",
                msg.c_str());
   std::exit(2);
 }
@@ -76,6 +79,8 @@ Cli parseArgs(int argc, char** argv) {
       c.poisson = value(i, a, "--poisson");
     } else if (is(a, "-d", "--profile")) {
       c.profile = value(i, a, "--profile");
+    } else if (is(a, "--partition", "--partition")) {
+      c.partition = value(i, a, "--partition");
     } else if (a == "-h" || a == "--help") {
       usage("Graphene Benchmark Tool");
     } else if (!a.empty() && a[0] == '-') {
@@ -95,6 +100,9 @@ Cli parseArgs(int argc, char** argv) {
     usage("--matrix: file does not exist: " + c.matrix);
   if (!c.profile.empty() && !std::filesystem::is_directory(c.profile))
     usage("--profile: directory does not exist: " + c.profile);
+  if (c.partition != "graph" && c.partition != "contiguous" &&
+      !std::filesystem::is_regular_file(c.partition))
+    usage("--partition: graph, contiguous or an existing file: " + c.partition);
   return c;
 }
 
@@ -193,7 +201,6 @@ int run(const Cli& cli) {
   text << cf.rdbuf();
   const Json config = Json::parse(text.str());
   if (cli.emulate) std::printf("--emulate: ignored, running on the GPU\n");
-  std::printf("Using %ld tiles (contiguous row ranges)\n", cli.tiles);
 
   const double t0 = seconds();
   host::TripletMatrix coo;
@@ -204,8 +211,23 @@ int run(const Cli& cli) {
     parsePoisson(cli.poisson, nx, ny, nz);
     coo = host::generate3DPoissonTripletMatrix(nx, ny, nz);
   }
+  // partition (CRSHostMatrix.cpp:139-271 calls METIS here)
+  std::vector<int32_t> r2t;
+  const int32_t T = (int32_t)cli.tiles;
+  if (T > 1 && cli.partition == "graph") {
+    r2t = host::partitionGraph(coo.nrows, coo.rows, coo.cols, T);
+  } else if (T > 1 && cli.partition != "contiguous") {
+    std::ifstream pf(cli.partition);
+    r2t.resize((size_t)coo.nrows);
+    for (auto& t : r2t)
+      if (!(pf >> t) || t < 0 || t >= T)
+        throw std::runtime_error("--partition file: needs one tile id in [0, tiles) per row");
+  }
+  std::printf("Using %d tiles (%s partition)\n", T,
+              T == 1 ? "single" : cli.partition == "graph" || cli.partition == "contiguous"
+                                      ? cli.partition.c_str() : "file");
   std::shared_ptr<host::HostMatrix> h =
-      host::buildHostMatrix(std::move(coo), (int32_t)cli.tiles, nullptr);
+      host::buildHostMatrix(std::move(coo), T, r2t.empty() ? nullptr : r2t.data());
   DeviceMatrix A(h, MatrixFormat::CRS);
   const int64_t n = A.n();
   const double tSetup = seconds() - t0;

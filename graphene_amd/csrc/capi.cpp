@@ -172,6 +172,21 @@ int gph_partition_grid3d(int64_t nx, int64_t ny, int64_t nz, int64_t bx,
   });
 }
 
+int gph_partition_graph(int64_t nrows, int64_t nnz, const int64_t* rows,
+                        const int64_t* cols, int32_t num_tiles,
+                        int32_t* row_to_tile) {
+  return guard([&] {
+    need(row_to_tile, "row_to_tile");
+    if (nnz > 0) {
+      need(rows, "rows");
+      need(cols, "cols");
+    }
+    std::vector<int64_t> r(rows, rows + nnz), c(cols, cols + nnz);
+    auto v = hostns::partitionGraph(nrows, r, c, num_tiles);
+    std::memcpy(row_to_tile, v.data(), v.size() * sizeof(int32_t));
+  });
+}
+
 int gph_host_matrix_from_coo(int64_t nrows, int64_t nnz, const int64_t* rows,
                              const int64_t* cols, const float* vals,
                              int32_t num_tiles, const int32_t* row_to_tile,

@@ -354,6 +354,144 @@ std::vector<int32_t> partitionContiguous(const std::vector<int64_t>& rowPtr,
   return r2t;
 }
 
+std::vector<int32_t> partitionGraph(int64_t n, const std::vector<int64_t>& rows,
+                                    const std::vector<int64_t>& cols, int32_t numTiles) {
+  if (numTiles < 1) fail(kInvalid, "numTiles must be >= 1");
+  if (numTiles > n && n > 0) fail(kInvalid, "more tiles than rows");
+  if (rows.size() != cols.size()) fail(kInvalid, "rows and cols differ in length");
+  // symmetrized adjacency without self loops (CRS, ascending, unique)
+  std::vector<int64_t> deg((size_t)n + 1, 0), weight((size_t)n, 1);
+  for (size_t k = 0; k < rows.size(); k++) {
+    const int64_t r = rows[k], c = cols[k];
+    if (r < 0 || r >= n || c < 0 || c >= n) fail(kInvalid, "index out of range");
+    weight[(size_t)r]++;
+    if (r != c) {
+      deg[(size_t)r + 1]++;
+      deg[(size_t)c + 1]++;
+    }
+  }
+  for (int64_t i = 0; i < n; i++) deg[(size_t)i + 1] += deg[(size_t)i];
+  std::vector<int64_t> adj((size_t)deg[(size_t)n]);
+  {
+    std::vector<int64_t> pos(deg.begin(), deg.end() - 1);
+    for (size_t k = 0; k < rows.size(); k++) {
+      const int64_t r = rows[k], c = cols[k];
+      if (r == c) continue;
+      adj[(size_t)pos[(size_t)r]++] = c;
+      adj[(size_t)pos[(size_t)c]++] = r;
+    }
+  }
+  std::vector<int64_t> ptr((size_t)n + 1, 0);
+  {
+    int64_t w = 0;
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t a = deg[(size_t)i], b = deg[(size_t)i + 1];
+      std::sort(adj.begin() + a, adj.begin() + b);
+      ptr[(size_t)i] = w;
+      for (int64_t k = a; k < b; k++)
+        if (k == a || adj[(size_t)k] != adj[(size_t)k - 1]) adj[(size_t)w++] = adj[(size_t)k];
+    }
+    ptr[(size_t)n] = w;
+    adj.resize((size_t)w);
+  }
+  // recursive bisection over vertex sets; part id = set label
+  std::vector<int32_t> label((size_t)n, 0), r2t((size_t)n, 0);
+  std::vector<int64_t> order, queue, dist((size_t)n, -1);
+  struct Job {
+    std::vector<int64_t> verts;
+    int32_t first, parts;
+  };
+  std::vector<Job> stack;
+  {
+    Job all;
+    all.verts.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) all.verts[(size_t)i] = i;
+    all.first = 0;
+    all.parts = numTiles;
+    stack.push_back(std::move(all));
+  }
+  int32_t nextLabel = 1;
+  // BFS inside the set `lab` from `src`; returns the visit order
+  auto bfs = [&](int64_t src, int32_t lab, std::vector<int64_t>& out) {
+    out.clear();
+    queue.clear();
+    queue.push_back(src);
+    dist[(size_t)src] = 0;
+    for (size_t h = 0; h < queue.size(); h++) {
+      const int64_t v = queue[h];
+      out.push_back(v);
+      for (int64_t k = ptr[(size_t)v]; k < ptr[(size_t)v + 1]; k++) {
+        const int64_t u = adj[(size_t)k];
+        if (label[(size_t)u] == lab && dist[(size_t)u] < 0) {
+          dist[(size_t)u] = dist[(size_t)v] + 1;
+          queue.push_back(u);
+        }
+      }
+    }
+    for (int64_t v : out) dist[(size_t)v] = -1;
+  };
+  while (!stack.empty()) {
+    Job job = std::move(stack.back());
+    stack.pop_back();
+    if (job.parts == 1) {
+      for (int64_t v : job.verts) r2t[(size_t)v] = job.first;
+      continue;
+    }
+    const int32_t lab = label[(size_t)job.verts.front()];
+    int64_t total = 0;
+    for (int64_t v : job.verts) total += weight[(size_t)v];
+    const int32_t p1 = job.parts / 2;
+    const double target = (double)total * p1 / job.parts;
+    // pseudo-peripheral start: the last vertex reached from the first
+    // vertex, twice
+    int64_t src = job.verts.front();
+    for (int rep = 0; rep < 2; rep++) {
+      bfs(src, lab, order);
+      src = order.back();
+    }
+    // first part: BFS order from src, continued over the set's other
+    // components in index order, until the target weight
+    const int32_t labA = nextLabel++, labB = nextLabel++;
+    int64_t acc = 0;
+    Job a, b;
+    a.first = job.first;
+    a.parts = p1;
+    b.first = job.first + p1;
+    b.parts = job.parts - p1;
+    auto takeA = [&](int64_t v) {
+      label[(size_t)v] = labA;
+      a.verts.push_back(v);
+      acc += weight[(size_t)v];
+    };
+    const int64_t need = (int64_t)a.parts;  // every part needs a vertex
+    bfs(src, lab, order);
+    size_t oi = 0, vi = 0;
+    auto remainingVerts = [&]() { return (int64_t)job.verts.size() - (int64_t)a.verts.size(); };
+    while ((double)acc < target || (int64_t)a.verts.size() < need) {
+      if (remainingVerts() <= (int64_t)b.parts) break;  // leave one per part of B
+      if (oi < order.size()) {
+        const int64_t v = order[oi++];
+        if (label[(size_t)v] == lab) takeA(v);
+        continue;
+      }
+      // next component: the smallest unvisited vertex of the set
+      while (vi < job.verts.size() && label[(size_t)job.verts[vi]] != lab) vi++;
+      if (vi >= job.verts.size()) break;
+      bfs(job.verts[vi], lab, order);
+      oi = 0;
+    }
+    for (int64_t v : job.verts)
+      if (label[(size_t)v] == lab) {
+        label[(size_t)v] = labB;
+        b.verts.push_back(v);
+      }
+    std::sort(a.verts.begin(), a.verts.end());
+    stack.push_back(std::move(b));
+    stack.push_back(std::move(a));
+  }
+  return r2t;
+}
+
 // ---------------------------------------------------------------------------
 // COO -> CRS (CoordinateFormat.cpp:29-40 + CRSHostMatrix.cpp:87-137):
 // sorted by (row, col), diagonal split out, duplicates summed.

@@ -66,6 +66,14 @@ std::vector<int32_t> partitionGrid3D(int64_t nx, int64_t ny, int64_t nz,
                                      int32_t* numTiles);
 std::vector<int32_t> partitionContiguous(const std::vector<int64_t>& rowPtr,
                                          int32_t numTiles);
+// Graph partition of the symmetrized pattern of a COO matrix (rows/cols
+// 0-based, diagonal and duplicates allowed) into numTiles parts by
+// recursive breadth-first bisection: each split grows the first part from a
+// pseudo-peripheral vertex in BFS order until it holds its share of the
+// weight (row length + 1, as METIS gets in CRSHostMatrix.cpp:197).
+// Deterministic (neighbours visited in ascending order).
+std::vector<int32_t> partitionGraph(int64_t nrows, const std::vector<int64_t>& rows,
+                                    const std::vector<int64_t>& cols, int32_t numTiles);
 
 // Prepared host matrix (CRSHostMatrix of the reference).
 class HostMatrix {

@@ -101,6 +101,13 @@ int gph_uniform_pm1(uint64_t seed, int64_t n, float* out);
 int gph_partition_grid3d(int64_t nx, int64_t ny, int64_t nz, int64_t bx,
                          int64_t by, int64_t bz, int32_t* row_to_tile,
                          int32_t* num_tiles);
+/* Stand-in for the METIS partition of CRSHostMatrix.cpp:139-271 (METIS is
+ * not available): recursive breadth-first bisection of the symmetrized
+ * pattern of a COO matrix into num_tiles parts weighted by row length + 1;
+ * deterministic. row_to_tile receives nrows entries. */
+int gph_partition_graph(int64_t nrows, int64_t nnz, const int64_t* rows,
+                        const int64_t* cols, int32_t num_tiles,
+                        int32_t* row_to_tile);
 
 /* ---- host matrices -------------------------------------------------------- */
 /* HostMatrix(CRS, triplets, numTiles), lg/matrix/host/HostMatrix.cpp:31-44 and

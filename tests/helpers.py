@@ -41,6 +41,17 @@ def convdiff_case(nx, block=None, peclet=10.0, seed=1, ny=None, nz=None):
     return h, L
 
 
+def graph_partition_case(nx, T):
+    """27-point convection-diffusion on an nx^3 grid split by the library's
+    graph partitioner (the METIS stand-in) into T tiles."""
+    n = nx ** 3
+    rows, cols, vals = g.generateConvectionDiffusion27Triplets(nx, nx, nx, 10.0, 3)
+    r2t = g.partition_graph(n, rows, cols, T)
+    h = g.HostMatrix.from_coo(n, rows, cols, vals, T, r2t)
+    L = oracle_layout(n, rows, cols, vals, T, r2t)
+    return h, L
+
+
 def powerlaw_case(n, T=1, seed=2):
     rows, cols, vals = g.generatePowerLawTriplets(n, seed=seed)
     h = g.HostMatrix.from_coo(n, rows, cols, vals, T)

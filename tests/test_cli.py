@@ -55,6 +55,8 @@ def test_cli_usage_errors(tmp_path):
     assert r.returncode == 2 and "--profile" in r.stderr
     r = run(cfg, "--tiles=2", "--poisson=4,4,4", "--bogus")
     assert r.returncode == 2 and "unknown option" in r.stderr
+    r = run(cfg, "--tiles=2", "--poisson=4,4,4", "--partition", tmp_path / "none.txt")
+    assert r.returncode == 2 and "--partition" in r.stderr
 
 
 @pytest.mark.gpu
@@ -96,6 +98,21 @@ def test_cli_spmv_matrix_file(tmp_path):
     norm = float(line.split("= ")[1].split(",")[0])
     ref = np.linalg.norm(A @ np.full(n, 0.5))
     assert abs(norm - ref) <= 1e-5 * ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("part", ["graph", "contiguous", "file"])
+def test_cli_partitions(tmp_path, part):
+    """The three partition sources give the same converged solve (x = 1)."""
+    cfg = write(tmp_path / "c.json", SOLVE_CFG)
+    arg = part
+    if part == "file":
+        arg = tmp_path / "p.txt"
+        np.savetxt(arg, (np.arange(12 ** 3) // 300).astype(int), fmt="%d")
+    r = run(cfg, "-t", "6", "-p", "12,12,12", "--partition", arg, "-d", tmp_path)
+    assert r.returncode == 0, r.stderr
+    prof = json.load(open(tmp_path / "profile.json"))
+    assert prof["tiles"] == 6 and prof["converged"]
 
 
 @pytest.mark.gpu

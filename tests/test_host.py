@@ -9,7 +9,8 @@ import pytest
 
 import graphene_amd as g
 import oracle as o
-from helpers import convdiff_case, oracle_layout, poisson_case, powerlaw_case
+from helpers import (convdiff_case, graph_partition_case, oracle_layout, poisson_case,
+                     powerlaw_case)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -19,6 +20,7 @@ CASES = {
     "convdiff10_T1": lambda: convdiff_case(10),
     "convdiff12_blocks": lambda: convdiff_case(12, block=(4, 4, 6)),
     "powerlaw2000_T7": lambda: powerlaw_case(2000, T=7),
+    "convdiff11_graph9": lambda: graph_partition_case(11, 9),
 }
 
 
@@ -172,3 +174,28 @@ def test_config_rules():
         assert msg in str(e.value)
     assert g.check_config({"type": "restarter", "innerSolver": {"type": "PBiCGStab"}}) == \
         "Restarter of BiCGStab"
+
+
+def test_graph_partition_properties():
+    """The graph partitioner (METIS stand-in): every tile non-empty, weights
+    (row length + 1) balanced, a small edge cut on a grid, deterministic,
+    input errors reported."""
+    nx = 48
+    r, c, v = g.generateConvectionDiffusion9Triplets(nx, nx, 1.0, 5.0, 1)
+    n = nx * nx
+    p = g.partition_graph(n, r, c, 12)
+    assert p.min() == 0 and p.max() == 11
+    w = np.bincount(r, minlength=n) + 1
+    tw = np.bincount(p, weights=w, minlength=12)
+    assert tw.max() <= 1.05 * tw.mean() and tw.min() >= 0.95 * tw.mean()
+    offd = r != c
+    assert np.mean(p[r[offd]] != p[c[offd]]) < 0.15
+    assert np.array_equal(p, g.partition_graph(n, r, c, 12))
+    # one tile per row, and disconnected pieces
+    assert np.array_equal(np.sort(g.partition_graph(5, [0, 1, 2, 3, 4], [0, 1, 2, 3, 4], 5)),
+                          np.arange(5))
+    with pytest.raises(g.GrapheneError):
+        g.partition_graph(4, [0], [0], 5)
+    with pytest.raises(g.GrapheneError):
+        g.partition_graph(4, [0], [7], 2)
+
