@@ -268,13 +268,29 @@ __global__ __launch_bounds__(kTileThreads) void gsSweepKernel(
       const int i = t0 + li;
       X s = b[i];
       const int e = rowPtr[i + 1];
-      for (int a = rowPtr[i]; a < e; a++) {
-        const int j = col[a];
+      auto xOf = [&](int j) {
         const bool inTile = j >= t0 && j < t1;
         const bool fresh = inTile && (BACKWARD ? (j > i) : (j < i));
-        const X xj = fresh ? xs[j - t0] : xin[j];
-        s = s - cvt<X>(off[a]) * xj;
+        return fresh ? xs[j - t0] : xin[j];
+      };
+      int a = rowPtr[i];
+      // 8 entries per step: the column and value loads, then the x reads,
+      // then the subtractions in stored order
+      for (; a + 8 <= e; a += 8) {
+        int c[8];
+        VT v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          c[k] = col[a + k];
+          v[k] = off[a + k];
+        }
+        X xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) xv[k] = xOf(c[k]);
+#pragma unroll
+        for (int k = 0; k < 8; k++) s = s - cvt<X>(v[k]) * xv[k];
       }
+      for (; a < e; a++) s = s - cvt<X>(off[a]) * xOf(col[a]);
       xs[li] = s / cvt<X>(diag[i]);
     }
     __syncthreads();
