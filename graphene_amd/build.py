@@ -31,6 +31,7 @@ SOURCES = [
     "k_precond.hip",
     "k_ilu_packed.hip",
     "k_ilu_level.hip",
+    "k_ilu_rows.hip",
     "k_batch.hip",
 ]
 

@@ -324,9 +324,10 @@ ilu::Configuration::Configuration(const Json& c) {
   readInt(c, "rowsPerWave", rowsPerWave);
   if (const Json* j = c.get("substitution")) {
     substitution = asText(*j);
-    if (substitution != "level" && substitution != "packed" && substitution != "tile")
+    if (substitution != "level" && substitution != "packed" && substitution != "tile" &&
+        substitution != "rows")
       fail(kInvalid, "Invalid value for substitution: " + substitution +
-                         ". Allowed values are \"level\", \"packed\" or \"tile\"");
+                         ". Allowed values are \"packed\", \"rows\", \"level\" or \"tile\"");
   }
 }
 

@@ -109,8 +109,9 @@ struct Configuration : solver::Configuration {
   std::optional<gph::DType> factorType;
   // extension: rows per wave group of the substitution kernels
   int rowsPerWave = 1024;
-  // extension: substitution kernel: "packed" (product slabs), "level"
-  // (lane per row over level records), "tile" (row blocks)
+  // extension: substitution kernel: "packed" (product slabs), "rows" (lane
+  // per row, factor rows in registers), "level" (lane per row over level
+  // records), "tile" (row blocks)
   std::string substitution = "packed";
   explicit Configuration(const Json& c);
   std::string solverName() const override {

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <cstdio>
 
 namespace graphene::matrix {
 
@@ -355,6 +356,120 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
   }
   o.pairOff[(size_t)P] = (int64_t)o.bytes.size();
   o.bytes.resize(o.bytes.size() + (size_t)pad16(tail), 0);
+  return o;
+}
+
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
+  IluRowStream o;
+  const int32_t T = m.numTiles;
+  for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
+  if (4 * (int64_t)(o.maxRows + 1) > 65535) return o;
+  const uint16_t zero = (uint16_t)(4 * o.maxRows);
+  const uint8_t* luB = static_cast<const uint8_t*>(lu);
+  o.elemOff.assign((size_t)T + 1, 0);
+  o.hdrOff.assign((size_t)T + 1, 0);
+  o.metaOff.assign((size_t)T + 1, 0);
+  o.col.reserve((size_t)m.nnzOff() + (size_t)m.nrows + kRowsTail);
+  o.val.reserve(((size_t)m.nnzOff() + (size_t)m.nrows + kRowsTail) * (size_t)es);
+  o.meta.reserve((size_t)2 * m.nrows);
+  std::vector<int64_t> rows, off;
+  std::vector<int32_t> ents, order;
+  // one record: row r's entries [e0, e0 + l) (r < 0: an empty lane)
+  struct Rec {
+    int32_t r;
+    int64_t e0, l;
+    uint32_t flags;
+  };
+  std::vector<Rec> step;
+  for (int32_t t = 0; t < T; t++) {
+    o.elemOff[(size_t)t] = (uint32_t)o.col.size();
+    o.hdrOff[(size_t)t] = (int32_t)o.hdr.size();
+    o.metaOff[(size_t)t] = (int64_t)o.meta.size();
+    const int64_t t0 = m.tileStart[(size_t)t];
+    for (int dir = 0; dir < 2; dir++) {
+      const bool backward = dir == 1;
+      const int32_t nL = groupLevels(m, t, t + 1, backward);
+      for (int32_t L = 0; L < nL; L++) {
+        groupLevel(m, t, t + 1, L, backward, rows, ents, off);
+        const int32_t R = (int32_t)rows.size();
+        order.resize((size_t)R);
+        for (int32_t r = 0; r < R; r++) order[(size_t)r] = r;
+        auto len = [&](int32_t r) { return off[(size_t)r + 1] - off[(size_t)r]; };
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return len(a) > len(b); });
+        auto emit = [&]() {
+          int64_t W = 0, minLen = kRowsMaxLen, elems = 0;
+          for (const Rec& c : step) {
+            W = std::max(W, c.l);
+            minLen = std::min(minLen, c.l);
+            const int64_t loc = c.r >= 0 ? rows[(size_t)c.r] - t0 : 0;
+            o.meta.push_back((uint32_t)elems | (uint32_t)c.l << 11 | c.flags << 16 |
+                             (uint32_t)loc << 18);
+            for (int64_t e = 0; e < c.l; e++) {
+              const int32_t a = ents[(size_t)(off[(size_t)c.r] + c.e0 + e)];
+              o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0)));
+              o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
+            }
+            if (c.l % 2) {
+              o.col.push_back(zero);
+              o.val.insert(o.val.end(), (size_t)es, 0);
+            }
+            elems += c.l + (c.l % 2);
+          }
+          o.hdr.push_back((uint32_t)step.size() | (uint32_t)W << 7 | (uint32_t)minLen << 12 |
+                          (uint32_t)elems << 17 | (backward ? 1u << 31 : 0u));
+          step.clear();
+        };
+        // rows longer than kRowsMaxLen (first in `order`): chains of steps,
+        // the row's k-th chunk in the k-th step at the same lane (the lane
+        // carries the partial sum); free lanes take short rows
+        int32_t nLong = 0;
+        while (nLong < R && len(order[(size_t)nLong]) > kRowsMaxLen) nLong++;
+        int32_t next = nLong;  // next short row
+        for (int32_t c0 = 0; c0 < nLong; c0 += kRowsPerStep) {
+          const int32_t lanes = std::min<int32_t>(kRowsPerStep, nLong - c0);
+          const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
+          for (int64_t k = 0; k < chunks; k++) {
+            for (int32_t q = 0; q < lanes; q++) {
+              const int32_t r = order[(size_t)(c0 + q)];
+              const int64_t nk = (len(r) + kRowsMaxLen - 1) / kRowsMaxLen;
+              if (k < nk) {
+                const int64_t e0 = k * kRowsMaxLen;
+                step.push_back({r, e0, std::min<int64_t>(kRowsMaxLen, len(r) - e0),
+                                (k > 0 ? 1u : 0u) | (k + 1 < nk ? 2u : 0u)});
+              } else if (next < R) {
+                const int32_t s = order[(size_t)next++];
+                step.push_back({s, 0, len(s), 0u});
+              } else {
+                step.push_back({-1, 0, 0, 2u});
+              }
+            }
+            while ((int32_t)step.size() < kRowsPerStep && next < R) {
+              const int32_t s = order[(size_t)next++];
+              step.push_back({s, 0, len(s), 0u});
+            }
+            emit();
+          }
+        }
+        while (next < R) {
+          while ((int32_t)step.size() < kRowsPerStep && next < R) {
+            const int32_t s = order[(size_t)next++];
+            step.push_back({s, 0, len(s), 0u});
+          }
+          emit();
+        }
+      }
+    }
+    o.maxSteps = std::max<int32_t>(o.maxSteps, (int32_t)o.hdr.size() - o.hdrOff[(size_t)t]);
+    o.maxRecs = std::max<int64_t>(o.maxRecs, (int64_t)o.meta.size() - o.metaOff[(size_t)t]);
+    if (o.col.size() > (size_t)UINT32_MAX - 2 * kRowsTail) return o;
+  }
+  o.elemOff[(size_t)T] = (uint32_t)o.col.size();
+  o.hdrOff[(size_t)T] = (int32_t)o.hdr.size();
+  o.metaOff[(size_t)T] = (int64_t)o.meta.size();
+  o.col.insert(o.col.end(), (size_t)kRowsTail, zero);
+  o.val.insert(o.val.end(), (size_t)kRowsTail * es, 0);
+  o.ok = true;
   return o;
 }
 

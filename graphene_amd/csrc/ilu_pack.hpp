@@ -93,6 +93,50 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
                              int64_t maxStepBytes, int64_t ring, int64_t tail,
                              int64_t wAlign);
 
+// Row-per-lane stream of the ILU(0) factors (one wave per tile, two lanes
+// per row of the current level).
+//
+// Per tile, the forward then the backward substitution is a sequence of
+// steps; a step holds up to kRowsPerStep row records of one level (the rows
+// of a level are independent), longest first. A record is a row's entries
+// (forward: strictly lower in-tile entries ascending; backward: strictly
+// upper in-tile entries descending, the order ilu/Solver.cpp:380-442 visits
+// them), stored contiguously and padded to an even count: column as a
+// 16-bit byte offset into the tile's x (local id * 4) and the factor value,
+// both at the same element index of their streams. A row with more than
+// kRowsMaxLen entries is split into chunks placed at the same lane of
+// consecutive steps of its level; the lane carries the partial sum from one
+// chunk to the next (flags: continue from the carried sum; partial, the
+// row is not finished). Free lanes of such steps take other rows of the
+// level or empty records (partial, no entries).
+//   step header (u32): records R (bits 0-6), longest record W (7-11),
+//     shortest record (12-16), elements of the step (17-27), backward (31)
+//   record word (u32): element offset within the step (bits 0-10), entry
+//     count (11-15), continue (16), partial (17), local row id (18-31)
+// A tile's steps use elements [elemOff[t], elemOff[t+1]), headers
+// [hdrOff[t], hdrOff[t+1]) and record words [metaOff[t], metaOff[t+1]).
+// The element streams end with kRowsTail elements of padding (columns =
+// zero slot, values 0): a lane loads kRowsMaxLen elements from its record's
+// start.
+constexpr int kRowsMaxLen = 16;
+constexpr int kRowsPerStep = 32;
+constexpr int kRowsTail = 64;
+struct IluRowStream {
+  std::vector<uint16_t> col;       // byte offsets (4 * local id)
+  std::vector<uint8_t> val;        // factor values (es bytes each)
+  std::vector<uint32_t> elemOff;   // T+1
+  std::vector<int32_t> hdrOff;     // T+1
+  std::vector<int64_t> metaOff;    // T+1
+  std::vector<uint32_t> hdr;
+  std::vector<uint32_t> meta;
+  int32_t maxRows = 0;             // largest tile (the zero slot's id)
+  int32_t maxSteps = 0;            // most steps of one tile (both directions)
+  int64_t maxRecs = 0;             // most records of one tile
+  bool ok = false;  // false: ids or the stream too large for the formats
+};
+
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es);
+
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,
 // out[p*128 + 2l + (s&1)] = in[s*64 + l].

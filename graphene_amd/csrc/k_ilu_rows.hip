@@ -1,0 +1,320 @@
+// ILU(0) substitution, two lanes per row, factor rows in registers (gfx950).
+//
+// Same arithmetic as ilu/Solver.cpp:380-442 / :526-548 (block-Jacobi over
+// tiles, forward then backward substitution per tile). One wave per tile;
+// the tile's x (right-hand side with the fused PBiCGStab p- or s-update),
+// 1/diag, record words and step headers are staged in LDS. A step is up to
+// 32 row records of one level; lanes r and r + 32 own record r, entries 0-7
+// and 8-15: each reads the x values of its entries' columns from LDS (all
+// at once: they belong to earlier levels) and forms the products, then lane
+// r subtracts its products from the row's partial sum in the reference's
+// order and hands the sum to lane r + 32 (v_permlane32_swap), which
+// subtracts its products and finishes the row. Entries past the row's end
+// are masked to +0 products, which leave the sum unchanged. A row longer
+// than kRowsMaxLen is split over consecutive steps at the same lanes, which
+// carry the partial sum in a register. A step's LDS write precedes the next
+// step's reads in the wave's program order, and LDS executes a wave's
+// operations in order, so no barrier separates levels.
+//
+// Every lane's half record (kRowsMaxLen / 2 values and 16-bit column
+// offsets, loaded whatever the row's length) sits in a ring of D register
+// slots: the loads of step k + D are issued when step k is done, so memory
+// latency is covered by D steps of work. Steps past a tile's end have zero
+// headers (no records), so the loop needs no bounds tests and every ring
+// slot is refilled on every path.
+#include "kernels.hpp"
+
+namespace gph {
+
+struct RowsLds {
+  size_t xs, invd, meta, hdr, total;
+};
+
+// x (+ zero slot) | 1/diag | record words (+ 64 spare) | step headers
+// (+ 2D + 1 zero headers)
+__host__ __device__ inline RowsLds rowsLayout(const RowsArgs& p, size_t xsize, int D) {
+  RowsLds L;
+  size_t o = 0;
+  L.xs = o;
+  o += ((size_t)(p.maxRows + 1) * xsize + 15) & ~(size_t)15;
+  L.invd = o;
+  o += ((size_t)p.maxRows * xsize + 15) & ~(size_t)15;
+  L.meta = o;
+  o += (size_t)(p.maxRecs + 64) * 4;
+  L.hdr = o;
+  o += (size_t)(p.maxSteps + 2 * D + 1) * 4;
+  L.total = (o + 15) & ~(size_t)15;
+  return L;
+}
+
+template <typename FT>
+constexpr int rowsDepth() {
+  return sizeof(FT) == 8 ? 6 : 8;
+}
+
+size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt) {
+  const int D = ft == DType::F64 ? rowsDepth<double>() : rowsDepth<float>();
+  return rowsLayout(p, dtypeSize(xt), D).total;
+}
+
+// 16 bytes with 4-byte alignment: the row streams are dword aligned only.
+struct Words4 {
+  uint32_t w[4];
+};
+
+template <typename FT>
+__device__ __forceinline__ FT rowVal(const Words4* v, int j);
+template <>
+__device__ __forceinline__ float rowVal<float>(const Words4* v, int j) {
+  return __uint_as_float(v[j >> 2].w[j & 3]);
+}
+template <>
+__device__ __forceinline__ __half rowVal<__half>(const Words4* v, int j) {
+  const uint32_t w = v[j >> 3].w[(j >> 1) & 3];
+  return __ushort_as_half((unsigned short)((j & 1) ? w >> 16 : w & 0xffffu));
+}
+template <>
+__device__ __forceinline__ double rowVal<double>(const Words4* v, int j) {
+  const Words4& q = v[j >> 1];
+  const int h = (j & 1) * 2;
+  return __hiloint2double((int)q.w[h + 1], (int)q.w[h]);
+}
+
+// Buffer resource word 3 of gfx9 raw buffers (32-bit data format).
+static constexpr int kBufferFlags = 0x00020000;
+
+__device__ __forceinline__ Words4 bufLoad16(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return Words4{{(uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3]}};
+}
+
+// Lanes 0-31 of v moved to lanes 32-63 (lanes 0-31 keep their own), and
+// lanes 32-63 moved to lanes 0-31: gfx950 v_permlane32_swap.
+__device__ __forceinline__ float lowToHigh(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[0]);
+}
+__device__ __forceinline__ float highToLow(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ double lowToHigh(double v) {
+  const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
+  return __hiloint2double((int)__builtin_amdgcn_permlane32_swap(hi, hi, false, false)[0],
+                          (int)__builtin_amdgcn_permlane32_swap(lo, lo, false, false)[0]);
+}
+__device__ __forceinline__ double highToLow(double v) {
+  const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
+  return __hiloint2double((int)__builtin_amdgcn_permlane32_swap(hi, hi, false, false)[1],
+                          (int)__builtin_amdgcn_permlane32_swap(lo, lo, false, false)[1]);
+}
+
+// Copies n words to LDS and zero-fills up to nz, kStageW loads in flight.
+static constexpr int kStageW = 8;
+__device__ __forceinline__ void stageWords(uint32_t* dst, const uint32_t* __restrict__ src,
+                                           int n, int nz) {
+  const int lane = threadIdx.x;
+  for (int q0 = 0; q0 < nz; q0 += kStageW * kRowsThreads) {
+    uint32_t a[kStageW];
+#pragma unroll
+    for (int k = 0; k < kStageW; k++) {
+      const int q = q0 + k * kRowsThreads + lane;
+      a[k] = n > 0 ? src[q < n ? q : n - 1] : 0u;  // clamped: loads unconditional
+    }
+#pragma unroll
+    for (int k = 0; k < kStageW; k++) {
+      const int q = q0 + k * kRowsThreads + lane;
+      if (q < nz) dst[q] = q < n ? a[k] : 0u;
+    }
+  }
+}
+
+template <typename FT, typename X, typename F>
+__device__ __forceinline__ void stageRows(int t0, int nt, const FT* __restrict__ invD, X* xs,
+                                          X* invd, F f, X* wb) {
+  const int lane = threadIdx.x;
+  for (int q0 = 0; q0 < nt; q0 += kStageW * kRowsThreads) {
+    X a[kStageW];
+    FT dv[kStageW];
+#pragma unroll
+    for (int k = 0; k < kStageW; k++) {
+      const int q = q0 + k * kRowsThreads + lane;
+      const int i = t0 + (q < nt ? q : nt - 1);
+      a[k] = f(i);
+      dv[k] = invD[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kStageW; k++) {
+      const int q = q0 + k * kRowsThreads + lane;
+      if (q < nt) {
+        if (wb) wb[t0 + q] = a[k];
+        xs[q] = a[k];
+        invd[q] = cvt<X>(dv[k]);
+      }
+    }
+  }
+}
+
+template <typename FT, typename X>
+__global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
+    RowsArgs p, const FT* __restrict__ invD, const X* __restrict__ rhs,
+    X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
+  constexpr int D = rowsDepth<FT>();
+  constexpr int kHalf = kRowsMaxLen / 2;                // entries per lane
+  constexpr int NV = (kHalf * (int)sizeof(FT)) / 16;    // value words per lane
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (checkDone && st->done) return;
+  const RowsLds L = rowsLayout(p, sizeof(X), D);
+  X* xs = reinterpret_cast<X*>(smem + L.xs);
+  X* invd = reinterpret_cast<X*>(smem + L.invd);
+  uint32_t* meta = reinterpret_cast<uint32_t*>(smem + L.meta);
+  uint32_t* hdr = reinterpret_cast<uint32_t*>(smem + L.hdr);
+  const int tile = blockIdx.x, lane = threadIdx.x;
+  const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
+  const int h0 = p.hdrOff[tile], nS = p.hdrOff[tile + 1] - h0;
+
+  const int64_t m0 = p.metaOff[tile];
+  const int nRec = (int)(p.metaOff[tile + 1] - m0);
+  stageWords(meta, p.meta + m0, nRec, nRec + 64);
+  stageWords(hdr, p.hdr + h0, nS, nS + 2 * D + 1);
+  const X* r = static_cast<const X*>(pre.r);
+  X* pv = static_cast<X*>(pre.p);
+  const X* vv = static_cast<const X*>(pre.v);
+  if (pre.kind == kPreP) {
+    if (st->iter != 0) {
+      const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+      const X omega = (X)st->omega;
+      stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); },
+                pv);
+    } else {
+      stageRows(t0, nt, invD, xs, invd, [&](int i) { return pv[i]; }, (X*)nullptr);
+    }
+  } else if (pre.kind == kPreS) {
+    const X alpha = (X)st->alpha;
+    stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] - alpha * vv[i]; }, pv);
+  } else {
+    stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
+  }
+  for (int q = nt + lane; q <= p.maxRows; q += kRowsThreads) xs[q] = (X)0;  // zero slot
+  __syncthreads();
+
+  // the tile's streams (and the read-ahead past their end) as buffers
+  const uint32_t nElem = p.elemOff[tile + 1] - p.elemOff[tile] + kRowsTail;
+  const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<FT*>(static_cast<const FT*>(p.val) + p.elemOff[tile]), 0,
+      (int)(nElem * sizeof(FT)), kBufferFlags);
+  const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint16_t*>(p.col + p.elemOff[tile]), 0, (int)(nElem * 2u), kBufferFlags);
+  const unsigned char* xb = reinterpret_cast<const unsigned char*>(xs);
+  const int dilu = p.dilu;
+  const int half = lane >> 5, rl = lane & 31;  // half of record rl's entries
+  Words4 vr[D][NV], cr[D];
+  uint32_t mr[D], hr[D];
+  X carry = (X)0;  // partial sum of a row split over consecutive steps
+  int ldRow = 0;        // first record word of the next step to load
+  uint32_t ldElem = 0;  // its first element
+  // loads of a step's records into ring slot u; lanes past the step's
+  // records take an offset outside the buffer: their loads return zeros
+  // without a memory access
+  auto issue = [&](int u, uint32_t h, uint32_t m) {
+    const int R = (int)(h & 127u);
+    const uint32_t e = rl < R ? (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
+    const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+      vr[u][i] = bufLoad16(vrs, vo, (int)(ldElem * sizeof(FT)) + 16 * i);
+    cr[u] = bufLoad16(crs, co, (int)(ldElem * 2u));
+    mr[u] = m;
+    hr[u] = h;
+    ldRow += R;
+    ldElem += (h >> 17) & 0x7ffu;
+  };
+  // first fill, slot by slot in the steady state's order (the compiler's
+  // load waits then hold for every pass)
+#pragma unroll
+  for (int u = 0; u < D; u++) {
+    const uint32_t h = __builtin_amdgcn_readfirstlane(hdr[u]);
+    issue(u, h, meta[ldRow + rl]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  for (int k0 = 0; k0 < nS; k0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; u++) {
+      // descriptors of the step that refills this slot
+      const uint32_t hn = __builtin_amdgcn_readfirstlane(hdr[k0 + u + D]);
+      const uint32_t mn = meta[ldRow + rl];
+      const uint32_t h = hr[u], m = mr[u];
+      const int R = (int)(h & 127u);
+      const bool bwd = (h >> 31) != 0;
+      const int loc = (int)(m >> 18);
+      const int lenH = (int)((m >> 11) & 31u) - kHalf * half;  // entries in this half
+      const bool cont = (m >> 16) & 1u, partial = (m >> 17) & 1u;
+      X xv[kHalf];
+#pragma unroll
+      for (int j = 0; j < kHalf; j++) {
+        const uint32_t w = cr[u].w[j >> 1];
+        const uint32_t bo = (j & 1) ? w >> 16 : w & 0xffffu;  // 4 * local id
+        xv[j] = *reinterpret_cast<const X*>(xb + bo * (sizeof(X) / 4));
+      }
+      const X pX = xs[loc], pD = invd[loc];
+      // all reads issued before the first product waits on one of them
+      __builtin_amdgcn_sched_barrier(0);
+      X prod[kHalf];
+#pragma unroll
+      for (int j = 0; j < kHalf; j++) {
+        prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
+        prod[j] = j < lenH ? prod[j] : (X)0;
+      }
+      // entries 0-7 in the low lanes, then 8-15 in the high lanes, in order
+      X acc = cont ? carry : bwd ? (X)0 : pX;
+#pragma unroll
+      for (int j = 0; j < kHalf; j++) acc = acc - prod[j];
+      acc = lowToHigh(acc);
+#pragma unroll
+      for (int j = 0; j < kHalf; j++) acc = acc - prod[j];
+      X res;
+      if (bwd)
+        res = dilu ? pX + acc * pD : (pX + acc) * pD;
+      else
+        res = dilu ? acc * pD : acc;
+      carry = highToLow(acc);
+      if (half && rl < R && !partial) xs[loc] = res;
+      issue(u, hn, mn);
+    }
+  }
+  __syncthreads();
+  for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xs[q];
+}
+
+template <typename FT, typename X>
+static void rowsTyped(const RowsArgs& p, const void* invD, const void* rhs, void* out,
+                      const PreOp& pre, DevState* st, int checkDone, hipStream_t s) {
+  const size_t shm = rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
+  auto k = iluRowsKernel<FT, X>;
+  if (shm > 64 * 1024)
+    GPH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shm));
+  hipLaunchKernelGGL(k, dim3(p.T), dim3(kRowsThreads), shm, s, p,
+                     static_cast<const FT*>(invD), static_cast<const X*>(rhs),
+                     static_cast<X*>(out), pre, st, checkDone);
+  GPH_HIP(hipGetLastError());
+}
+
+void launchIluRows(const RowsArgs& p, const void* invD, DType ft, const void* rhs,
+                   void* out, DType xt, const PreOp& pre, DevState* st,
+                   int checkDone, hipStream_t s) {
+  if (p.T <= 0) return;
+  if (ft == DType::F32 && xt == DType::F32)
+    rowsTyped<float, float>(p, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F16 && xt == DType::F32)
+    rowsTyped<__half, float>(p, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F64 && xt == DType::F64)
+    rowsTyped<double, double>(p, invD, rhs, out, pre, st, checkDone, s);
+  else if (ft == DType::F32 && xt == DType::F64)
+    rowsTyped<float, double>(p, invD, rhs, out, pre, st, checkDone, s);
+  else
+    fail(kUnsupported, std::string("unsupported row-per-lane ILU types: factors ") +
+                           dtypeName(ft) + ", vectors " + dtypeName(xt));
+}
+
+}  // namespace gph

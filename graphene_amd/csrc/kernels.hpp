@@ -193,6 +193,28 @@ size_t levelSharedBytes(const LevelArgs& p, DType xt);
 void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
                     DType xt, const PreOp& pre, DevState* st, int checkDone,
                     hipStream_t s);
+// Row-per-lane ILU(0) substitution (k_ilu_rows.hip): one wave per tile,
+// one lane per row of the current level, each lane's factor row loaded
+// into a register ring several steps ahead (stream format in ilu_pack.hpp).
+struct RowsArgs {
+  int32_t T = 0, maxRows = 0, maxSteps = 0, maxRecs = 0;
+  int32_t dilu = 0;
+  const int32_t* tileStart = nullptr;  // T+1
+  const uint32_t* elemOff = nullptr;   // T+1 element offsets of the tiles' streams
+  const int32_t* hdrOff = nullptr;     // T+1 offsets of the tiles' step headers
+  const uint32_t* hdr = nullptr;       // step headers
+  const int64_t* metaOff = nullptr;    // T+1 offsets of the tiles' record words
+  const uint32_t* meta = nullptr;      // record words
+  const uint16_t* col = nullptr;       // column byte offsets (4 * local id)
+  const void* val = nullptr;           // factor values
+};
+constexpr int kRowsThreads = 64;
+constexpr int kRowsMaxLen = 16;  // entries loaded per record (longest record)
+constexpr int kRowsTail = 64;    // padding elements after the streams
+size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt);
+void launchIluRows(const RowsArgs& p, const void* invD, DType ft, const void* rhs,
+                   void* out, DType xt, const PreOp& pre, DevState* st,
+                   int checkDone, hipStream_t s);
 // Batched solves over a block-diagonal union of systems (k_batch.hip):
 // chunks of rows, each inside one system; per-system solver states.
 constexpr int kBatchThreads = 256;

@@ -107,6 +107,10 @@ class IluSolver : public Solver {
       lu = lu16_.get();
       inv = invD16_.get();
     }
+    if (rows_ && gph::rowsSharedBytes(rargs_, ft, out.type) <= kMaxPackedShared) {
+      gph::launchIluRows(rargs_, inv, ft, rhs.ptr, out.ptr, out.type, pre, st, st ? 1 : 0, s);
+      return;
+    }
     if (level_ && gph::levelSharedBytes(largs_, out.type) <= kMaxPackedShared) {
       gph::launchIluLevel(largs_, ft, rhs.ptr, out.ptr, out.type, pre, st, st ? 1 : 0, s);
       return;
@@ -143,7 +147,8 @@ class IluSolver : public Solver {
 
   size_t factorBytes() const {
     return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes() +
-           fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes();
+           fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes() +
+           rcol_.bytes() + rval_.bytes() + rmeta_.bytes();
   }
 
  private:
@@ -164,7 +169,9 @@ class IluSolver : public Solver {
       gph::launchCast(invD_.get(), work, invD16_.get(), DType::F16, n, s);
     }
     if (cfg_->substitution == "level") buildLevel(s);
-    if (!level_ && cfg_->substitution != "tile") buildPacked(s);  // also the level fallback
+    if (cfg_->substitution == "rows") buildRows(s);
+    // also the fallback of the other kernels
+    if (!level_ && !rows_ && cfg_->substitution != "tile") buildPacked(s);
     GPH_HIP(hipStreamSynchronize(s));
   }
 
@@ -196,6 +203,44 @@ class IluSolver : public Solver {
     largs_.maxRows = L.maxRows;
     largs_.dilu = cfg_->diagonalBased ? 1 : 0;
     level_ = true;
+  }
+
+  // Row-per-lane stream of the factors (built on the host from the device
+  // factors).
+  void buildRows(hipStream_t s) {
+    static_assert(kRowsMaxLen == gph::kRowsMaxLen && kRowsTail == gph::kRowsTail,
+                  "record length and padding of stream and kernel");
+    const int64_t nnz = A_.nnzOff(), n = A_.n();
+    const size_t es = gph::dtypeSize(ft_);
+    std::vector<uint8_t> lu((size_t)std::max<int64_t>(nnz, 1) * es);
+    const void* luD = ft_ == DType::F16 ? lu16_.get() : lu_.get();
+    GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipStreamSynchronize(s));
+    (void)n;
+    IluRowStream R = packIluRows(A_.hostMatrix(), lu.data(), (int)es);
+    if (!R.ok) return;
+    rcol_.upload(R.col.data(), R.col.size(), s);
+    rval_.upload(R.val.data(), R.val.size(), s);
+    relem_.upload(R.elemOff.data(), R.elemOff.size(), s);
+    rhoff_.upload(R.hdrOff.data(), R.hdrOff.size(), s);
+    rhdr_.upload(R.hdr.data(), std::max<size_t>(R.hdr.size(), 1), s);
+    rmeta_.upload(R.meta.data(), std::max<size_t>(R.meta.size(), 1), s);
+    rmoff_.upload(R.metaOff.data(), R.metaOff.size(), s);
+    rargs_.T = A_.numTiles();
+    rargs_.tileStart = A_.tileArgs().tileStart;
+    rargs_.maxRows = R.maxRows;
+    rargs_.maxSteps = R.maxSteps;
+    if (R.maxRecs > (1 << 30)) return;
+    rargs_.maxRecs = (int32_t)R.maxRecs;
+    rargs_.metaOff = rmoff_.as<int64_t>();
+    rargs_.dilu = cfg_->diagonalBased ? 1 : 0;
+    rargs_.elemOff = relem_.as<uint32_t>();
+    rargs_.hdrOff = rhoff_.as<int32_t>();
+    rargs_.hdr = rhdr_.as<uint32_t>();
+    rargs_.meta = rmeta_.as<uint32_t>();
+    rargs_.col = rcol_.as<uint16_t>();
+    rargs_.val = rval_.get();
+    rows_ = true;
   }
 
   // Level-major packed copy of the factors for the one-wave-per-tile kernel.
@@ -252,7 +297,9 @@ class IluSolver : public Solver {
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_, invD16_;
-  bool packed_ = false, level_ = false;
+  bool packed_ = false, level_ = false, rows_ = false;
+  gph::RowsArgs rargs_;
+  DevBuf rcol_, rval_, relem_, rhoff_, rhdr_, rmeta_, rmoff_;
   gph::PackedArgs pargs_;
   gph::LevelArgs largs_;
   DevBuf lbytes_, lgoff_, lsteps_, lgstart_;

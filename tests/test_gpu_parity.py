@@ -140,7 +140,7 @@ def test_ldu_spmv_bit_exact(case, sliced):
     assert np.array_equal(r.numpy(), o.ldu_residual_mixed(U, x64, b)[0])
 
 
-@pytest.mark.parametrize("sub", ["packed", "level", "tile"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level", "tile"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_ilu0_apply_bit_exact(case, ftype, sub):
     """ILU(0) factorization + substitution equal the oracle's, with each
@@ -158,7 +158,7 @@ def test_ilu0_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_dilu_apply_bit_exact(case, ftype, sub):
     """DILU (diagonalBased) factorization + substitution equal the oracle's."""
@@ -175,7 +175,7 @@ def test_dilu_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
 def test_ilu0_fp16_factors_bit_exact(case, sub):
     """FP16-stored factors: the FP32 factors rounded to FP16, applied in FP32."""
     name, h, L, A = case
@@ -285,20 +285,21 @@ def test_pbicgstab_f64_and_mpir(case):
 
 @pytest.mark.parametrize("ftype", ["float32", "float16"])
 def test_pbicgstab_level_substitution_identical(case, ftype):
-    """PBiCGStab with the lane-per-row ILU(0) kernel (its fused p- and
+    """PBiCGStab with the lane-per-row ILU(0) kernels (their fused p- and
     s-updates included) gives the same iterates, bit for bit, as with the
     product-slab kernel."""
     name, h, L, A = case
     xt, b = rhs_for(h, L, seed=5)
     outs = []
-    for sub in ("packed", "level"):
+    for sub in ("packed", "level", "rows"):
         cfg = _pbicgstab_cfg(1e-6)
         cfg["preconditioner"] = {"type": "ILU", "factorType": ftype, "substitution": sub}
         x = dev(np.zeros(h.nrows, np.float32))
         st = g.Solver(A, cfg).solve(x, dev(b))
         outs.append((st.iterations, x.numpy()))
-    assert outs[0][0] == outs[1][0]
-    assert np.array_equal(outs[0][1], outs[1][1])
+    for it, xv in outs[1:]:
+        assert it == outs[0][0]
+        assert np.array_equal(xv, outs[0][1])
 
 
 def test_solver_deterministic(case):
