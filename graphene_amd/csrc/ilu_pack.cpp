@@ -364,7 +364,7 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
   const int32_t T = m.numTiles;
   for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
   if (4 * (int64_t)(o.maxRows + 1) > 65535) return o;
-  const uint16_t zero = (uint16_t)(4 * o.maxRows);
+  const uint16_t zero = 0;  // slot 0 holds +0; row i of a tile is slot i + 1
   const uint8_t* luB = static_cast<const uint8_t*>(lu);
   o.elemOff.assign((size_t)T + 1, 0);
   o.hdrOff.assign((size_t)T + 1, 0);
@@ -402,19 +402,20 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
           for (const Rec& c : step) {
             W = std::max(W, c.l);
             minLen = std::min(minLen, c.l);
-            const int64_t loc = c.r >= 0 ? rows[(size_t)c.r] - t0 : 0;
-            o.meta.push_back((uint32_t)elems | (uint32_t)c.l << 11 | c.flags << 16 |
-                             (uint32_t)loc << 18);
+            const int64_t slot = c.r >= 0 ? rows[(size_t)c.r] - t0 + 1 : 0;
+            const int64_t padded = (c.l + kRowsPad - 1) / kRowsPad * kRowsPad;
+            o.meta.push_back((uint32_t)elems | (uint32_t)padded << 11 | c.flags << 16 |
+                             (uint32_t)slot << 18);
             for (int64_t e = 0; e < c.l; e++) {
               const int32_t a = ents[(size_t)(off[(size_t)c.r] + c.e0 + e)];
-              o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0)));
+              o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
               o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
             }
-            if (c.l % 2) {
+            for (int64_t e = c.l; e < padded; e++) {
               o.col.push_back(zero);
               o.val.insert(o.val.end(), (size_t)es, 0);
             }
-            elems += c.l + (c.l % 2);
+            elems += padded;
           }
           o.hdr.push_back((uint32_t)step.size() | (uint32_t)W << 7 | (uint32_t)minLen << 12 |
                           (uint32_t)elems << 17 | (backward ? 1u << 31 : 0u));

@@ -101,24 +101,27 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
 // of a level are independent), longest first. A record is a row's entries
 // (forward: strictly lower in-tile entries ascending; backward: strictly
 // upper in-tile entries descending, the order ilu/Solver.cpp:380-442 visits
-// them), stored contiguously and padded to an even count: column as a
-// 16-bit byte offset into the tile's x (local id * 4) and the factor value,
-// both at the same element index of their streams. A row with more than
-// kRowsMaxLen entries is split into chunks placed at the same lane of
-// consecutive steps of its level; the lane carries the partial sum from one
-// chunk to the next (flags: continue from the carried sum; partial, the
-// row is not finished). Free lanes of such steps take other rows of the
-// level or empty records (partial, no entries).
+// them), stored contiguously and padded with zero entries to a multiple of
+// kRowsPad: column as a 16-bit byte offset into the tile's x slots (4 *
+// slot; slot 0 holds +0, row i of the tile is slot i + 1) and the factor
+// value, both at the same element index of their streams. A zero entry
+// (value +0, column slot 0) adds a +0 product, which leaves every partial
+// sum unchanged. A row with more than kRowsMaxLen entries is split into
+// chunks placed at the same lanes of consecutive steps of its level; the
+// lanes carry the partial sum from one chunk to the next (flags: continue
+// from the carried sum; partial, the row is not finished). Free lanes of
+// such steps take other rows of the level or empty records (partial, no
+// entries).
 //   step header (u32): records R (bits 0-6), longest record W (7-11),
 //     shortest record (12-16), elements of the step (17-27), backward (31)
-//   record word (u32): element offset within the step (bits 0-10), entry
-//     count (11-15), continue (16), partial (17), local row id (18-31)
+//   record word (u32): element offset within the step (bits 0-10), padded
+//     entry count (11-15), continue (16), partial (17), the row's slot
+//     (18-31)
 // A tile's steps use elements [elemOff[t], elemOff[t+1]), headers
 // [hdrOff[t], hdrOff[t+1]) and record words [metaOff[t], metaOff[t+1]).
-// The element streams end with kRowsTail elements of padding (columns =
-// zero slot, values 0): a lane loads kRowsMaxLen elements from its record's
-// start.
+// The element streams end with kRowsTail elements of zero entries.
 constexpr int kRowsMaxLen = 16;
+constexpr int kRowsPad = 8;
 constexpr int kRowsPerStep = 32;
 constexpr int kRowsTail = 64;
 struct IluRowStream {

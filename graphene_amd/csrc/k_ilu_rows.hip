@@ -9,15 +9,18 @@
 // at once: they belong to earlier levels) and forms the products, then lane
 // r subtracts its products from the row's partial sum in the reference's
 // order and hands the sum to lane r + 32 (v_permlane32_swap), which
-// subtracts its products and finishes the row. Entries past the row's end
-// are masked to +0 products, which leave the sum unchanged. A row longer
+// subtracts its products and finishes the row. Records are padded with
+// zero entries (value +0, the +0 slot) to a multiple of 8, and the lanes
+// past a record's padded end or past the step's records load nothing (their
+// buffer offsets are out of range, so the loads return +0 values and slot
+// 0): their +0 products leave the sum unchanged. A row longer
 // than kRowsMaxLen is split over consecutive steps at the same lanes, which
 // carry the partial sum in a register. A step's LDS write precedes the next
 // step's reads in the wave's program order, and LDS executes a wave's
 // operations in order, so no barrier separates levels.
 //
 // Every lane's half record (kRowsMaxLen / 2 values and 16-bit column
-// offsets, loaded whatever the row's length) sits in a ring of D register
+// offsets) sits in a ring of D register
 // slots: the loads of step k + D are issued when step k is done, so memory
 // latency is covered by D steps of work. Steps past a tile's end have zero
 // headers (no records), so the loop needs no bounds tests and every ring
@@ -30,15 +33,15 @@ struct RowsLds {
   size_t xs, invd, meta, hdr, total;
 };
 
-// x (+ zero slot) | 1/diag | record words (+ 64 spare) | step headers
-// (+ 2D + 1 zero headers)
+// x (slot 0 = +0, row i = slot i + 1) | 1/diag (same slots) | record words
+// (+ 64 spare) | step headers (+ 2D + 1 zero headers)
 __host__ __device__ inline RowsLds rowsLayout(const RowsArgs& p, size_t xsize, int D) {
   RowsLds L;
   size_t o = 0;
   L.xs = o;
   o += ((size_t)(p.maxRows + 1) * xsize + 15) & ~(size_t)15;
   L.invd = o;
-  o += ((size_t)p.maxRows * xsize + 15) & ~(size_t)15;
+  o += ((size_t)(p.maxRows + 1) * xsize + 15) & ~(size_t)15;
   L.meta = o;
   o += (size_t)(p.maxRecs + 64) * 4;
   L.hdr = o;
@@ -148,8 +151,8 @@ __device__ __forceinline__ void stageRows(int t0, int nt, const FT* __restrict__
       const int q = q0 + k * kRowsThreads + lane;
       if (q < nt) {
         if (wb) wb[t0 + q] = a[k];
-        xs[q] = a[k];
-        invd[q] = cvt<X>(dv[k]);
+        xs[q + 1] = a[k];
+        invd[q + 1] = cvt<X>(dv[k]);
       }
     }
   }
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   } else {
     stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
   }
-  for (int q = nt + lane; q <= p.maxRows; q += kRowsThreads) xs[q] = (X)0;  // zero slot
+  if (lane == 0) xs[0] = (X)0;  // the zero slot
   __syncthreads();
 
   // the tile's streams (and the read-ahead past their end) as buffers
@@ -218,7 +221,8 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   // without a memory access
   auto issue = [&](int u, uint32_t h, uint32_t m) {
     const int R = (int)(h & 127u);
-    const uint32_t e = rl < R ? (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
+    const bool has = rl < R && kHalf * half < (int)((m >> 11) & 31u);
+    const uint32_t e = has ? (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
     const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
 #pragma unroll
     for (int i = 0; i < NV; i++)
@@ -246,8 +250,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       const uint32_t h = hr[u], m = mr[u];
       const int R = (int)(h & 127u);
       const bool bwd = (h >> 31) != 0;
-      const int loc = (int)(m >> 18);
-      const int lenH = (int)((m >> 11) & 31u) - kHalf * half;  // entries in this half
+      const int loc = (int)(m >> 18);  // the row's slot
       const bool cont = (m >> 16) & 1u, partial = (m >> 17) & 1u;
       X xv[kHalf];
 #pragma unroll
@@ -261,10 +264,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       __builtin_amdgcn_sched_barrier(0);
       X prod[kHalf];
 #pragma unroll
-      for (int j = 0; j < kHalf; j++) {
-        prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
-        prod[j] = j < lenH ? prod[j] : (X)0;
-      }
+      for (int j = 0; j < kHalf; j++) prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
       // entries 0-7 in the low lanes, then 8-15 in the high lanes, in order
       X acc = cont ? carry : bwd ? (X)0 : pX;
 #pragma unroll
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     }
   }
   __syncthreads();
-  for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xs[q];
+  for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xs[q + 1];
 }
 
 template <typename FT, typename X>
