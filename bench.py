@@ -10,7 +10,7 @@ mixed-precision iterative-refinement solve from x = 0 to relative residual
 Printed as one JSON line: value = solves/s of that MPIR solve, plus the
 plain-FP64 PBiCGStab+ILU(0) solve time on the same GPU, the FP16-storage
 variant, HBM rooflines of the two kernels an inner iteration runs twice (the
-sliced-ELL SpMV and the packed ILU(0) substitution, HIP-event timing on their
+sliced-ELL SpMV and the row-per-lane ILU(0) substitution, HIP-event timing on their
 launch stream) and a CPU baseline (the oracle restatement, one core, bounded
 sample).
 
@@ -333,7 +333,7 @@ def main():
     rl_spmv = roofline(sp["GBs"], spmv_bytes, f"spmv-{args.format}",
                        f"sellKernel<float> SpMV ({args.format.upper()})")
     rl_ilu = roofline(ilu["GBs"], ilu_bytes, "ilu-apply",
-                      "iluPackedKernel<float,float> (ILU(0) fwd+bwd substitution)")
+                      "iluRowsKernel<float,float> (ILU(0) fwd+bwd substitution)")
     if args.kernels_only:
         cal = copy_calibration(g)
         print(json.dumps({"spmv": spmv, "ilu": ilu, "copy": cal,

@@ -115,26 +115,30 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   gph::DevState zero{};
   GPH_HIP(hipMemcpyAsync(glob_.get(), &zero, sizeof(zero), hipMemcpyHostToDevice, st));
 
-  // rows-per-wave of the ILU kernel: the configured value, or the fastest of
-  // a few candidates on this union (three timed applications each)
-  const bool tune = configJson.find("rowsPerWave") == std::string::npos;
-  std::vector<int32_t> cands;
+  // ILU substitution kernel: the configured one, or the fastest on this
+  // union (three timed applications each) of the row-per-lane kernel and
+  // the packed kernel with a few rows-per-wave groupings
+  const bool tune = configJson.find("rowsPerWave") == std::string::npos &&
+                    configJson.find("substitution") == std::string::npos;
+  std::vector<std::pair<std::string, int32_t>> cands;
   if (tune) {
-    // candidates that group the tiles differently (equal groupings are tried once)
+    cands.push_back({"rows", ilu->rowsPerWave});
+    // packed groupings that differ (equal groupings are tried once)
     std::vector<std::vector<int32_t>> seen;
     for (int32_t c : {512, 1024, 2048, 4096}) {
       std::vector<int32_t> gr = matrix::groupTiles(h, c);
       if (std::find(seen.begin(), seen.end(), gr) == seen.end()) {
         seen.push_back(std::move(gr));
-        cands.push_back(c);
+        cands.push_back({"packed", c});
       }
     }
   } else {
-    cands.push_back(ilu->rowsPerWave);
+    cands.push_back({ilu->substitution, ilu->rowsPerWave});
   }
   double best = 1e30;
-  for (int32_t c : cands) {
+  for (const auto& [sub, c] : cands) {
     auto ic = std::make_shared<solver::ilu::Configuration>(*ilu);
+    ic->substitution = sub;
     ic->rowsPerWave = c;
     auto S = solver::Solver::createSolver(A, ic);
     const Vec in{r_.get(), DType::F32}, o{y_.get(), DType::F32};
@@ -147,7 +151,8 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
     const double dt = now() - t0;
     if (dt < best) {
       best = dt;
-      rowsPerWave_ = c;
+      rowsPerWave_ = sub == "packed" ? c : 0;
+      substitution_ = sub;
       ilu_ = std::move(S);
     }
   }

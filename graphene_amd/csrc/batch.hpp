@@ -14,8 +14,8 @@
 // masked vector updates. The inner iteration is captured once in a HIP
 // graph of several iterations and replayed until every system has
 // finished; finished systems are skipped and a finished batch makes every
-// kernel return at once. Setup tunes the ILU kernel's rows per wave on the
-// union (a few timed applications per candidate).
+// kernel return at once. Setup picks the ILU substitution kernel (and the
+// packed kernel's rows per wave) on the union by a few timed applications.
 #pragma once
 
 #include <cstdint>
@@ -47,6 +47,7 @@ class BatchSolver {
   int64_t systemRowStart(int32_t s) const { return sysRow_[(size_t)s]; }
   bool refinement() const { return mpir_; }
   int32_t tunedRowsPerWave() const { return rowsPerWave_; }
+  const std::string& tunedSubstitution() const { return substitution_; }
 
   // MPIR: x float64, b float32. PBiCGStab: x and b float32. Tile-major
   // order; x is overwritten (the solve starts from zero).
@@ -64,6 +65,7 @@ class BatchSolver {
   std::vector<int64_t> sysRow_;
   bool mpir_ = false;
   int32_t rowsPerWave_ = 0;
+  std::string substitution_;
   gph::DevState innerInit_{}, outerInit_{};
   std::unique_ptr<matrix::solver::Solver> ilu_;
   gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_, sliceSys_;

@@ -109,10 +109,11 @@ struct Configuration : solver::Configuration {
   std::optional<gph::DType> factorType;
   // extension: rows per wave group of the substitution kernels
   int rowsPerWave = 1024;
-  // extension: substitution kernel: "packed" (product slabs), "rows" (lane
-  // per row, factor rows in registers), "level" (lane per row over level
-  // records), "tile" (row blocks)
-  std::string substitution = "packed";
+  // extension: substitution kernel: "rows" (lanes per row, factor rows in
+  // registers; the packed kernel where its formats do not fit), "packed"
+  // (product slabs), "level" (lane per row over level records), "tile" (row
+  // blocks)
+  std::string substitution = "rows";
   explicit Configuration(const Json& c);
   std::string solverName() const override {
     return diagonalBased ? "DILU" : "ILU(0)";

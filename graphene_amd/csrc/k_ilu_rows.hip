@@ -167,6 +167,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   constexpr int NV = (kHalf * (int)sizeof(FT)) / 16;    // value words per lane
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
+  if (p.tileSys && p.sysState[p.tileSys[blockIdx.x]].done) return;  // finished system
   const RowsLds L = rowsLayout(p, sizeof(X), D);
   X* xs = reinterpret_cast<X*>(smem + L.xs);
   X* invd = reinterpret_cast<X*>(smem + L.invd);

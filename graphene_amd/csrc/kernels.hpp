@@ -207,6 +207,10 @@ struct RowsArgs {
   const uint32_t* meta = nullptr;      // record words
   const uint16_t* col = nullptr;       // column byte offsets (4 * local id)
   const void* val = nullptr;           // factor values
+  // optional (batched solves): tile t is skipped when system tileSys[t]
+  // has finished (sysState[tileSys[t]].done)
+  const DevState* sysState = nullptr;
+  const int32_t* tileSys = nullptr;
 };
 constexpr int kRowsThreads = 64;
 constexpr int kRowsMaxLen = 16;  // entries loaded per record (longest record)

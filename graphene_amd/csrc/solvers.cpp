@@ -126,6 +126,20 @@ class IluSolver : public Solver {
 
   void setSystemSkip(const DevState* states,
                      const std::vector<int64_t>& systemRowStart) override {
+    if (rows_) {  // the system of every tile (systems are tile ranges)
+      const std::vector<int64_t>& ts = A_.hostMatrix().tileStart;
+      std::vector<int32_t> sys(ts.size() - 1);
+      size_t k = 0;
+      for (size_t t = 0; t + 1 < ts.size(); t++) {
+        while (k + 2 < systemRowStart.size() && systemRowStart[k + 1] <= ts[t]) k++;
+        sys[t] = (int32_t)k;
+      }
+      tileSys_.upload(sys.data(), sys.size(), A_.stream());
+      GPH_HIP(hipStreamSynchronize(A_.stream()));
+      rargs_.tileSys = tileSys_.as<int32_t>();
+      rargs_.sysState = states;
+      return;
+    }
     if (!packed_) return;
     // systems of each wave group (groups and systems are row ranges)
     const size_t G = groupRowStart_.size() - 1;
@@ -299,7 +313,7 @@ class IluSolver : public Solver {
   DevBuf lu_, invD_, lu16_, invD16_;
   bool packed_ = false, level_ = false, rows_ = false;
   gph::RowsArgs rargs_;
-  DevBuf rcol_, rval_, relem_, rhoff_, rhdr_, rmeta_, rmoff_;
+  DevBuf rcol_, rval_, relem_, rhoff_, rhdr_, rmeta_, rmoff_, tileSys_;
   gph::PackedArgs pargs_;
   gph::LevelArgs largs_;
   DevBuf lbytes_, lgoff_, lsteps_, lgstart_;

@@ -277,15 +277,17 @@ int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
  * float32 working type) around PBiCGStab, or PBiCGStab alone, each with an
  * ILU preconditioner; every system follows the single-system algorithm from
  * x = 0 with its own scalars and convergence. The inner iterations run as a
- * captured HIP graph; the ILU kernel's rows per wave is tuned at creation
- * unless the ILU configuration sets "rowsPerWave". */
+ * captured HIP graph; the ILU substitution kernel (and the packed kernel's
+ * rows per wave) is picked by timing at creation unless the ILU
+ * configuration sets "substitution" or "rowsPerWave". */
 typedef struct gph_batch gph_batch;
 int gph_batch_create(const gph_matrix* A, int32_t n_systems,
                      const int32_t* system_tile_start, const char* json,
                      gph_batch** out);
 void gph_batch_destroy(gph_batch* b);
 /* refinement: 1 for IterativeRefinement (x float64), 0 for PBiCGStab (x
- * float32); rows_per_wave: the ILU kernel's setting chosen at creation. */
+ * float32); rows_per_wave: the packed ILU kernel's rows per wave chosen at
+ * creation, 0 when the row-per-lane kernel was chosen. */
 int gph_batch_info(const gph_batch* b, int32_t* n_systems, int32_t* refinement,
                    int32_t* rows_per_wave);
 /* x (overwritten) and b (float32) in the matrix's tile-major order; stats

@@ -26,11 +26,14 @@ def make_systems(seed=7):
     return out
 
 
-def mpir_cfg(inner_rel=1e-3):
+def mpir_cfg(inner_rel=1e-3, sub=None):
+    pre = {"type": "ILU"}
+    if sub:
+        pre["substitution"] = sub
     return {"type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 20,
             "extendedPrecisionType": "float64", "workingType": "float32",
             "innerSolver": {"type": "PBiCGStab", "relResidual": inner_rel,
-                            "maxIterations": 200, "preconditioner": {"type": "ILU"}}}
+                            "maxIterations": 200, "preconditioner": pre}}
 
 
 @pytest.fixture(scope="module")
@@ -61,11 +64,15 @@ def true_rel_residuals(B, x_local):
     return np.array(out)
 
 
-def test_batch_mpir_converges_and_matches_alone(batch):
+@pytest.mark.parametrize("sub", [None, "rows", "packed"])
+def test_batch_mpir_converges_and_matches_alone(batch, sub):
+    """With the substitution kernel picked by timing, and with each kernel
+    (finished systems' tiles or wave groups skipped)."""
     B = batch
-    bt = g.Batch(B["A"], B["ts"], mpir_cfg())
+    bt = g.Batch(B["A"], B["ts"], mpir_cfg(sub=sub))
     assert bt.num_systems == len(SIZES) and bt.refinement
-    assert bt.rows_per_wave in (512, 1024, 2048, 4096)
+    assert bt.rows_per_wave in ((0,) if sub == "rows" else (512, 1024, 2048, 4096)
+                                if sub == "packed" else (0, 512, 1024, 2048, 4096))
     x = g.DeviceVector(B["h"].nrows, np.float64)
     stats, sec = bt.solve(x, g.DeviceVector.from_numpy(B["b"]))
     assert all(s.converged for s in stats), stats
@@ -108,7 +115,7 @@ def test_batch_repeat_identical(batch):
 def test_batch_pbicgstab_only(batch):
     B = batch
     cfg = {"type": "PBiCGStab", "relResidual": 1e-6, "maxIterations": 500,
-           "preconditioner": {"type": "ILU", "rowsPerWave": 1024}}
+           "preconditioner": {"type": "ILU", "substitution": "packed", "rowsPerWave": 1024}}
     bt = g.Batch(B["A"], B["ts"], cfg)
     assert not bt.refinement and bt.rows_per_wave == 1024
     x = g.DeviceVector(B["h"].nrows, np.float32)
