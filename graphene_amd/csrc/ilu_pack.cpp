@@ -359,7 +359,7 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
   return o;
 }
 
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int perStep) {
   IluRowStream o;
   const int32_t T = m.numTiles;
   for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
@@ -427,8 +427,8 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
         int32_t nLong = 0;
         while (nLong < R && len(order[(size_t)nLong]) > kRowsMaxLen) nLong++;
         int32_t next = nLong;  // next short row
-        for (int32_t c0 = 0; c0 < nLong; c0 += kRowsPerStep) {
-          const int32_t lanes = std::min<int32_t>(kRowsPerStep, nLong - c0);
+        for (int32_t c0 = 0; c0 < nLong; c0 += perStep) {
+          const int32_t lanes = std::min<int32_t>(perStep, nLong - c0);
           const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
           for (int64_t k = 0; k < chunks; k++) {
             for (int32_t q = 0; q < lanes; q++) {
@@ -445,7 +445,7 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
                 step.push_back({-1, 0, 0, 2u});
               }
             }
-            while ((int32_t)step.size() < kRowsPerStep && next < R) {
+            while ((int32_t)step.size() < perStep && next < R) {
               const int32_t s = order[(size_t)next++];
               step.push_back({s, 0, len(s), 0u});
             }
@@ -453,7 +453,7 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
           }
         }
         while (next < R) {
-          while ((int32_t)step.size() < kRowsPerStep && next < R) {
+          while ((int32_t)step.size() < perStep && next < R) {
             const int32_t s = order[(size_t)next++];
             step.push_back({s, 0, len(s), 0u});
           }

@@ -97,7 +97,7 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
 // per row of the current level).
 //
 // Per tile, the forward then the backward substitution is a sequence of
-// steps; a step holds up to kRowsPerStep row records of one level (the rows
+// steps; a step holds up to perStep (<= 32) row records of one level (the rows
 // of a level are independent), longest first. A record is a row's entries
 // (forward: strictly lower in-tile entries ascending; backward: strictly
 // upper in-tile entries descending, the order ilu/Solver.cpp:380-442 visits
@@ -122,7 +122,6 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
 // The element streams end with kRowsTail elements of zero entries.
 constexpr int kRowsMaxLen = 16;
 constexpr int kRowsPad = 8;
-constexpr int kRowsPerStep = 32;
 constexpr int kRowsTail = 64;
 struct IluRowStream {
   std::vector<uint16_t> col;       // byte offsets (4 * local id)
@@ -138,7 +137,9 @@ struct IluRowStream {
   bool ok = false;  // false: ids or the stream too large for the formats
 };
 
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es);
+// perStep: records per step (32 when a wave runs one tile, 16 when it
+// runs two).
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int perStep);
 
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,

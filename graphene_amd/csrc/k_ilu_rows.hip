@@ -57,7 +57,7 @@ constexpr int rowsDepth() {
 
 size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt) {
   const int D = ft == DType::F64 ? rowsDepth<double>() : rowsDepth<float>();
-  return rowsLayout(p, dtypeSize(xt), D).total;
+  return (size_t)(p.tilesPerWave == 2 ? 2 : 1) * rowsLayout(p, dtypeSize(xt), D).total;
 }
 
 // 16 bytes with 4-byte alignment: the row streams are dword aligned only.
@@ -158,77 +158,105 @@ __device__ __forceinline__ void stageRows(int t0, int nt, const FT* __restrict__
   }
 }
 
-template <typename FT, typename X>
+template <typename FT, typename X, int TPW>
 __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     RowsArgs p, const FT* __restrict__ invD, const X* __restrict__ rhs,
     X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
   constexpr int D = rowsDepth<FT>();
   constexpr int kHalf = kRowsMaxLen / 2;                // entries per lane
   constexpr int NV = (kHalf * (int)sizeof(FT)) / 16;    // value words per lane
+  constexpr int kRecs = 32 / TPW;                       // records per tile and step
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
-  if (p.tileSys && p.sysState[p.tileSys[blockIdx.x]].done) return;  // finished system
-  const RowsLds L = rowsLayout(p, sizeof(X), D);
-  X* xs = reinterpret_cast<X*>(smem + L.xs);
-  X* invd = reinterpret_cast<X*>(smem + L.invd);
-  uint32_t* meta = reinterpret_cast<uint32_t*>(smem + L.meta);
-  uint32_t* hdr = reinterpret_cast<uint32_t*>(smem + L.hdr);
-  const int tile = blockIdx.x, lane = threadIdx.x;
-  const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
-  const int h0 = p.hdrOff[tile], nS = p.hdrOff[tile + 1] - h0;
+  const RowsLds L = rowsLayout(p, sizeof(X), D);  // one region per tile of the wave
+  const int lane = threadIdx.x;
+  const int tileBase = blockIdx.x * TPW;
+  auto live = [&](int tile) {  // a tile to solve (batched: its system is running)
+    return tile < p.T && !(p.tileSys && p.sysState[p.tileSys[tile]].done);
+  };
 
-  const int64_t m0 = p.metaOff[tile];
-  const int nRec = (int)(p.metaOff[tile + 1] - m0);
-  stageWords(meta, p.meta + m0, nRec, nRec + 64);
-  stageWords(hdr, p.hdr + h0, nS, nS + 2 * D + 1);
+  // every tile of the wave: step headers and record words (zero-filled past
+  // the tile's own), right-hand side (with the fused PBiCGStab update) and
+  // 1/diag into its LDS region
   const X* r = static_cast<const X*>(pre.r);
   X* pv = static_cast<X*>(pre.p);
   const X* vv = static_cast<const X*>(pre.v);
-  if (pre.kind == kPreP) {
-    if (st->iter != 0) {
-      const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
-      const X omega = (X)st->omega;
-      stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); },
-                pv);
+  int nSmax = 0;
+#pragma unroll
+  for (int tt = 0; tt < TPW; tt++) {
+    const int tile = tileBase + tt;
+    unsigned char* reg = smem + (size_t)tt * L.total;
+    X* xs = reinterpret_cast<X*>(reg + L.xs);
+    X* invd = reinterpret_cast<X*>(reg + L.invd);
+    uint32_t* meta = reinterpret_cast<uint32_t*>(reg + L.meta);
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(reg + L.hdr);
+    const bool on = live(tile);
+    const int t0 = on ? p.tileStart[tile] : 0, nt = on ? p.tileStart[tile + 1] - t0 : 0;
+    const int h0 = on ? p.hdrOff[tile] : 0, nS = on ? p.hdrOff[tile + 1] - h0 : 0;
+    const int64_t m0 = on ? p.metaOff[tile] : 0;
+    const int nRec = on ? (int)(p.metaOff[tile + 1] - m0) : 0;
+    nSmax = nS > nSmax ? nS : nSmax;
+    stageWords(meta, p.meta + m0, nRec, nRec + 64);
+    stageWords(hdr, p.hdr + h0, nS, p.maxSteps + 2 * D + 1);
+    if (pre.kind == kPreP) {
+      if (st->iter != 0) {
+        const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+        const X omega = (X)st->omega;
+        stageRows(t0, nt, invD, xs, invd,
+                  [&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); }, pv);
+      } else {
+        stageRows(t0, nt, invD, xs, invd, [&](int i) { return pv[i]; }, (X*)nullptr);
+      }
+    } else if (pre.kind == kPreS) {
+      const X alpha = (X)st->alpha;
+      stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] - alpha * vv[i]; }, pv);
     } else {
-      stageRows(t0, nt, invD, xs, invd, [&](int i) { return pv[i]; }, (X*)nullptr);
+      stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
     }
-  } else if (pre.kind == kPreS) {
-    const X alpha = (X)st->alpha;
-    stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] - alpha * vv[i]; }, pv);
-  } else {
-    stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
+    if (lane == 0) xs[0] = (X)0;  // the zero slot
   }
-  if (lane == 0) xs[0] = (X)0;  // the zero slot
   __syncthreads();
+  if (nSmax == 0) return;
 
-  // the tile's streams (and the read-ahead past their end) as buffers
-  const uint32_t nElem = p.elemOff[tile + 1] - p.elemOff[tile] + kRowsTail;
+  // this lane: tile tl of the wave, half `half` of record rl's entries
+  const int tl = TPW == 2 ? (lane >> 4) & 1 : 0;
+  const int rl = lane & (kRecs - 1), half = lane >> 5;
+  unsigned char* reg = smem + (size_t)tl * L.total;
+  X* xs = reinterpret_cast<X*>(reg + L.xs);
+  const X* invd = reinterpret_cast<const X*>(reg + L.invd);
+  const uint32_t* meta = reinterpret_cast<const uint32_t*>(reg + L.meta);
+  const uint32_t* hdr = reinterpret_cast<const uint32_t*>(reg + L.hdr);
+  // the wave's streams (and the read-ahead past their end) as buffers
+  const int tileEnd = tileBase + TPW < p.T ? tileBase + TPW : p.T;
+  const uint32_t e0 = p.elemOff[tileBase];
+  const uint32_t nElem = p.elemOff[tileEnd] - e0 + kRowsTail;
   const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<FT*>(static_cast<const FT*>(p.val) + p.elemOff[tile]), 0,
-      (int)(nElem * sizeof(FT)), kBufferFlags);
+      const_cast<FT*>(static_cast<const FT*>(p.val) + e0), 0, (int)(nElem * sizeof(FT)),
+      kBufferFlags);
   const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint16_t*>(p.col + p.elemOff[tile]), 0, (int)(nElem * 2u), kBufferFlags);
+      const_cast<uint16_t*>(p.col + e0), 0, (int)(nElem * 2u), kBufferFlags);
   const unsigned char* xb = reinterpret_cast<const unsigned char*>(xs);
   const int dilu = p.dilu;
-  const int half = lane >> 5, rl = lane & 31;  // half of record rl's entries
   Words4 vr[D][NV], cr[D];
   uint32_t mr[D], hr[D];
   X carry = (X)0;  // partial sum of a row split over consecutive steps
-  int ldRow = 0;        // first record word of the next step to load
-  uint32_t ldElem = 0;  // its first element
+  int ldRow = 0;   // first record word of the next step to load
+  // its first element, from the wave's first stream element
+  uint32_t ldElem = (tileBase + tl < p.T ? p.elemOff[tileBase + tl] : p.elemOff[tileEnd]) - e0;
+  auto header = [&](int k) {
+    return TPW == 1 ? __builtin_amdgcn_readfirstlane(hdr[k]) : hdr[k];
+  };
   // loads of a step's records into ring slot u; lanes past the step's
   // records take an offset outside the buffer: their loads return zeros
   // without a memory access
   auto issue = [&](int u, uint32_t h, uint32_t m) {
     const int R = (int)(h & 127u);
     const bool has = rl < R && kHalf * half < (int)((m >> 11) & 31u);
-    const uint32_t e = has ? (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
+    const uint32_t e = has ? ldElem + (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
     const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
 #pragma unroll
-    for (int i = 0; i < NV; i++)
-      vr[u][i] = bufLoad16(vrs, vo, (int)(ldElem * sizeof(FT)) + 16 * i);
-    cr[u] = bufLoad16(crs, co, (int)(ldElem * 2u));
+    for (int i = 0; i < NV; i++) vr[u][i] = bufLoad16(vrs, vo, 16 * i);
+    cr[u] = bufLoad16(crs, co, 0);
     mr[u] = m;
     hr[u] = h;
     ldRow += R;
@@ -238,15 +266,14 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   // load waits then hold for every pass)
 #pragma unroll
   for (int u = 0; u < D; u++) {
-    const uint32_t h = __builtin_amdgcn_readfirstlane(hdr[u]);
-    issue(u, h, meta[ldRow + rl]);
+    issue(u, header(u), meta[ldRow + rl]);
     __builtin_amdgcn_sched_barrier(0);
   }
-  for (int k0 = 0; k0 < nS; k0 += D) {
+  for (int k0 = 0; k0 < nSmax; k0 += D) {
 #pragma unroll
     for (int u = 0; u < D; u++) {
       // descriptors of the step that refills this slot
-      const uint32_t hn = __builtin_amdgcn_readfirstlane(hdr[k0 + u + D]);
+      const uint32_t hn = header(k0 + u + D);
       const uint32_t mn = meta[ldRow + rl];
       const uint32_t h = hr[u], m = mr[u];
       const int R = (int)(h & 127u);
@@ -257,7 +284,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
 #pragma unroll
       for (int j = 0; j < kHalf; j++) {
         const uint32_t w = cr[u].w[j >> 1];
-        const uint32_t bo = (j & 1) ? w >> 16 : w & 0xffffu;  // 4 * local id
+        const uint32_t bo = (j & 1) ? w >> 16 : w & 0xffffu;  // 4 * slot
         xv[j] = *reinterpret_cast<const X*>(xb + bo * (sizeof(X) / 4));
       }
       const X pX = xs[loc], pD = invd[loc];
@@ -284,18 +311,26 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     }
   }
   __syncthreads();
-  for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xs[q + 1];
+#pragma unroll
+  for (int tt = 0; tt < TPW; tt++) {
+    const int tile = tileBase + tt;
+    if (!live(tile)) continue;
+    const X* xt = reinterpret_cast<const X*>(smem + (size_t)tt * L.total + L.xs);
+    const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
+    for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xt[q + 1];
+  }
 }
 
 template <typename FT, typename X>
 static void rowsTyped(const RowsArgs& p, const void* invD, const void* rhs, void* out,
                       const PreOp& pre, DevState* st, int checkDone, hipStream_t s) {
-  const size_t shm = rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
-  auto k = iluRowsKernel<FT, X>;
+  const int tpw = p.tilesPerWave == 2 ? 2 : 1;
+  const size_t shm = (size_t)tpw * rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
+  auto k = tpw == 2 ? iluRowsKernel<FT, X, 2> : iluRowsKernel<FT, X, 1>;
   if (shm > 64 * 1024)
     GPH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
-  hipLaunchKernelGGL(k, dim3(p.T), dim3(kRowsThreads), shm, s, p,
+  hipLaunchKernelGGL(k, dim3((p.T + tpw - 1) / tpw), dim3(kRowsThreads), shm, s, p,
                      static_cast<const FT*>(invD), static_cast<const X*>(rhs),
                      static_cast<X*>(out), pre, st, checkDone);
   GPH_HIP(hipGetLastError());

@@ -32,13 +32,37 @@ __global__ __launch_bounds__(kVecThreads) void updateXRKernel(VecArgs v,
   X* x = static_cast<X*>(v.o1);
   X* r = static_cast<X*>(v.o2);
   X acc[2] = {0, 0};
-  for (int i = blockIdx.x * kVecThreads + threadIdx.x; i < v.n;
-       i += gridDim.x * kVecThreads) {
-    x[i] = (x[i] + alpha * y[i]) + omega * z[i];
-    const X ri = s[i] - omega * t[i];
-    r[i] = ri;
-    acc[0] = acc[0] + ri * ri;
-    acc[1] = acc[1] + r0[i] * ri;
+  // kU grid-stride elements per pass: all their loads are issued before
+  // the first store (the stores may alias them as far as the compiler
+  // knows); the sums take the elements in the same order as a plain
+  // grid-stride loop
+  constexpr int kU = 8;
+  const int64_t stride = (int64_t)gridDim.x * kVecThreads;
+  for (int64_t i0 = blockIdx.x * kVecThreads + threadIdx.x; i0 < v.n; i0 += kU * stride) {
+    X xv[kU], yv[kU], zv[kU], sv[kU], tv[kU], wv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int64_t i = i0 + u * stride;
+      if (i < v.n) {
+        xv[u] = x[i];
+        yv[u] = y[i];
+        zv[u] = z[i];
+        sv[u] = s[i];
+        tv[u] = t[i];
+        wv[u] = r0[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int64_t i = i0 + u * stride;
+      if (i < v.n) {
+        x[i] = (xv[u] + alpha * yv[u]) + omega * zv[u];
+        const X ri = sv[u] - omega * tv[u];
+        r[i] = ri;
+        acc[0] = acc[0] + ri * ri;
+        acc[1] = acc[1] + wv[u] * ri;
+      }
+    }
   }
   X tot[2];
   if (gridReduce<X, 2, kVecThreads>(acc, static_cast<X*>(v.partials),

@@ -199,6 +199,7 @@ void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
 struct RowsArgs {
   int32_t T = 0, maxRows = 0, maxSteps = 0, maxRecs = 0;
   int32_t dilu = 0;
+  int32_t tilesPerWave = 1;  // 1, or 2 (lanes 0-15 / 32-47 and 16-31 / 48-63)
   const int32_t* tileStart = nullptr;  // T+1
   const uint32_t* elemOff = nullptr;   // T+1 element offsets of the tiles' streams
   const int32_t* hdrOff = nullptr;     // T+1 offsets of the tiles' step headers

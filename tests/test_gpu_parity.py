@@ -140,7 +140,15 @@ def test_ldu_spmv_bit_exact(case, sliced):
     assert np.array_equal(r.numpy(), o.ldu_residual_mixed(U, x64, b)[0])
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level", "tile"])
+def _sub(sub):
+    """ILU substitution settings: "rows2" = the row-per-lane kernel with two
+    tiles per wave."""
+    if sub == "rows2":
+        return {"substitution": "rows", "tilesPerWave": 2}
+    return {"substitution": sub}
+
+
+@pytest.mark.parametrize("sub", ["packed", "rows", "rows2", "level", "tile"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_ilu0_apply_bit_exact(case, ftype, sub):
     """ILU(0) factorization + substitution equal the oracle's, with each
@@ -148,7 +156,7 @@ def test_ilu0_apply_bit_exact(case, ftype, sub):
     blocks)."""
     name, h, L, A = case
     dt = np.float32 if ftype == "float32" else np.float64
-    S = g.Solver(A, {"type": "ILU", "factorType": ftype, "substitution": sub})
+    S = g.Solver(A, {"type": "ILU", "factorType": ftype, **_sub(sub)})
     b = o.uniform_pm1(17, h.nrows).astype(dt)
     x = dev(np.zeros(h.nrows, dt))
     st = S.solve(x, dev(b))
@@ -158,14 +166,14 @@ def test_ilu0_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "rows2", "level"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_dilu_apply_bit_exact(case, ftype, sub):
     """DILU (diagonalBased) factorization + substitution equal the oracle's."""
     name, h, L, A = case
     dt = np.float32 if ftype == "float32" else np.float64
     S = g.Solver(A, {"type": "ILU", "factorType": ftype, "diagonalBased": True,
-                     "substitution": sub})
+                     **_sub(sub)})
     assert S.name() == "DILU"
     b = o.uniform_pm1(27, h.nrows).astype(dt)
     x = dev(np.zeros(h.nrows, dt))
@@ -175,11 +183,11 @@ def test_dilu_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "rows2", "level"])
 def test_ilu0_fp16_factors_bit_exact(case, sub):
     """FP16-stored factors: the FP32 factors rounded to FP16, applied in FP32."""
     name, h, L, A = case
-    S = g.Solver(A, {"type": "ILU", "factorType": "float16", "substitution": sub})
+    S = g.Solver(A, {"type": "ILU", "factorType": "float16", **_sub(sub)})
     b = o.uniform_pm1(28, h.nrows).astype(np.float32)
     x = dev(np.zeros(h.nrows, np.float32))
     S.solve(x, dev(b))
@@ -291,9 +299,9 @@ def test_pbicgstab_level_substitution_identical(case, ftype):
     name, h, L, A = case
     xt, b = rhs_for(h, L, seed=5)
     outs = []
-    for sub in ("packed", "level", "rows"):
+    for sub in ("packed", "level", "rows", "rows2"):
         cfg = _pbicgstab_cfg(1e-6)
-        cfg["preconditioner"] = {"type": "ILU", "factorType": ftype, "substitution": sub}
+        cfg["preconditioner"] = {"type": "ILU", "factorType": ftype, **_sub(sub)}
         x = dev(np.zeros(h.nrows, np.float32))
         st = g.Solver(A, cfg).solve(x, dev(b))
         outs.append((st.iterations, x.numpy()))
