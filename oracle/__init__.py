@@ -27,7 +27,7 @@ def build():
     """Compiles liboracle.so with g++ (strict FP order, no contraction)."""
     tmp = LIB + ".tmp"
     cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-           "-fPIC", "-shared", "-o", tmp, SRC]
+           "-fopenmp", "-fPIC", "-shared", "-o", tmp, SRC]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB
@@ -72,8 +72,18 @@ def lib():
         L.orc_pbicgstab_f32.restype = C.c_int32
         L.orc_pbicgstab_f64.restype = C.c_int32
         L.orc_mpir.restype = C.c_int32
+        L.orc_max_threads.restype = C.c_int32
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().orc_set_threads(C.c_int32(int(n)))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def _p(a):

@@ -26,6 +26,7 @@
 // lg/dsl/tensor/tests/TensorTests.cpp, BiCGStab vs scipy.sparse.linalg.bicgstab
 // (no preconditioner), MPIR final FP64 residual.
 
+#include <omp.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -513,6 +514,8 @@ static void ilu0_factor(int64_t n, int32_t T, const int64_t* tileStart,
       if (col[a] == j) v = off[a];
     return v;
   };
+  // tiles are independent blocks: one thread per tile (same results)
+#pragma omp parallel for schedule(dynamic, 4)
   for (int32_t t = 0; t < T; t++) {
     int64_t t0 = tileStart[t], t1 = tileStart[t + 1];
     for (int64_t i = t0 + 1; i < t1; i++) {
@@ -578,6 +581,7 @@ template <typename F, typename X, bool DILU = false>
 static void ilu0_apply(int32_t T, const int64_t* tileStart,
                        const int64_t* rowPtr, const int64_t* col, const F* lu,
                        const F* invD, const X* b, X* x) {
+#pragma omp parallel for schedule(dynamic, 4)
   for (int32_t t = 0; t < T; t++) {
     int64_t t0 = tileStart[t], t1 = tileStart[t + 1];
     for (int64_t i = t0; i < t1; i++) {
@@ -899,13 +903,17 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
   int64_t n = s->n;
   const int64_t* tsp = s->tileStart.data();
   const int32_t T = s->T;
+  // element loops run on all threads (each element computed as in a serial
+  // loop); the reductions stay serial in the reference's order
   auto spmv = [&](const X* in, X* out) {
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; p++)
       out[p] = row_sum<float, X, X>(p, s->rowPtr.data(), s->col.data(),
                                     s->off.data(), s->diag.data(), in);
   };
   auto dot = [&](const X* u, const X* v) {
     std::vector<X> t(n);
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) t[i] = u[i] * v[i];
     return reduce_add<X>(t.data(), T, tsp, 0);
   };
@@ -935,12 +943,14 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
     if (st->singular) break;
     if (st->iterations != 0) {
       X beta = (rho / rhoOld) * (alpha / omega);
+#pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < n; i++)
         p[i] = r[i] + beta * (p[i] - omega * v[i]);
     }
     precond_apply<X>(s, c, p.data(), y.data());
     spmv(y.data(), v.data());
     alpha = rho / dot(r0.data(), v.data());
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) sv[i] = r[i] - alpha * v[i];
     precond_apply<X>(s, c, sv.data(), z.data());
     spmv(z.data(), t.data());
@@ -948,6 +958,7 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
     omega = dot(t.data(), sv.data()) / tt;
     st->singular = st->singular || std::fabs((double)omega) < vsmall;
     if (st->singular) break;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
       x[i] = (x[i] + alpha * y[i]) + omega * z[i];
       r[i] = sv[i] - omega * t[i];
@@ -960,6 +971,10 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
     st->iterations++;
   }
 }
+
+// Threads of the parallel loops (the results do not depend on it).
+ORC_API void orc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+ORC_API int32_t orc_max_threads(void) { return omp_get_max_threads(); }
 
 ORC_API int32_t orc_pbicgstab_f32(void* h, const OrcPBiCGStabCfg* c, float* x,
                           const float* b, OrcStats* st) {
