@@ -181,8 +181,11 @@ constexpr int sellChunk() {
   return (sizeof(AT) == 8 || MODE == kModeResidual || RED != kRedNone) ? 2 : 4;
 }
 
+// SKIP: batched solves, slices whose systems have all finished are skipped
+// (a separate instantiation: the check's loads in the slice loop would make
+// the compiler's load waits conservative for every launch)
 template <typename VT, typename XT, typename BT, typename OT, typename AT,
-          int MODE, int RED>
+          int MODE, int RED, bool SKIP>
 __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
   __shared__ AT redLds[2 * kSpmvThreads / 64];
   if (a.checkDone && a.state->done) return;
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
       const int g = c0 + i;
       const int slice = g * W + wave;
       bool sliceOk = g < g1 && slice < a.numSlices;
-      if (sliceOk && a.sliceSys) {  // batched: skip slices of finished systems
+      if (SKIP && sliceOk) {  // batched: skip slices of finished systems
         bool done = true;
         for (int k = a.sliceSys[2 * slice]; k <= a.sliceSys[2 * slice + 1]; k++)
           done = done && a.sysState[k].done;
@@ -283,11 +286,16 @@ template <int S, typename VT, typename XT, typename BT, typename OT,
 static void launchOne(const SpmvArgs& a, hipStream_t s) {
   if (S == 0) {
     if (a.numSlices <= 0) return;
-    static const int grid = residentBlocks(sellKernel<VT, XT, BT, OT, AT, MODE, RED>);
     const int groups = (a.numSlices + kSpmvThreads / 64 - 1) / (kSpmvThreads / 64);
-    const int nb = groups < grid ? groups : grid;
-    hipLaunchKernelGGL((sellKernel<VT, XT, BT, OT, AT, MODE, RED>), dim3(nb),
-                       dim3(kSpmvThreads), 0, s, a);
+    if (a.sliceSys) {
+      static const int grid = residentBlocks(sellKernel<VT, XT, BT, OT, AT, MODE, RED, true>);
+      hipLaunchKernelGGL((sellKernel<VT, XT, BT, OT, AT, MODE, RED, true>),
+                         dim3(groups < grid ? groups : grid), dim3(kSpmvThreads), 0, s, a);
+    } else {
+      static const int grid = residentBlocks(sellKernel<VT, XT, BT, OT, AT, MODE, RED, false>);
+      hipLaunchKernelGGL((sellKernel<VT, XT, BT, OT, AT, MODE, RED, false>),
+                         dim3(groups < grid ? groups : grid), dim3(kSpmvThreads), 0, s, a);
+    }
   } else {
     if (a.numBlocks <= 0) return;
     hipLaunchKernelGGL((spmvKernel<S == 0 ? 1 : S, VT, XT, BT, OT, AT, MODE, RED>),
