@@ -97,8 +97,9 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
   scratchState_.alloc(sizeof(gph::DevState));
   GPH_HIP(hipMemsetAsync(scratchState_.get(), 0, sizeof(gph::DevState), s));
   int32_t maxBlocks = std::max({numBlocksCrs_, numBlocksLdu_, 4096, gph::kSellMaxGrid});
-  partialsBytes_ = (size_t)maxBlocks * 2 * sizeof(double);
+  partialsBytes_ = gph::kArriveBytes + (size_t)maxBlocks * 2 * sizeof(double);
   scratchPartials_.alloc(partialsBytes_);
+  GPH_HIP(hipMemsetAsync(scratchPartials_.get(), 0, gph::kArriveBytes, s));
   GPH_HIP(hipStreamSynchronize(s));
 }
 

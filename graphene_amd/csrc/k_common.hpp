@@ -92,42 +92,90 @@ __device__ __forceinline__ void blockSum(R (&v)[K], R* lds /* K*NT/64 */) {
   __syncthreads();
 }
 
+// Layout of a grid reduction's scratch buffer: first-level arrival counters
+// (kArriveSlots of them, one per 128-byte line, zero at allocation), then
+// the partial sums.
+constexpr int kArriveGroup = 32;    // blocks per first-level counter (at least)
+constexpr int kArriveSlots = 128;   // first-level counters
+constexpr int kArriveStride = 32;   // u32 per counter line
+constexpr size_t kArriveBytes = (size_t)kArriveSlots * kArriveStride * 4;
+
 // Grid-wide reduction epilogue: every block publishes its K partial sums;
 // the last block to arrive adds all partials in a fixed order and returns
 // true (in all of its threads) with the totals in `tot`. The arrival
-// counter is reset by the last block, so no memset is needed between uses.
+// counters are reset by the blocks that finish them, so no memset is needed
+// between uses.
+//
+// Arrival is counted in two levels: blocks of a group of consecutive block
+// ids increment their group's counter, and the last block of each group
+// increments the top counter. Atomics on one address are executed one after
+// the other at the device's coherence point (about 12 ns each here), so a
+// single counter for a grid of 2048 blocks serialises for about 25 us at the
+// end of a launch; the groups' counters sit on separate lines and proceed
+// in parallel.
 //
 // The partials are written and read with device-scope atomic stores/loads,
 // which are coherent across the XCDs' separate L2 caches, and the writer
-// waits for its stores to complete before it increments the counter. This
+// waits for its stores to complete before it increments a counter. This
 // avoids a device-scope release fence per block (an L2 write-back on this
-// hardware, which costs far more than the reduction itself).
+// hardware, which costs far more than the reduction itself). The last block
+// issues all of its loads before the first add (a device-scope load takes
+// about a microsecond); the adds take the partials in the same fixed order
+// as a plain strided loop.
 template <typename R, int K, int NT>
-__device__ bool gridReduce(R (&v)[K], R* partials, uint32_t* counter,
+__device__ bool gridReduce(R (&v)[K], void* scratch, uint32_t* counter,
                            R (&tot)[K], R* lds) {
   __shared__ int isLast;
   blockSum<R, K, NT>(v, lds);
   const int nb = gridDim.x;
+  R* partials = reinterpret_cast<R*>(static_cast<char*>(scratch) + kArriveBytes);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++)
       __hip_atomic_store(&partials[k * nb + blockIdx.x], v[k], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);  // stores complete before the arrival
-    uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    isLast = (prev == (uint32_t)(nb - 1));
+    const int gs = max(kArriveGroup, (nb + kArriveSlots - 1) / kArriveSlots);
+    const int ng = (nb + gs - 1) / gs;
+    bool last = false;
+    if (ng == 1) {
+      last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == (uint32_t)(nb - 1);
+    } else {
+      const int g = blockIdx.x / gs;
+      const uint32_t members = (uint32_t)min(gs, nb - g * gs);
+      uint32_t* sub = static_cast<uint32_t*>(scratch) + g * kArriveStride;
+      if (__hip_atomic_fetch_add(sub, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+          members - 1) {
+        __hip_atomic_store(sub, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) == (uint32_t)(ng - 1);
+      }
+    }
+    isLast = last;
   }
   __syncthreads();
   if (!isLast) return false;
   R acc[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    R s = 0;
-    for (int b = threadIdx.x; b < nb; b += NT)
-      s = s + __hip_atomic_load(&partials[k * nb + b], __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-    acc[k] = s;
+  for (int k = 0; k < K; k++) acc[k] = 0;
+  constexpr int kB = 8;  // loads in flight per thread and partial array
+  for (int b0 = threadIdx.x; b0 < nb; b0 += kB * NT) {
+    R w[K][kB];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        const int b = b0 + j * NT;
+        w[k][j] = b < nb ? __hip_atomic_load(&partials[k * nb + b], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                         : (R)0;
+      }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+      for (int j = 0; j < kB; j++)
+        if (b0 + j * NT < nb) acc[k] = acc[k] + w[k][j];
   }
   blockSum<R, K, NT>(acc, lds);
   if (threadIdx.x == 0) {

@@ -89,7 +89,7 @@ __device__ __forceinline__ void spmvFinish(const SpmvArgs& a,
   if constexpr (RED != kRedNone) {
     constexpr int K = RedAcc<AT, RED>::K;
     AT tot[K];
-    if (gridReduce<AT, K, NT>(acc.v, static_cast<AT*>(a.partials),
+    if (gridReduce<AT, K, NT>(acc.v, a.partials,
                               &a.state->counter[0], tot, redLds)) {
       if (threadIdx.x == 0)
         finalize<AT>(a.state, a.fin, (double)tot[0],

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kVecThreads) void updateXRKernel(VecArgs v,
     }
   }
   X tot[2];
-  if (gridReduce<X, 2, kVecThreads>(acc, static_cast<X*>(v.partials),
+  if (gridReduce<X, 2, kVecThreads>(acc, v.partials,
                                     &st->counter[0], tot, lds)) {
     if (threadIdx.x == 0)
       finalize<X>(st, kFinUpdate, (double)tot[0], (double)tot[1]);
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kVecThreads) void norm2Kernel(VecArgs v) {
        i += gridDim.x * kVecThreads)
     acc[0] = acc[0] + a[i] * a[i];
   T tot[1];
-  if (gridReduce<T, 1, kVecThreads>(acc, static_cast<T*>(v.partials),
+  if (gridReduce<T, 1, kVecThreads>(acc, v.partials,
                                     &st->counter[0], tot, lds)) {
     if (threadIdx.x == 0) finalize<T>(st, v.fin, (double)tot[0], 0.0);
   }

@@ -37,7 +37,7 @@ struct SpmvArgs {
   double* out64 = nullptr;      // optional unrounded FP64 result
   const void* w = nullptr;      // dot partner
   DevState* state = nullptr;
-  void* partials = nullptr;
+  void* partials = nullptr;     // grid reduction scratch: arrival counters, then sums
   int32_t fin = kFinNone;
   int32_t checkDone = 0;
   int32_t remap = 1;
@@ -75,7 +75,7 @@ struct VecArgs {
   void* o1 = nullptr;
   void* o2 = nullptr;
   DevState* state = nullptr;
-  void* partials = nullptr;
+  void* partials = nullptr;     // grid reduction scratch: arrival counters, then sums
   int32_t fin = kFinNone;
   int32_t checkDone = 0;
 };

@@ -451,6 +451,7 @@ class PBiCGStabSolver : public Solver {
     state_.alloc(sizeof(DevState));
     GPH_HIP(hipMemset(state_.get(), 0, sizeof(DevState)));
     partials_.alloc(A.partialsBytes());
+    GPH_HIP(hipMemset(partials_.get(), 0, gph::kArriveBytes));  // arrival counters
   }
   std::string name() const override { return cfg_->solverName(); }
   bool usesInitialGuess() const override { return true; }
@@ -690,6 +691,7 @@ class IterativeRefinementSolver : public Solver {
     state_.alloc(sizeof(DevState));
     GPH_HIP(hipMemset(state_.get(), 0, sizeof(DevState)));
     partials_.alloc(A.partialsBytes());
+    GPH_HIP(hipMemset(partials_.get(), 0, gph::kArriveBytes));  // arrival counters
   }
   std::string name() const override { return cfg_->solverName(); }
   bool usesInitialGuess() const override { return true; }
