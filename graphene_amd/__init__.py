@@ -476,7 +476,9 @@ class Matrix:
         return self.host.num_tiles
 
     def set_sliced(self, sliced=True):
-        """Row-gather kernel form: sliced ELL (default) or LDS row blocks."""
+        """Row-gather kernel form: sliced ELL or LDS row blocks (without a
+        call: chosen per matrix, row blocks for irregular rows with far
+        columns)."""
         _check(lib().gph_matrix_set_sliced(self._A, int(bool(sliced))))
 
     def getFormat(self):

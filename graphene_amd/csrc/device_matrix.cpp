@@ -146,7 +146,9 @@ void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
 // 32-bit columns (sliceBase[s] = -1, entry at word sliceColW[s] + 2(64k +
 // lane)). Structured-grid matrices in tile order have most slices narrow. When row lengths vary strongly (max >
 // 2x mean), rows are sorted by length inside windows of 4096 rows so that a
-// slice holds rows of similar length (perm gives each lane's row).
+// slice holds rows of similar length (perm gives each lane's row). Such a
+// matrix with more than 10 % wide slices runs in the row-block kernel unless
+// the caller asks for the sliced form.
 void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
                              const std::vector<int32_t>& cols,
                              const std::vector<float>& vals, Sell& out) {
@@ -191,6 +193,7 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
   std::vector<int32_t> base((size_t)numSlices, -1);
   std::vector<uint32_t> colW((size_t)numSlices + 1, 0);
   uint64_t words = 0;
+  int64_t wide = 0;
   for (int64_t sl = 0; sl < numSlices; sl++) {
     int32_t lo = INT32_MAX, hi = -1;
     for (int l = 0; l < 64; l++) {
@@ -209,6 +212,7 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
       words += (uint64_t)sw;
     } else {
       words += 2 * (uint64_t)sw;
+      wide++;
     }
     if (words > UINT32_MAX) return;
   }
@@ -248,6 +252,7 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
   out.col.upload(c.data(), c.size(), s);
   out.val32.upload(v.data(), v.size(), s);
   out.slots = slots;
+  out.preferred = !(sorted && 10 * wide > numSlices);
   out.numSlices = (int32_t)numSlices;
 }
 
@@ -301,9 +306,9 @@ gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
   gph::SpmvArgs a;
   a.n = n_;
   a.diag = crsDiag(vt);
-  if (useSell_) {
+  {
     const Sell& sl = fmt_ == MatrixFormat::CRS ? sellCrs_ : sellLdu_;
-    if (sl.numSlices > 0) {
+    if (sellActive(sl)) {
       a.numSlices = sl.numSlices;
       a.sliceOff = sl.sliceOff.as<int32_t>();
       a.rowLen = sl.rowLen.as<uint16_t>();

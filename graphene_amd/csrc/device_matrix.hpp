@@ -64,8 +64,13 @@ class DeviceMatrix {
   size_t deviceBytes() const;
 
   // Kernel form of the row gather: sliced ELL (default) or row blocks.
-  void setUseSell(bool v) { useSell_ = v && sellCrs_.numSlices > 0; }
-  bool useSell() const { return useSell_; }
+  // Without a call, the sliced form is used where it is the faster one: rows
+  // of similar length whose columns mostly fit the 16-bit slice offsets.
+  // Rows of strongly varying length with far columns (power-law graphs)
+  // run faster in the row-block kernel (4M rows, mean 14 entries, 10 %
+  // uniformly random columns: 284 us against 460 us on MI355X).
+  void setUseSell(bool v) { sellMode_ = v ? 1 : 0; }
+  bool useSell() const { return sellActive(fmt_ == MatrixFormat::CRS ? sellCrs_ : sellLdu_); }
 
  private:
   void buildRowBlocks(const std::vector<int64_t>& rowLen, int32_t rowsPerBlock,
@@ -103,12 +108,16 @@ class DeviceMatrix {
     gph::DevBuf sliceOff, rowLen, perm, sliceBase, sliceColW, col, val32;
     mutable gph::DevBuf val64, val16;
     int64_t slots = 0;
+    bool preferred = true;  // the default kernel form for this matrix
   };
   void buildSell(const std::vector<int64_t>& rowPtr, const std::vector<int32_t>& cols,
                  const std::vector<float>& vals, Sell& out);
   const void* sellVal(const Sell& s, gph::DType vt) const;
   Sell sellCrs_, sellLdu_;
-  bool useSell_ = true;
+  int sellMode_ = -1;  // -1: automatic, 0: row blocks, 1: sliced
+  bool sellActive(const Sell& sl) const {
+    return sl.numSlices > 0 && (sellMode_ == 1 || (sellMode_ < 0 && sl.preferred));
+  }
 
   gph::DevBuf scratchState_, scratchPartials_;
   size_t partialsBytes_ = 0;

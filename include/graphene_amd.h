@@ -190,8 +190,10 @@ int gph_vector_cast(gph_type in_type, const void* in, gph_type out_type,
 int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
                       int32_t rows_per_block, gph_matrix** out);
 void gph_matrix_destroy(gph_matrix* A);
-/* No reference counterpart: kernel form of the row gather, 1 = sliced ELL
- * (default), 0 = row blocks staged in LDS. Both give identical results. */
+/* No reference counterpart: kernel form of the row gather, 1 = sliced ELL,
+ * 0 = row blocks staged in LDS. Both give identical results. Without a call
+ * the form is chosen per matrix: sliced ELL, except for rows of strongly
+ * varying length whose columns mostly span more than 2^16 (row blocks). */
 int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced);
 int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes);
 
