@@ -531,23 +531,45 @@ class PBiCGStabSolver : public Solver {
     GPH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int cur = 0;
     int64_t launched = 0;
-    auto enqueueChunk = [&](int slot) {
-      for (int k = 0; k < chunk; k++) enqueueIteration(st, s);
-      launched += chunk;
+    int size[2] = {chunk, chunk};
+    auto enqueueChunk = [&](int slot, int m) {
+      for (int k = 0; k < m; k++) enqueueIteration(st, s);
+      launched += m;
+      size[slot] = m;
       GPH_HIP(hipMemcpyAsync(pinned_[slot], st, sizeof(DevState),
                              hipMemcpyDeviceToHost, s));
       GPH_HIP(hipEventRecord(ev[slot], s));
     };
-    enqueueChunk(cur);
+    // Length of the next chunk from the residual history: iterations
+    // enqueued after convergence exit at once but still cost a launch each
+    // (about 5 us per kernel), so near the predicted end the chunks shrink
+    // to one iteration. The prediction extrapolates the mean log-residual
+    // decrease per iteration so far; it only sets chunk lengths, never
+    // the iterates.
+    int next = chunk;
+    auto plan = [&](const DevState& h, int inFlight) {
+      next = chunk;
+      if (h.iter <= 0 || !(h.rF > 0) || !(h.rF < h.r0n)) return;
+      double target = std::max((double)h.absTol, h.r0n * (double)h.relRes);
+      if (h.relTol > 0) target = std::max(target, h.bNorm * (double)h.relTol);
+      if (!(target > 0)) return;
+      const double rate = std::log(h.r0n / h.rF) / h.iter;
+      double m = std::log(h.rF / target) / rate;
+      m = std::max(m, (double)(h.minIt - h.iter));
+      const double left = m - inFlight;
+      if (left < chunk) next = std::max(1, (int)std::ceil(left));
+    };
+    enqueueChunk(cur, chunk);
     SolverStats stats;
     stats.solverName = name();
     stats.norm = cfg_->norm;
     stats.numTiles = A_.numTiles();
     for (;;) {
-      int next = cur ^ 1;
-      if (pipelined) enqueueChunk(next);
+      const int other = cur ^ 1;
+      if (pipelined) enqueueChunk(other, next);
       GPH_HIP(hipEventSynchronize(ev[cur]));
       DevState hs = *pinned_[cur];
+      if (pipelined) plan(hs, size[other]);
       if (cfg_->printPerformanceEachIteration) {
         stats.initialResidual = hs.r0n;
         stats.finalResidual = hs.rF;
@@ -559,9 +581,9 @@ class PBiCGStabSolver : public Solver {
       }
       if (hs.done) break;
       if (pipelined)
-        cur = next;
+        cur = other;
       else
-        enqueueChunk(cur);
+        enqueueChunk(cur, chunk);
     }
     GPH_HIP(hipStreamSynchronize(s));
     GPH_HIP(hipMemcpy(pinned_[0], st, sizeof(DevState), hipMemcpyDeviceToHost));
