@@ -388,12 +388,21 @@ def main():
     extra["mpir_fp16_storage"] = {"ms": 1e3 * float(np.median(_)),
                                   "outer": st16.iterations, "inner": st16.innerIterations,
                                   "true_rel_residual": rel16}
+    # FP32 matrix values, FP16-stored ILU(0) factors (the preconditioner only:
+    # the operator and every vector stay FP32)
+    tf, stf, relf = run(mpir_cfg(args.inner_rel, args.inner_max, "float32", "float16"),
+                        np.float64, "MPIR FP16-stored factors", reps=2)
+    extra["mpir_fp16_factors"] = {"ms": 1e3 * float(np.median(tf)),
+                                  "outer": stf.iterations, "inner": stf.innerIterations,
+                                  "true_rel_residual": relf}
     if not args.skip_fp64:
         t64, st64, rel64 = run(fp64_cfg("float64"), np.float64, "plain FP64 PBiCGStab", reps=2)
         extra["fp64"] = {"ms": 1e3 * float(np.median(t64)), "iterations": st64.iterations,
                          "true_rel_residual": rel64,
                          "mpir_over_fp64": ms_step / (1e3 * float(np.median(t64))),
                          "mpir_fp16_over_fp64": extra["mpir_fp16_storage"]["ms"] /
+                         (1e3 * float(np.median(t64))),
+                         "mpir_fp16_factors_over_fp64": extra["mpir_fp16_factors"]["ms"] /
                          (1e3 * float(np.median(t64)))}
         t64b, st64b, rel64b = run(fp64_cfg("float32"), np.float64,
                                   "FP64 PBiCGStab (FP32-stored A)", reps=1)
