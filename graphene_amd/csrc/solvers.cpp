@@ -58,7 +58,8 @@ static double nowSeconds() {
 // Host-pinned copy of a DevState for asynchronous read-back.
 class PinnedState {
  public:
-  PinnedState() { GPH_HIP(hipHostMalloc((void**)&p_, 2 * sizeof(DevState), 0)); }
+  // slots 0 and 1: states copied back after each chunk; 2: initial state
+  PinnedState() { GPH_HIP(hipHostMalloc((void**)&p_, 3 * sizeof(DevState), 0)); }
   ~PinnedState() {
     if (p_) (void)hipHostFree(p_);
   }
@@ -492,7 +493,7 @@ class PBiCGStabSolver : public Solver {
     init.relRes = cfg_->relResidual;
     init.minIt = cfg_->minIterations;
     init.maxIt = cfg_->maxIterations;
-    DevState* h = pinned_[0];
+    DevState* h = pinned_[2];
     *h = init;
     GPH_HIP(hipMemcpyAsync(state_.get(), h, sizeof(DevState),
                            hipMemcpyHostToDevice, s));
@@ -526,9 +527,7 @@ class PBiCGStabSolver : public Solver {
     if (cfg_->printPerformanceEachIteration) chunk = 1;
     const bool pipelined = !cfg_->printPerformanceEachIteration;
     // pipelined polling: keep one chunk in flight while checking the previous
-    hipEvent_t ev[2];
-    GPH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    GPH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    hipEvent_t* ev = ev_.get();
     int cur = 0;
     int64_t launched = 0;
     int size[2] = {chunk, chunk};
@@ -564,6 +563,7 @@ class PBiCGStabSolver : public Solver {
     stats.solverName = name();
     stats.norm = cfg_->norm;
     stats.numTiles = A_.numTiles();
+    DevState last{};
     for (;;) {
       const int other = cur ^ 1;
       if (pipelined) enqueueChunk(other, next);
@@ -579,20 +579,26 @@ class PBiCGStabSolver : public Solver {
         stats.seconds = nowSeconds() - t0;
         stats.print();
       }
-      if (hs.done) break;
+      if (hs.done) {
+        last = hs;
+        break;
+      }
       if (pipelined)
         cur = other;
       else
         enqueueChunk(cur, chunk);
     }
-    GPH_HIP(hipStreamSynchronize(s));
-    GPH_HIP(hipMemcpy(pinned_[0], st, sizeof(DevState), hipMemcpyDeviceToHost));
-    DevState hs = *pinned_[0];
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
+    // The state read at convergence is final: kernels enqueued after it exit
+    // before touching it.
+    DevState hs = last;
+    if (!deferSync_) {
+      GPH_HIP(hipStreamSynchronize(s));
+      GPH_HIP(hipMemcpy(pinned_[0], st, sizeof(DevState), hipMemcpyDeviceToHost));
+      hs = *pinned_[0];
+    }
     if (x.ptr != xIn.ptr) {
       gph::launchCast(x.ptr, X, xIn.ptr, xIn.type, n, s);
-      GPH_HIP(hipStreamSynchronize(s));
+      if (!deferSync_) GPH_HIP(hipStreamSynchronize(s));
     }
     stats.initialResidual = hs.r0n;
     stats.finalResidual = hs.rF;
@@ -606,6 +612,7 @@ class PBiCGStabSolver : public Solver {
   }
 
   int64_t lastLaunchedIterations() const { return lastLaunched_; }
+  void setDeferredSync(bool v) override { deferSync_ = v; }
 
  private:
   void enqueueIteration(DevState* st, hipStream_t s) {
@@ -694,6 +701,19 @@ class PBiCGStabSolver : public Solver {
   DType X_ = DType::F32, V_ = DType::F32;
   Vec r_, r0_, p_, v_, y_, s_, z_, t_, x_;
   int64_t lastLaunched_ = 0;
+  bool deferSync_ = false;
+  struct Events {  // chunk completion events, reused across solves
+    hipEvent_t e[2] = {nullptr, nullptr};
+    Events() {
+      GPH_HIP(hipEventCreateWithFlags(&e[0], hipEventDisableTiming));
+      GPH_HIP(hipEventCreateWithFlags(&e[1], hipEventDisableTiming));
+    }
+    ~Events() {
+      for (hipEvent_t v : e)
+        if (v) (void)hipEventDestroy(v);
+    }
+    hipEvent_t* get() { return e; }
+  } ev_;
 };
 
 // ---------------------------------------------------------------------------
@@ -710,6 +730,7 @@ class IterativeRefinementSolver : public Solver {
     if (cfg_->norm != VectorNorm::L2)
       fail(kUnsupported, "only the L2 norm is supported by this build");
     inner_ = Solver::createSolver(A, cfg_->innerSolver);
+    inner_->setDeferredSync(true);  // every outer step ends in a synchronizing read
     state_.alloc(sizeof(DevState));
     GPH_HIP(hipMemset(state_.get(), 0, sizeof(DevState)));
     partials_.alloc(A.partialsBytes());

@@ -46,6 +46,11 @@ class Solver {
   // right-hand side is formed on the fly by `pre` (p- or s-update of
   // PBiCGStab). Kernels skip their work once st->done is set.
   virtual bool supportsDeviceApply() const { return false; }
+  // Inner solves of a refinement loop: the solve may return while kernels
+  // that exit at once (enqueued past convergence) are still queued on the
+  // matrix's stream; the caller orders its own work on that stream and
+  // synchronizes before it hands results out.
+  virtual void setDeferredSync(bool v) { (void)v; }
   // Batched solves: the matrix is block-diagonal with system k on rows
   // systemRowStart[k] .. systemRowStart[k+1]-1; work of systems whose
   // states[k].done is set may be skipped by applyDevice (optional).
