@@ -1,0 +1,285 @@
+"""Each benchmark configuration's own workload, at full size, on the GPU.
+
+cfg0  32^3 Poisson-7, one tile, FP32 PBiCGStab + ILU(0) to relResidual 1e-6
+      (the reference's own CPU-runnable case): SpMV bit-exact against the
+      oracle, iteration count within 5 % of the oracle's.
+cfg1  128^3 27-point convection-diffusion (Peclet 10), 16x8x8-cell tiles:
+      CRS and LDU SpMV bit-exact against the oracle at full size; MPIR
+      (FP64 outer, FP32 PBiCGStab + block ILU(0) inner) to 1e-10 with the
+      true FP64 residual checked and the outer/inner counts compared with
+      one oracle MPIR run; forward error against an FP64 solve to 1e-14.
+cfg2  4,194,304-row power-law matrix, 4096 contiguous tiles: CRS SpMV (both
+      kernel forms) bit-exact against the oracle; PBiCGStab + symmetric
+      Gauss-Seidel (2 sweeps) iteration count against the oracle, and the
+      same inner solve under FP64 refinement to a true residual of 1e-8.
+cfg3  512 2D 9-point systems, log-uniform 1e3..2e5 rows, in one batch:
+      every system's true FP64 residual <= 1e-10; three sampled systems
+      equal their solo solves.
+cfg4  request server at 50, 200 and 800 requests/s over resident systems
+      of the three generators (1e4..2e6 rows, 1-8 right-hand sides): every
+      request converges to 1e-10 and a sampled request's result is the
+      same alone and in the mix.
+"""
+import numpy as np
+import pytest
+
+import graphene_amd as g
+import oracle as o
+from helpers import oracle_layout, poisson_case, powerlaw_case, rhs_for
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return g.DeviceVector.from_numpy(np.ascontiguousarray(a))
+
+
+def true_rel(A, x64, b32):
+    """||b - A x|| / ||b|| with the residual accumulated in FP64 on the GPU
+    (the residual kernel is bit-exact against the oracle)."""
+    r = A.residual(x64, b32, destType=g.FLOAT64, intermediateType=g.FLOAT64).numpy()
+    return float(np.linalg.norm(r) / np.linalg.norm(b32.numpy().astype(np.float64)))
+
+
+def mpir_cfg(inner, rel=1e-10, maxit=50):
+    return {"type": "IterativeRefinement", "relResidual": rel, "maxIterations": maxit,
+            "extendedPrecisionType": "float64", "workingType": "float32",
+            "innerSolver": inner}
+
+
+ILU_INNER = {"type": "PBiCGStab", "relResidual": 1e-3, "maxIterations": 100,
+             "workingType": "float32",
+             "preconditioner": {"type": "ILU", "factorType": "float32"}}
+
+
+# ---- cfg0 -------------------------------------------------------------------
+def test_cfg0_poisson32_pbicgstab_ilu0():
+    h, L = poisson_case(32, 32, 32)
+    assert h.nrows == 32768 and h.nnz_off == 190464 and h.num_tiles == 1
+    A = h.copyToTile()
+    x = o.uniform_pm1(11, h.nrows)
+    assert np.array_equal((A * dev(x)).numpy().view(np.uint32), o.spmv(L, x).view(np.uint32))
+    xt, b = rhs_for(h, L, seed=0)
+    cfg = {"type": "PBiCGStab", "relResidual": 1e-6, "maxIterations": 1000,
+           "workingType": "float32",
+           "preconditioner": {"type": "ILU", "factorType": "float32"}}
+    xg = dev(np.zeros(h.nrows, np.float32))
+    st = g.Solver(A, cfg).solve(xg, dev(b))
+    S = o.System(L)
+    _, sto = S.pbicgstab(np.zeros(h.nrows, np.float32), b, S.pbicgstab_cfg(1e-6, 1000))
+    print(f"cfg0: GPU {st}, oracle {sto.iterations} iterations", flush=True)
+    assert st.converged and sto.converged
+    assert abs(st.iterations - sto.iterations) <= max(1, 0.05 * sto.iterations)
+    assert true_rel(A, dev(xg.numpy().astype(np.float64)), dev(b)) <= 1e-5
+
+
+# ---- cfg1 -------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def cfg1():
+    n = 128
+    rows, cols, vals = g.generateConvectionDiffusion27Triplets(n, n, n, 10.0, 1)
+    r2t, T = g.partition_grid3d(n, n, n, 16, 8, 8)
+    h = g.HostMatrix.from_coo(n ** 3, rows, cols, vals, T, r2t)
+    L = oracle_layout(n ** 3, rows, cols, vals, T, r2t)
+    del rows, cols, vals
+    assert h.nrows == 2097152 and h.nnz_off == 53645816 and T == 2048
+    xt, b = rhs_for(h, L, seed=1)
+    return dict(h=h, L=L, b=b, T=T)
+
+
+def test_cfg1_spmv_full_size_bit_exact(cfg1):
+    h, L = cfg1["h"], cfg1["L"]
+    x = o.uniform_pm1(41, h.nrows)
+    ref = o.spmv(L, x).view(np.uint32)
+    A = h.copyToTile(g.CRS)
+    for sliced in (True, False):
+        A.set_sliced(sliced)
+        assert np.array_equal((A * dev(x)).numpy().view(np.uint32), ref), sliced
+    del A
+    U = o.ldu_build(L)
+    A = h.copyToTile(g.LDU)
+    assert np.array_equal((A * dev(x)).numpy(), o.ldu_spmv_f32(U, x))
+    x64 = x.astype(np.float64) / 3.0
+    bb = o.uniform_pm1(42, h.nrows)
+    assert np.array_equal(A.residual(dev(x64), dev(bb), destType=g.FLOAT32,
+                                     intermediateType=g.FLOAT64).numpy(),
+                          o.ldu_residual_mixed(U, x64, bb)[0])
+
+
+def test_cfg1_mpir_against_oracle(cfg1):
+    h, L, b = cfg1["h"], cfg1["L"], cfg1["b"]
+    A = h.copyToTile(g.LDU)
+    phi = dev(np.zeros(h.nrows, np.float64))
+    st = g.Solver(A, mpir_cfg(ILU_INNER)).solve(phi, dev(b))
+    rel = true_rel(A, phi, dev(b))
+    S = o.System(L)
+    _, sto, inner = S.mpir(b, 1e-10, 50, inner=S.pbicgstab_cfg(1e-3, 100))
+    print(f"cfg1 MPIR: GPU {st} (true rel. residual {rel:.3e}), oracle "
+          f"{sto.iterations} outer / {inner} inner", flush=True)
+    assert st.converged and sto.converged
+    assert rel <= 1e-10
+    assert abs(st.iterations - sto.iterations) <= 1
+    if st.iterations == sto.iterations:
+        assert abs(st.innerIterations - inner) <= max(3, 0.1 * inner)
+    # forward error against FP64 PBiCGStab + ILU(0) converged to 1e-14
+    cfg64 = {"type": "PBiCGStab", "relResidual": 1e-14, "maxIterations": 5000,
+             "workingType": "float64",
+             "preconditioner": {"type": "ILU", "factorType": "float64"}}
+    x64 = dev(np.zeros(h.nrows, np.float64))
+    s64 = g.Solver(A, cfg64).solve(x64, dev(b.astype(np.float64)))
+    assert s64.converged and true_rel(A, x64, dev(b)) <= 1e-13
+    xr, xm = x64.numpy(), phi.numpy()
+    fwd = float(np.linalg.norm(xm - xr) / np.linalg.norm(xr))
+    print(f"cfg1 forward error of MPIR vs FP64 (1e-14): {fwd:.3e}", flush=True)
+    assert fwd <= 1e-8
+
+
+# ---- cfg2 -------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def cfg2():
+    h, L = powerlaw_case(4194304, T=4096, seed=2)
+    assert h.nrows == 4194304 and h.num_tiles == 4096
+    return dict(h=h, L=L)
+
+
+def test_cfg2_spmv_full_size_bit_exact(cfg2):
+    h, L = cfg2["h"], cfg2["L"]
+    A = h.copyToTile(g.CRS)
+    x = o.uniform_pm1(51, h.nrows)
+    ref = o.spmv(L, x).view(np.uint32)
+    for sliced in (False, True):
+        A.set_sliced(sliced)
+        assert np.array_equal((A * dev(x)).numpy().view(np.uint32), ref), sliced
+
+
+GS2 = {"type": "GaussSeidel", "numFixedIterations": 2, "symmetric": True}
+
+
+def test_cfg2_solve(cfg2):
+    h, L = cfg2["h"], cfg2["L"]
+    A = h.copyToTile(g.CRS)
+    xt, b = rhs_for(h, L, seed=2)
+    # plain FP32 PBiCGStab + SGS(2) against the oracle
+    cfg = {"type": "PBiCGStab", "relResidual": 1e-8, "maxIterations": 2000,
+           "workingType": "float32", "preconditioner": GS2}
+    x = dev(np.zeros(h.nrows, np.float32))
+    st = g.Solver(A, cfg).solve(x, dev(b))
+    S = o.System(L)
+    _, sto = S.pbicgstab(np.zeros(h.nrows, np.float32), b,
+                         S.pbicgstab_cfg(1e-8, 2000, precond="gs", gsSweeps=2,
+                                         gsSymmetric=True))
+    print(f"cfg2: GPU {st}, oracle {sto.iterations} iterations", flush=True)
+    assert st.converged and sto.converged
+    assert abs(st.iterations - sto.iterations) <= max(1, 0.05 * sto.iterations)
+    # the same inner solve under FP64 refinement reaches a true 1e-8
+    inner = dict(cfg, relResidual=1e-3, maxIterations=200)
+    phi = dev(np.zeros(h.nrows, np.float64))
+    sm = g.Solver(A, mpir_cfg(inner, rel=1e-8)).solve(phi, dev(b))
+    rel = true_rel(A, phi, dev(b))
+    print(f"cfg2 MPIR: {sm}, true rel. residual {rel:.3e}", flush=True)
+    assert sm.converged and rel <= 1e-8
+
+
+# ---- cfg3 -------------------------------------------------------------------
+def make_batch_systems(S=512, seed=3):
+    """512 2D 9-point convection-diffusion systems, sizes log-uniform in
+    [1e3, 2e5] rows (square grids of side round(sqrt(n))), D ~ U[0.5, 2],
+    Pe ~ U[0, 20] per system."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(2e5), S))
+    systems = []
+    for k, nrow in enumerate(sizes):
+        side = max(2, int(round(np.sqrt(nrow))))
+        D, Pe = rng.uniform(0.5, 2.0), rng.uniform(0.0, 20.0)
+        r, c, v = g.generateConvectionDiffusion9Triplets(side, side, D, Pe, 1000 + k)
+        systems.append((side, side, r, c, v))
+    return systems
+
+
+def test_cfg3_batch_512_systems():
+    systems = make_batch_systems()
+    h, ts = g.batch_union(systems, (32, 32))
+    A = h.copyToTile(g.CRS)
+    n = h.nrows
+    xt = h.to_local(g.uniform_pm1(11, n).astype(np.float64))
+    b = A.spmv(dev(xt)).numpy().astype(np.float32)
+    inner = {"type": "PBiCGStab", "relResidual": 1e-3, "maxIterations": 200,
+             "preconditioner": {"type": "ILU"}}
+    bt = g.Batch(A, ts, mpir_cfg(inner, maxit=20))
+    x = g.DeviceVector(n, np.float64)
+    stats, sec = bt.solve(x, dev(b))
+    assert len(stats) == 512 and all(s.converged for s in stats)
+    r = A.residual(x, dev(b), destType=g.FLOAT64, intermediateType=g.FLOAT64).numpy()
+    rs = h.export()["tile_start"][ts]
+    bb = b.astype(np.float64)
+    rel = np.array([np.linalg.norm(r[rs[k]:rs[k + 1]]) / np.linalg.norm(bb[rs[k]:rs[k + 1]])
+                    for k in range(512)])
+    print(f"cfg3: {n} rows, solve {1e3 * sec:.1f} ms, max true rel. residual "
+          f"{rel.max():.2e}", flush=True)
+    assert (rel <= 1e-10).all(), rel.max()
+    # three sampled systems (smallest, largest, one in between) solved alone
+    sizes = np.array([s[0] * s[1] for s in systems])
+    xs = x.numpy()
+    for k in (int(np.argmin(sizes)), int(np.argmax(sizes)), 200):
+        nx, ny, rr, cc, vv = systems[k]
+        nk = nx * ny
+        r2t, T = g.partition_grid3d(nx, ny, 1, 32, 32, 1)
+        hk = g.HostMatrix.from_coo(nk, rr, cc, vv, T, r2t)
+        Ak = hk.copyToTile(g.CRS)
+        # the system's rows in the union are its tile-major rows in order
+        bk = b[rs[k]:rs[k + 1]]
+        xk = dev(np.zeros(nk, np.float64))
+        sk = g.Solver(Ak, mpir_cfg(inner, maxit=20)).solve(xk, dev(bk))
+        assert sk.converged
+        assert abs(sk.iterations - stats[k].iterations) <= 1, (k, sk, stats[k])
+        assert abs(sk.innerIterations - stats[k].innerIterations) <= \
+            max(2, 0.1 * sk.innerIterations), (k, sk, stats[k])
+        xa = xk.numpy()
+        err = np.linalg.norm(xa - xs[rs[k]:rs[k + 1]]) / np.linalg.norm(xa)
+        assert err < 1e-7, (k, err)
+
+
+# ---- cfg4 -------------------------------------------------------------------
+def test_cfg4_serving_rates():
+    """Resident systems of the three generators with sizes log-uniform in
+    [1e4, 2e6] rows; Poisson arrivals at 50, 200 and 800 requests/s, 1-8
+    right-hand sides each, MPIR to 1e-10."""
+    rng = np.random.default_rng(4)
+    mats = []
+    for k in range(6):
+        rows = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        kind = k % 3
+        if kind < 2:
+            m = max(8, int(round(rows ** (1 / 3))))
+            r2t, T = g.partition_grid3d(m, m, m, 16, 8, 8)
+            hk = (g.generate3DPoissonMatrix(m, m, m, numTiles=T, rowToTile=r2t) if kind == 0
+                  else g.generateConvectionDiffusion27Matrix(m, m, m, 10.0, 10 + k,
+                                                             numTiles=T, rowToTile=r2t))
+            mats.append(hk.copyToTile(g.LDU if kind == 1 else g.CRS))
+        else:
+            nr = int(rows)
+            hk = g.generatePowerLawMatrix(nr, seed=10 + k, numTiles=max(1, nr // 1024))
+            mats.append(hk.copyToTile(g.CRS))
+    srv = g.Server(8, 64 << 30)
+    cfg = mpir_cfg(ILU_INNER)
+    ids = [srv.add_system(M, cfg) for M in mats]
+    probe = 1  # a convection-diffusion system
+    alone, sol_a = srv.run([ids[probe]], [3], [0.0], [99],
+                           keep_solution=[(0, mats[probe].nrows)])
+    assert alone["converged"][0] == 1
+    for rate in (50, 200, 800):
+        nreq = 24
+        gaps = rng.exponential(1.0 / rate, nreq)
+        arrival = np.cumsum(gaps) - gaps[0]
+        sysv = rng.integers(0, len(ids), nreq)
+        rhs = rng.integers(1, 9, nreq)
+        seeds = rng.integers(1, 1 << 40, nreq)
+        k = nreq // 2
+        sysv[k], rhs[k], seeds[k] = ids[probe], 3, 99
+        res, sol = srv.run([ids[i] for i in sysv], rhs, arrival, seeds,
+                           keep_solution=[(k, mats[probe].nrows)])
+        print(f"cfg4 at {rate} req/s: p50 {1e3 * np.median(res['latency']):.1f} ms, "
+              f"max rel. residual {res['rel_residual'].max():.2e}", flush=True)
+        assert np.all(res["converged"] == 1)
+        assert np.all(res["rel_residual"] <= 1e-10)
+        assert np.array_equal(sol[k], sol_a[0])

@@ -199,3 +199,19 @@ def test_graph_partition_properties():
     with pytest.raises(g.GrapheneError):
         g.partition_graph(4, [0], [7], 2)
 
+
+
+def test_tile_layout_hand_derived_fixture():
+    """The product's tile layout of the hand-derived 3-tile 4 x 3 grid
+    (tests/test_oracle.py holds the derivation from CRSHostMatrix.cpp:273-354
+    and DistributedTileLayout.cpp:91-126)."""
+    from test_oracle import (HAND_HALO, HAND_NEW_TO_OLD, HAND_NUM_INTERIOR, HAND_NUM_REGIONS,
+                             HAND_R2T, HAND_TILE_START)
+    h = g.generate3DPoissonMatrix(4, 3, 1, numTiles=3, rowToTile=HAND_R2T)
+    e = h.export()
+    assert e["new_to_old"].tolist() == HAND_NEW_TO_OLD
+    assert e["tile_start"].tolist() == HAND_TILE_START
+    assert e["num_interior"].tolist() == HAND_NUM_INTERIOR
+    assert e["num_separator_regions"].tolist() == HAND_NUM_REGIONS
+    hs = e["halo_start"]
+    assert [e["halo_global"][hs[t]:hs[t + 1]].tolist() for t in range(3)] == HAND_HALO

@@ -401,3 +401,157 @@ def test_singular_rhs_sets_flag():
     b = np.zeros(64, np.float32)
     x, st = S.pbicgstab(np.zeros_like(b), b, S.pbicgstab_cfg(1e-6, 50))
     assert st.singular and st.iterations == 0
+
+
+# --- block (T > 1) branches -------------------------------------------------------
+# 2D 5-point Poisson on a 4 x 3 grid (row = 4 j + i), three tiles:
+#   j=2:  8  9 10 11      tiles  1 1 2 2
+#   j=1:  4  5  6  7             0 1 1 2
+#   j=0:  0  1  2  3             0 0 0 2
+# Derived by hand from CRSHostMatrix.cpp:273-354 (a row with no foreign
+# neighbour is interior; otherwise it joins the separator region keyed by
+# its set of neighbour tiles, regions in first-encounter order while rows are
+# visited ascending) and DistributedTileLayout.cpp:91-126 (interior rows,
+# then each separator region's cells):
+#   tile 0 {0,1,2,4}:  0 interior; 1 -> {1}; 2 -> {1,2}; 4 -> {1}
+#                      local order 0 | 1 4 | 2
+#   tile 1 {5,6,8,9}:  5 -> {0}; 6 -> {0,2}; 8 -> {0}; 9 -> {2}
+#                      local order 5 8 | 6 | 9
+#   tile 2 {3,7,10,11}: 3 -> {0}; 7 -> {1}; 10 -> {1}; 11 interior
+#                      local order 11 | 3 | 7 10
+# Halo cells: the foreign columns, taken from each source tile's separator
+# regions in order (source tile ascending; the reference's halo region order
+# follows thread timing, this order is the oracle's documented choice).
+HAND_R2T = np.array([0, 0, 0, 2, 0, 1, 1, 2, 1, 1, 2, 2], np.int32)
+HAND_NEW_TO_OLD = [0, 1, 4, 2, 5, 8, 6, 9, 11, 3, 7, 10]
+HAND_TILE_START = [0, 4, 8, 12]
+HAND_NUM_INTERIOR = [1, 0, 1]
+HAND_NUM_REGIONS = [2, 3, 2]
+HAND_HALO = [[5, 8, 6, 3], [1, 4, 2, 7, 10], [2, 6, 9]]
+
+
+def hand_layout():
+    r, c, v = o.poisson7(4, 3, 1)
+    return oracle_layout(12, r, c, v, 3, HAND_R2T)
+
+
+def test_tile_layout_hand_derived():
+    L = hand_layout()
+    assert L["new_to_old"].tolist() == HAND_NEW_TO_OLD
+    assert L["tile_start"].tolist() == HAND_TILE_START
+    assert L["num_interior"].tolist() == HAND_NUM_INTERIOR
+    assert L["num_separator_regions"].tolist() == HAND_NUM_REGIONS
+    hs = L["halo_start"]
+    assert [L["halo_global"][hs[t]:hs[t + 1]].tolist() for t in range(3)] == HAND_HALO
+    # local columns: in-tile local ids, then halo ids (n_t + position in the
+    # halo list), ascending per row; e.g. global row 2 (tile 0, local 3) has
+    # neighbours 1 (local 1), 6 (halo 2 -> local 6), 3 (halo 3 -> local 7)
+    pos = {g: i for i, g in enumerate(HAND_NEW_TO_OLD)}
+    p = pos[2]
+    rp = L["row_ptr"]
+    assert L["col_local"][rp[p]:rp[p + 1]].tolist() == [1, 6, 7]
+    assert L["col"][rp[p]:rp[p + 1]].tolist() == [pos[1], pos[6], pos[3]]
+    for i in range(12):
+        assert np.all(np.diff(L["col_local"][rp[i]:rp[i + 1]]) > 0)
+
+
+def _tiled_case(T_block=(3, 3, 3), m=6, seed=3):
+    import graphene_amd as g
+    r, c, v = g.generateConvectionDiffusion27Triplets(m, m, m, 10.0, seed)
+    r2t, T = g.partition_grid3d(m, m, m, *T_block)
+    return oracle_layout(m ** 3, r, c, v, T, r2t), T
+
+
+def _block_factors(L, lu, invD):
+    """Block-diagonal unit-lower L and upper U (diagonal 1/invD) from the
+    in-tile factor entries (FP64 dense)."""
+    n = len(invD)
+    ts = L["tile_start"]
+    rows = np.repeat(np.arange(n), np.diff(L["row_ptr"]))
+    tile = np.searchsorted(ts, rows, side="right") - 1
+    intile = (L["col"] >= ts[tile]) & (L["col"] < ts[tile + 1])
+    Lm = np.eye(n)
+    Um = np.diag(1.0 / invD)
+    lo = intile & (L["col"] < rows)
+    up = intile & (L["col"] > rows)
+    Lm[rows[lo], L["col"][lo]] = lu[lo]
+    Um[rows[up], L["col"][up]] = lu[up]
+    return Lm, Um, intile
+
+
+def test_ilu0_block_identity_T8():
+    """Per tile, (L U)_ij = A_ij on the tile block's pattern; entries that
+    couple to another tile (halo columns) are left as they are."""
+    L, T = _tiled_case()
+    assert T == 8
+    lu, invD = o.ilu0_factor(L, np.float64)
+    Lm, Um, intile = _block_factors(L, lu, invD)
+    assert np.array_equal(lu[~intile], L["off"][~intile].astype(np.float64))
+    A = to_scipy(L).toarray()
+    ts = L["tile_start"]
+    LU = Lm @ Um
+    for t in range(T):
+        a, b = ts[t], ts[t + 1]
+        blk = A[a:b, a:b]
+        mask = blk != 0
+        assert np.allclose(LU[a:b, a:b][mask], blk[mask], rtol=1e-12, atol=1e-12), t
+        # block-diagonal: no coupling between tiles in the factors
+        assert not LU[a:b, :a].any() and not LU[a:b, b:].any()
+
+
+def test_ilu0_block_apply_vs_triangular_solves_T8():
+    L, T = _tiled_case(seed=5)
+    lu, invD = o.ilu0_factor(L, np.float64)
+    Lm, Um, _ = _block_factors(L, lu, invD)
+    n = len(invD)
+    b = o.uniform_pm1(21, n).astype(np.float64)
+    y = spla.spsolve_triangular(sp.csr_matrix(Lm), b, lower=True)
+    x = spla.spsolve_triangular(sp.csr_matrix(Um), y, lower=False)
+    assert np.allclose(o.ilu0_apply(L, lu, invD, b), x, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("random_partition", [False, True])
+def test_gs_block_sweeps_read_halo_before_sweep(random_partition):
+    """Block Gauss-Seidel (gauss-seidel/Solver.cpp:45-160): the halo is
+    exchanged before the sweep, so each tile sweeps its rows in local order
+    with the newest values of its own rows and the pre-sweep values of every
+    other tile's rows."""
+    m = 6
+    n = m ** 3
+    if random_partition:
+        r, c, v = o.poisson7(m, m, m)
+        r2t = np.random.default_rng(8).integers(0, 5, n).astype(np.int32)
+        L = oracle_layout(n, r, c, v, 5, r2t)
+    else:
+        L, _ = _tiled_case(seed=7)
+    A = to_scipy(L).toarray()
+    ts = L["tile_start"]
+    b = o.uniform_pm1(22, n).astype(np.float64)
+    x0 = o.uniform_pm1(23, n).astype(np.float64)
+
+    def sweep(xin, backward):
+        x = xin.copy()
+        for t in range(len(ts) - 1):
+            a, e = ts[t], ts[t + 1]
+            rng_ = range(e - 1, a - 1, -1) if backward else range(a, e)
+            for i in rng_:
+                s = b[i]
+                for j in np.nonzero(A[i])[0]:
+                    if j != i:
+                        s -= A[i, j] * (x[j] if a <= j < e else xin[j])
+                x[i] = s / A[i, i]
+        return x
+
+    xf = sweep(x0, False)
+    assert np.allclose(o.gs_sweep(L, b, x0), xf, rtol=1e-13, atol=1e-14)
+    xb = sweep(xf, True)
+    assert np.allclose(o.gs_sweep(L, b, xf, backward=True), xb, rtol=1e-13, atol=1e-14)
+    # a sweep that read the other tiles' updated values would differ
+    assert not np.allclose(xf, _gs_global(A, b, x0))
+
+
+def _gs_global(A, b, x0):
+    x = x0.copy()
+    for i in range(len(b)):
+        x[i] = (b[i] - A[i, :i] @ x[:i] - A[i, i + 1:] @ x[i + 1:]) / A[i, i]
+    return x
