@@ -99,6 +99,7 @@ _SIGS = {
     "gph_spmv": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int], C.c_int),
     "gph_residual": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P], C.c_int),
     "gph_vector_norm": ([_P, C.c_int, _P, C.POINTER(C.c_double)], C.c_int),
+    "gph_vector_norm_noalloc": ([_P, C.c_int, _P, C.POINTER(C.c_double)], C.c_int),
     "gph_bench_spmv": ([_P, C.c_int, C.c_int, _P, _P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "gph_solver_create": ([_P, C.c_char_p, C.POINTER(_P)], C.c_int),
     "gph_solver_destroy": ([_P], None),

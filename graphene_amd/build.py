@@ -33,6 +33,7 @@ SOURCES = [
     "k_ilu_level.hip",
     "k_ilu_rows.hip",
     "k_batch.hip",
+    "k_noalloc.hip",
 ]
 
 COMMON = [

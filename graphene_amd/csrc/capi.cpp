@@ -478,6 +478,15 @@ int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
     *out = A->A->norm2(Vec{const_cast<void*>(x), dt(t)}, A->A->stream());
   });
 }
+int gph_vector_norm_noalloc(const gph_matrix* A, gph_type t, const void* x,
+                            double* out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(x, "x");
+    need(out, "out");
+    *out = gph::norm2NoAlloc(x, dt(t), A->A->n(), A->A->stream());
+  });
+}
 int gph_bench_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
                    const void* x, void* y, int32_t iters, double* avg_ms) {
   return guard([&] {

@@ -91,6 +91,10 @@ void launchUpdateS(const VecArgs& v, DType x, hipStream_t s);
 // sum of squares of a (type t), finalised with v.fin into the state
 void launchNorm2(const VecArgs& v, DType t, hipStream_t s);
 void launchFill(void* x, DType t, int64_t n, double value, hipStream_t s);
+// ||x||_2 with the reduction scratch in a static device array instead of an
+// allocation (k_noalloc.hip); synchronous on stream s.
+double norm2NoAlloc(const void* x, DType t, int64_t n, hipStream_t s);
+size_t noAllocScratchBytes();
 // x_i = U[-1,1) from a counter-based hash of (seed, i) (device-side RHS)
 void launchUniform(double* x, int64_t n, uint64_t seed, hipStream_t s);
 void launchCast(const void* in, DType it, void* out, DType ot, int64_t n,

@@ -209,6 +209,15 @@ int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
 /* ||x||_2 (Matrix::vectorNorm, lg/matrix/Matrix.hpp:115-117). */
 int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
                     double* out);
+/* No reference counterpart: ||x||_2 as gph_vector_norm (same kernel, same
+ * result bits), for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops). It calls no hipMalloc/hipFree: the
+ * reduction scratch is a static device array of 33,024 bytes per GPU, held
+ * until the process ends. n is the matrix's row count, at most 2^31 - 1.
+ * Calls that overlap on one GPU share that array and race: issue them one
+ * at a time per GPU. */
+int gph_vector_norm_noalloc(const gph_matrix* A, gph_type t, const void* x,
+                            double* out);
 /* Mean device time (ms) of `iters` back-to-back SpMV launches, HIP events on
  * the launch stream. */
 int gph_bench_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
