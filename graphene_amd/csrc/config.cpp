@@ -322,10 +322,7 @@ ilu::Configuration::Configuration(const Json& c) {
   readBool(c, "diagonalBased", diagonalBased);
   readType(c, "factorType", factorType);
   readInt(c, "rowsPerWave", rowsPerWave);
-  readInt(c, "tilesPerWave", tilesPerWave);
-  if (tilesPerWave < 0 || tilesPerWave > 2)
-    fail(kInvalid, "Invalid value for tilesPerWave: " + std::to_string(tilesPerWave) +
-                       ". Allowed values are 0, 1 or 2");
+
   if (const Json* j = c.get("substitution")) {
     substitution = asText(*j);
     if (substitution != "level" && substitution != "packed" && substitution != "tile" &&

@@ -114,8 +114,6 @@ struct Configuration : solver::Configuration {
   // (product slabs), "level" (lane per row over level records), "tile" (row
   // blocks)
   std::string substitution = "rows";
-  // extension: tiles per wave of the "rows" kernel: 1, 2, or 0 (default: 1)
-  int tilesPerWave = 0;
   explicit Configuration(const Json& c);
   std::string solverName() const override {
     return diagonalBased ? "DILU" : "ILU(0)";

@@ -359,7 +359,7 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
   return o;
 }
 
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int perStep) {
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
   IluRowStream o;
   const int32_t T = m.numTiles;
   for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
@@ -374,13 +374,18 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int 
   o.meta.reserve((size_t)2 * m.nrows);
   std::vector<int64_t> rows, off;
   std::vector<int32_t> ents, order;
-  // one record: row r's entries [e0, e0 + l) (r < 0: an empty lane)
+  // one record: row r's entries [e0, e0 + l) (r < 0: none); a column holds
+  // one long record (rec[0], `lng`) or up to two short ones
   struct Rec {
-    int32_t r;
-    int64_t e0, l;
-    uint32_t flags;
+    int32_t r = -1;
+    int64_t e0 = 0, l = 0;
+    uint32_t flags = 0;  // kRowsCont | kRowsPartial
   };
-  std::vector<Rec> step;
+  struct Column {
+    bool lng = false;
+    Rec rec[2];
+  };
+  std::vector<Column> step;
   for (int32_t t = 0; t < T; t++) {
     o.elemOff[(size_t)t] = (uint32_t)o.col.size();
     o.hdrOff[(size_t)t] = (int32_t)o.hdr.size();
@@ -397,68 +402,102 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int 
         auto len = [&](int32_t r) { return off[(size_t)r + 1] - off[(size_t)r]; };
         std::stable_sort(order.begin(), order.end(),
                          [&](int32_t a, int32_t b) { return len(a) > len(b); });
-        auto emit = [&]() {
-          int64_t W = 0, minLen = kRowsMaxLen, elems = 0;
-          for (const Rec& c : step) {
-            W = std::max(W, c.l);
-            minLen = std::min(minLen, c.l);
-            const int64_t slot = c.r >= 0 ? rows[(size_t)c.r] - t0 + 1 : 0;
-            const int64_t padded = (c.l + kRowsPad - 1) / kRowsPad * kRowsPad;
-            o.meta.push_back((uint32_t)elems | (uint32_t)padded << 11 | c.flags << 16 |
-                             (uint32_t)slot << 18);
-            for (int64_t e = 0; e < c.l; e++) {
-              const int32_t a = ents[(size_t)(off[(size_t)c.r] + c.e0 + e)];
-              o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
-              o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
-            }
-            for (int64_t e = c.l; e < padded; e++) {
-              o.col.push_back(zero);
-              o.val.insert(o.val.end(), (size_t)es, 0);
-            }
-            elems += padded;
-          }
-          o.hdr.push_back((uint32_t)step.size() | (uint32_t)W << 7 | (uint32_t)minLen << 12 |
-                          (uint32_t)elems << 17 | (backward ? 1u << 31 : 0u));
-          step.clear();
-        };
-        // rows longer than kRowsMaxLen (first in `order`): chains of steps,
-        // the row's k-th chunk in the k-th step at the same lane (the lane
-        // carries the partial sum); free lanes take short rows
         int32_t nLong = 0;
-        while (nLong < R && len(order[(size_t)nLong]) > kRowsMaxLen) nLong++;
+        while (nLong < R && len(order[(size_t)nLong]) > kRowsShortLen) nLong++;
         int32_t next = nLong;  // next short row
-        for (int32_t c0 = 0; c0 < nLong; c0 += perStep) {
-          const int32_t lanes = std::min<int32_t>(perStep, nLong - c0);
-          const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
-          for (int64_t k = 0; k < chunks; k++) {
-            for (int32_t q = 0; q < lanes; q++) {
-              const int32_t r = order[(size_t)(c0 + q)];
-              const int64_t nk = (len(r) + kRowsMaxLen - 1) / kRowsMaxLen;
-              if (k < nk) {
-                const int64_t e0 = k * kRowsMaxLen;
-                step.push_back({r, e0, std::min<int64_t>(kRowsMaxLen, len(r) - e0),
-                                (k > 0 ? 1u : 0u) | (k + 1 < nk ? 2u : 0u)});
-              } else if (next < R) {
-                const int32_t s = order[(size_t)next++];
-                step.push_back({s, 0, len(s), 0u});
+        auto shortRec = [&]() {
+          Rec c;
+          if (next < R) {
+            c.r = order[(size_t)next++];
+            c.l = len(c.r);
+          }
+          return c;
+        };
+        // fills the free columns with short rows and emits the step
+        auto emit = [&](std::vector<Column>& cols) {
+          for (Column& c : cols)
+            if (!c.lng && c.rec[0].r < 0) {
+              c.rec[0] = shortRec();
+              c.rec[1] = shortRec();
+            }
+          while ((int32_t)cols.size() < kRowsColumns && next < R) {
+            Column c;
+            c.rec[0] = shortRec();
+            c.rec[1] = shortRec();
+            cols.push_back(c);
+          }
+          while (!cols.empty() && !cols.back().lng && cols.back().rec[0].r < 0) cols.pop_back();
+          // record words, then the chunks in lane order: lane l (lane row
+          // l >> 4, column l & 15) takes record word 2 (l & 15) + (l >> 5)
+          // and, of a short record, chunk (l >> 4) & 1, of a long record
+          // chunk g(l >> 4) with g = 0, 1, 3, 2; lanes without a chunk
+          // store nothing
+          bool anyLong = false;
+          std::vector<const Rec*> word;
+          for (const Column& c : cols) {
+            for (int hh = 0; hh < 2; hh++) {
+              const Rec& rc = c.lng ? c.rec[0] : c.rec[hh];
+              word.push_back(&rc);
+              uint32_t w = 0;
+              if (rc.r >= 0) {
+                const int64_t chunks = (rc.l + kRowsPad - 1) / kRowsPad;
+                const int64_t slot = rows[(size_t)rc.r] - t0 + 1;
+                w = (uint32_t)chunks << 10 | (c.lng ? kRowsLong : 0u) | rc.flags |
+                    (uint32_t)slot << 16;
+              }
+              o.meta.push_back(w);
+            }
+            anyLong = anyLong || c.lng;
+          }
+          int64_t elems = 0;
+          for (int l = 0; l < 64; l++) {
+            const int lr = l >> 4;
+            const size_t wi = (size_t)(2 * (l & 15) + (l >> 5));
+            if (wi >= word.size() || word[wi]->r < 0) continue;
+            const Rec& rc = *word[wi];
+            const bool lng = cols[wi / 2].lng;
+            const int64_t ch = lng ? (lr ^ (lr >> 1)) : (lr & 1);
+            const int64_t chunks = (rc.l + kRowsPad - 1) / kRowsPad;
+            if (ch >= chunks) continue;
+            for (int64_t e = ch * kRowsPad; e < (ch + 1) * kRowsPad; e++) {
+              if (e < rc.l) {
+                const int32_t a = ents[(size_t)(off[(size_t)rc.r] + rc.e0 + e)];
+                o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
+                o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
               } else {
-                step.push_back({-1, 0, 0, 2u});
+                o.col.push_back(zero);
+                o.val.insert(o.val.end(), (size_t)es, 0);
               }
             }
-            while ((int32_t)step.size() < perStep && next < R) {
-              const int32_t s = order[(size_t)next++];
-              step.push_back({s, 0, len(s), 0u});
+            elems += kRowsPad;
+          }
+          o.hdr.push_back((uint32_t)elems | (uint32_t)(2 * cols.size()) << 12 |
+                          (anyLong ? 1u << 30 : 0u) | (backward ? 1u << 31 : 0u));
+          cols.clear();
+        };
+        // long rows, one column each, kRowsColumns at a time; a row longer
+        // than kRowsMaxLen continues in the same column of the next steps
+        for (int32_t c0 = 0; c0 < nLong; c0 += kRowsColumns) {
+          const int32_t n = std::min<int32_t>(kRowsColumns, nLong - c0);
+          const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
+          for (int64_t k = 0; k < chunks; k++) {
+            for (int32_t q = 0; q < n; q++) {
+              const int32_t r = order[(size_t)(c0 + q)];
+              const int64_t nk = (len(r) + kRowsMaxLen - 1) / kRowsMaxLen;
+              Column c;
+              if (k < nk) {
+                c.lng = true;
+                c.rec[0].r = r;
+                c.rec[0].e0 = k * kRowsMaxLen;
+                c.rec[0].l = std::min<int64_t>(kRowsMaxLen, len(r) - c.rec[0].e0);
+                c.rec[0].flags = (k > 0 ? kRowsCont : 0u) | (k + 1 < nk ? kRowsPartial : 0u);
+              }
+              step.push_back(c);
             }
-            emit();
+            emit(step);
           }
         }
-        while (next < R) {
-          while ((int32_t)step.size() < perStep && next < R) {
-            const int32_t s = order[(size_t)next++];
-            step.push_back({s, 0, len(s), 0u});
-          }
-          emit();
-        }
+        while (next < R) emit(step);
       }
     }
     o.maxSteps = std::max<int32_t>(o.maxSteps, (int32_t)o.hdr.size() - o.hdrOff[(size_t)t]);

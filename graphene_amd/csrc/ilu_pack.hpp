@@ -93,36 +93,46 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
                              int64_t maxStepBytes, int64_t ring, int64_t tail,
                              int64_t wAlign);
 
-// Row-per-lane stream of the ILU(0) factors (one wave per tile, two lanes
-// per row of the current level).
+// Row stream of the ILU(0) factors for the one-wave-per-tile substitution
+// kernel (k_ilu_rows.hip), whose 64 lanes form 4 rows of 16.
 //
 // Per tile, the forward then the backward substitution is a sequence of
-// steps; a step holds up to perStep (<= 32) row records of one level (the rows
-// of a level are independent), longest first. A record is a row's entries
-// (forward: strictly lower in-tile entries ascending; backward: strictly
-// upper in-tile entries descending, the order ilu/Solver.cpp:380-442 visits
-// them), stored contiguously and padded with zero entries to a multiple of
-// kRowsPad: column as a 16-bit byte offset into the tile's x slots (4 *
-// slot; slot 0 holds +0, row i of the tile is slot i + 1) and the factor
-// value, both at the same element index of their streams. A zero entry
-// (value +0, column slot 0) adds a +0 product, which leaves every partial
-// sum unchanged. A row with more than kRowsMaxLen entries is split into
-// chunks placed at the same lanes of consecutive steps of its level; the
-// lanes carry the partial sum from one chunk to the next (flags: continue
-// from the carried sum; partial, the row is not finished). Free lanes of
-// such steps take other rows of the level or empty records (partial, no
-// entries).
-//   step header (u32): records R (bits 0-6), longest record W (7-11),
-//     shortest record (12-16), elements of the step (17-27), backward (31)
-//   record word (u32): element offset within the step (bits 0-10), padded
-//     entry count (11-15), continue (16), partial (17), the row's slot
-//     (18-31)
+// steps, each holding rows of one level (the rows of a level are
+// independent). A step has up to kRowsColumns columns; a column holds either
+// two short records (at most kRowsShortLen entries each: lane rows 0-1 and
+// 2-3) or one long record (at most kRowsMaxLen entries, all four lane rows).
+// A record is a row's entries (forward: strictly lower in-tile entries
+// ascending; backward: strictly upper in-tile entries descending, the order
+// ilu/Solver.cpp:380-442 visits them), cut into chunks of kRowsPad entries,
+// the last chunk padded with zero entries. A step stores its chunks in the
+// order of the lanes that hold them (lane l: lane row l >> 4, column l & 15,
+// record word 2 (l & 15) + (l >> 5); chunk (l >> 4) & 1 of a short record,
+// chunk 0, 1, 3, 2 for lane rows 0-3 of a long record; lanes without a chunk
+// store nothing). An entry is its column as a
+// 16-bit byte offset into the tile's x slots (4 * slot; slot 0 holds +0,
+// row i of the tile is slot i + 1) and the factor value, at the same element
+// index of their streams. A zero entry (value +0, column slot 0) adds a +0
+// product, which leaves every partial sum unchanged. A row with more than
+// kRowsMaxLen entries takes the same column of consecutive steps of its
+// level, kRowsMaxLen entries per step; the lanes carry the partial sum from
+// one step to the next (flags: continue from the carried sum; partial, the
+// row is not finished). Rows are placed longest first.
+//   step header (u32): elements of the step (bits 0-11), record words R =
+//     2 * columns (12-17), a long record present (30), backward (31)
+//   record words (u32, two per column, the same word twice for a long
+//     record, 0 for an empty half): chunks (bits 10-12), long (13),
+//     continue (14), partial (15), the row's slot (16-31)
 // A tile's steps use elements [elemOff[t], elemOff[t+1]), headers
 // [hdrOff[t], hdrOff[t+1]) and record words [metaOff[t], metaOff[t+1]).
 // The element streams end with kRowsTail elements of zero entries.
-constexpr int kRowsMaxLen = 16;
+constexpr int kRowsShortLen = 16;
+constexpr int kRowsMaxLen = 32;
 constexpr int kRowsPad = 8;
+constexpr int kRowsColumns = 16;
 constexpr int kRowsTail = 64;
+constexpr uint32_t kRowsLong = 1u << 13;
+constexpr uint32_t kRowsCont = 1u << 14;
+constexpr uint32_t kRowsPartial = 1u << 15;
 struct IluRowStream {
   std::vector<uint16_t> col;       // byte offsets (4 * local id)
   std::vector<uint8_t> val;        // factor values (es bytes each)
@@ -137,9 +147,7 @@ struct IluRowStream {
   bool ok = false;  // false: ids or the stream too large for the formats
 };
 
-// perStep: records per step (32 when a wave runs one tile, 16 when it
-// runs two).
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, int perStep);
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es);
 
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,

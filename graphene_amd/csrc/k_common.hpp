@@ -59,6 +59,21 @@ __device__ __forceinline__ double cvt<double, __half>(__half v) {
   return (double)__half2float(v);
 }
 
+// Element type of the inverted diagonal stored with ILU(0) factors of type
+// FT: float16 off-diagonal factors keep a float32 inverse diagonal (one
+// value per row, so the narrow storage saves little there, while rounding
+// it to float16 perturbs every row of the substitution).
+template <typename FT>
+struct InvDiagOf {
+  using type = FT;
+};
+template <>
+struct InvDiagOf<__half> {
+  using type = float;
+};
+template <typename FT>
+using InvDiag = typename InvDiagOf<FT>::type;
+
 // Blocks are dispatched to the 8 XCDs round robin (block b -> XCD b % 8).
 // Remap so that each XCD works on one contiguous range of logical blocks,
 // keeping the gathered x window of neighbouring rows in one L2.

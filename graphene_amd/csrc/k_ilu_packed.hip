@@ -199,7 +199,7 @@ __device__ __forceinline__ void packedSweep(const PackedDirArgs& d, int tile,
 template <typename FT, typename X>
 struct PackedStage {
   int t0, nt;
-  const FT* invD;
+  const InvDiag<FT>* invD;
   const uint16_t* locF;
   const uint16_t* locB;
   X* xs;
@@ -211,7 +211,7 @@ struct PackedStage {
     const int lane = threadIdx.x;
     for (int q0 = 0; q0 < nt; q0 += kStage * kPackedThreads) {
       X a[kStage];
-      FT dv[kStage];
+      InvDiag<FT> dv[kStage];
       uint16_t rf[kStage], rb[kStage];
 #pragma unroll
       for (int k = 0; k < kStage; k++) {
@@ -239,7 +239,7 @@ struct PackedStage {
 
 template <typename FT, typename X>
 __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
-    PackedArgs p, const FT* __restrict__ invD, const X* __restrict__ rhs,
+    PackedArgs p, const InvDiag<FT>* __restrict__ invD, const X* __restrict__ rhs,
     X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
@@ -335,7 +335,7 @@ static void packedTyped(const PackedArgs& p, const void* invD, const void* rhs,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
   hipLaunchKernelGGL(k, dim3(p.T), dim3(kPackedThreads), shm, s, p,
-                     static_cast<const FT*>(invD), static_cast<const X*>(rhs),
+                     static_cast<const InvDiag<FT>*>(invD), static_cast<const X*>(rhs),
                      static_cast<X*>(out), pre, st, checkDone);
   GPH_HIP(hipGetLastError());
 }

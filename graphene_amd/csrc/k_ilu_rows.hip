@@ -1,30 +1,35 @@
-// ILU(0) substitution, two lanes per row, factor rows in registers (gfx950).
+// ILU(0) substitution with factor records in registers (gfx950).
 //
 // Same arithmetic as ilu/Solver.cpp:380-442 / :526-548 (block-Jacobi over
 // tiles, forward then backward substitution per tile). One wave per tile;
 // the tile's x (right-hand side with the fused PBiCGStab p- or s-update),
-// 1/diag, record words and step headers are staged in LDS. A step is up to
-// 32 row records of one level; lanes r and r + 32 own record r, entries 0-7
-// and 8-15: each reads the x values of its entries' columns from LDS (all
-// at once: they belong to earlier levels) and forms the products, then lane
-// r subtracts its products from the row's partial sum in the reference's
-// order and hands the sum to lane r + 32 (v_permlane32_swap), which
-// subtracts its products and finishes the row. Records are padded with
-// zero entries (value +0, the +0 slot) to a multiple of 8, and the lanes
-// past a record's padded end or past the step's records load nothing (their
-// buffer offsets are out of range, so the loads return +0 values and slot
-// 0): their +0 products leave the sum unchanged. A row longer
-// than kRowsMaxLen is split over consecutive steps at the same lanes, which
-// carry the partial sum in a register. A step's LDS write precedes the next
-// step's reads in the wave's program order, and LDS executes a wave's
-// operations in order, so no barrier separates levels.
+// 1/diag, record words and step headers are staged in LDS. The wave's 64
+// lanes are 4 rows of 16; a step is up to 16 columns of rows of one level
+// (stream format in ilu_pack.hpp). Each lane holds one chunk of 8 entries of
+// a record: a short record (<= 16 entries) uses lane rows 0 and 1 (or 2 and
+// 3) of its column, a long record (<= 32 entries) all four rows in the order
+// 0, 1, 3, 2; a step stores its chunks in lane order, so each wave-wide load
+// reads one contiguous range. Every lane reads the x values of its entries' columns from LDS
+// (all at once: they belong to earlier levels) and forms its products; then
+// the row's partial sum passes from lane to lane in entry order
+// (v_permlane16_swap between rows 0-1 / 2-3, v_permlane32_swap between rows
+// 1-3 and 2-0), each lane subtracting its products in the reference's order,
+// and the record's last lane finishes the row and writes it (bit-exact with
+// the sequential loop). The long-record passes run only in steps that hold
+// a long record. Chunks are padded with zero entries (value +0, the +0
+// slot); lanes without a chunk load nothing (their buffer offsets are out of
+// range, so the loads return +0 values and slot 0): their +0 products leave
+// the sum unchanged. A row longer than 32 entries continues in the same
+// column of the next steps, the lanes carrying the partial sum in a
+// register. A step's LDS write precedes the next step's reads in the wave's
+// program order, and LDS executes a wave's operations in order, so no
+// barrier separates levels.
 //
-// Every lane's half record (kRowsMaxLen / 2 values and 16-bit column
-// offsets) sits in a ring of D register
-// slots: the loads of step k + D are issued when step k is done, so memory
-// latency is covered by D steps of work. Steps past a tile's end have zero
-// headers (no records), so the loop needs no bounds tests and every ring
-// slot is refilled on every path.
+// Every lane's chunk (8 values and 16-bit column offsets) sits in a ring of
+// D register slots: the loads of step k + D are issued when step k is done,
+// so memory latency is covered by D steps of work. Steps past a tile's end
+// have zero headers (no records), so the loop needs no bounds tests and
+// every ring slot is refilled on every path.
 #include "kernels.hpp"
 
 namespace gph {
@@ -57,7 +62,7 @@ constexpr int rowsDepth() {
 
 size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt) {
   const int D = ft == DType::F64 ? rowsDepth<double>() : rowsDepth<float>();
-  return (size_t)(p.tilesPerWave == 2 ? 2 : 1) * rowsLayout(p, dtypeSize(xt), D).total;
+  return rowsLayout(p, dtypeSize(xt), D).total;
 }
 
 // 16 bytes with 4-byte alignment: the row streams are dword aligned only.
@@ -92,7 +97,9 @@ __device__ __forceinline__ Words4 bufLoad16(__amdgpu_buffer_rsrc_t r, int voff, 
 }
 
 // Lanes 0-31 of v moved to lanes 32-63 (lanes 0-31 keep their own), and
-// lanes 32-63 moved to lanes 0-31: gfx950 v_permlane32_swap.
+// lanes 32-63 moved to lanes 0-31: gfx950 v_permlane32_swap. Lane rows of
+// 16 (v_permlane16_swap): rows 0 and 2 moved to rows 1 and 3 (rowUp), rows
+// 1 and 3 moved to rows 0 and 2 (rowDown).
 __device__ __forceinline__ float lowToHigh(float v) {
   const uint32_t u = __float_as_uint(v);
   return __uint_as_float(__builtin_amdgcn_permlane32_swap(u, u, false, false)[0]);
@@ -110,6 +117,25 @@ __device__ __forceinline__ double highToLow(double v) {
   const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
   return __hiloint2double((int)__builtin_amdgcn_permlane32_swap(hi, hi, false, false)[1],
                           (int)__builtin_amdgcn_permlane32_swap(lo, lo, false, false)[1]);
+}
+
+__device__ __forceinline__ float rowUp(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane16_swap(u, u, false, false)[0]);
+}
+__device__ __forceinline__ float rowDown(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return __uint_as_float(__builtin_amdgcn_permlane16_swap(u, u, false, false)[1]);
+}
+__device__ __forceinline__ double rowUp(double v) {
+  const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
+  return __hiloint2double((int)__builtin_amdgcn_permlane16_swap(hi, hi, false, false)[0],
+                          (int)__builtin_amdgcn_permlane16_swap(lo, lo, false, false)[0]);
+}
+__device__ __forceinline__ double rowDown(double v) {
+  const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
+  return __hiloint2double((int)__builtin_amdgcn_permlane16_swap(hi, hi, false, false)[1],
+                          (int)__builtin_amdgcn_permlane16_swap(lo, lo, false, false)[1]);
 }
 
 // Copies n words to LDS and zero-fills up to nz, kStageW loads in flight.
@@ -132,13 +158,13 @@ __device__ __forceinline__ void stageWords(uint32_t* dst, const uint32_t* __rest
   }
 }
 
-template <typename FT, typename X, typename F>
-__device__ __forceinline__ void stageRows(int t0, int nt, const FT* __restrict__ invD, X* xs,
+template <typename DT, typename X, typename F>
+__device__ __forceinline__ void stageRows(int t0, int nt, const DT* __restrict__ invD, X* xs,
                                           X* invd, F f, X* wb) {
   const int lane = threadIdx.x;
   for (int q0 = 0; q0 < nt; q0 += kStageW * kRowsThreads) {
     X a[kStageW];
-    FT dv[kStageW];
+    DT dv[kStageW];
 #pragma unroll
     for (int k = 0; k < kStageW; k++) {
       const int q = q0 + k * kRowsThreads + lane;
@@ -158,78 +184,64 @@ __device__ __forceinline__ void stageRows(int t0, int nt, const FT* __restrict__
   }
 }
 
-template <typename FT, typename X, int TPW>
+template <typename FT, typename X>
 __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
-    RowsArgs p, const FT* __restrict__ invD, const X* __restrict__ rhs,
+    RowsArgs p, const InvDiag<FT>* __restrict__ invD, const X* __restrict__ rhs,
     X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
   constexpr int D = rowsDepth<FT>();
-  constexpr int kHalf = kRowsMaxLen / 2;                // entries per lane
-  constexpr int NV = (kHalf * (int)sizeof(FT)) / 16;    // value words per lane
-  constexpr int kRecs = 32 / TPW;                       // records per tile and step
+  constexpr int NV = (kRowsChunk * (int)sizeof(FT)) / 16;  // 16-byte value words per lane
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
-  const RowsLds L = rowsLayout(p, sizeof(X), D);  // one region per tile of the wave
+  const int tile = blockIdx.x;
+  // batched solves: the tile's system has finished
+  if (p.tileSys && p.sysState[p.tileSys[tile]].done) return;
+  const RowsLds L = rowsLayout(p, sizeof(X), D);
   const int lane = threadIdx.x;
-  const int tileBase = blockIdx.x * TPW;
-  auto live = [&](int tile) {  // a tile to solve (batched: its system is running)
-    return tile < p.T && !(p.tileSys && p.sysState[p.tileSys[tile]].done);
-  };
+  X* xs = reinterpret_cast<X*>(smem + L.xs);
+  X* invd = reinterpret_cast<X*>(smem + L.invd);
+  uint32_t* meta = reinterpret_cast<uint32_t*>(smem + L.meta);
+  uint32_t* hdr = reinterpret_cast<uint32_t*>(smem + L.hdr);
 
-  // every tile of the wave: step headers and record words (zero-filled past
-  // the tile's own), right-hand side (with the fused PBiCGStab update) and
-  // 1/diag into its LDS region
+  // step headers and record words (zero-filled past the tile's own),
+  // right-hand side (with the fused PBiCGStab update) and 1/diag into LDS
+  const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
+  const int h0 = p.hdrOff[tile], nS = p.hdrOff[tile + 1] - h0;
+  const int64_t m0 = p.metaOff[tile];
+  const int nRec = (int)(p.metaOff[tile + 1] - m0);
+  stageWords(meta, p.meta + m0, nRec, nRec + 64);
+  stageWords(hdr, p.hdr + h0, nS, p.maxSteps + 2 * D + 1);
   const X* r = static_cast<const X*>(pre.r);
   X* pv = static_cast<X*>(pre.p);
   const X* vv = static_cast<const X*>(pre.v);
-  int nSmax = 0;
-#pragma unroll
-  for (int tt = 0; tt < TPW; tt++) {
-    const int tile = tileBase + tt;
-    unsigned char* reg = smem + (size_t)tt * L.total;
-    X* xs = reinterpret_cast<X*>(reg + L.xs);
-    X* invd = reinterpret_cast<X*>(reg + L.invd);
-    uint32_t* meta = reinterpret_cast<uint32_t*>(reg + L.meta);
-    uint32_t* hdr = reinterpret_cast<uint32_t*>(reg + L.hdr);
-    const bool on = live(tile);
-    const int t0 = on ? p.tileStart[tile] : 0, nt = on ? p.tileStart[tile + 1] - t0 : 0;
-    const int h0 = on ? p.hdrOff[tile] : 0, nS = on ? p.hdrOff[tile + 1] - h0 : 0;
-    const int64_t m0 = on ? p.metaOff[tile] : 0;
-    const int nRec = on ? (int)(p.metaOff[tile + 1] - m0) : 0;
-    nSmax = nS > nSmax ? nS : nSmax;
-    stageWords(meta, p.meta + m0, nRec, nRec + 64);
-    stageWords(hdr, p.hdr + h0, nS, p.maxSteps + 2 * D + 1);
-    if (pre.kind == kPreP) {
-      if (st->iter != 0) {
-        const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
-        const X omega = (X)st->omega;
-        stageRows(t0, nt, invD, xs, invd,
-                  [&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); }, pv);
-      } else {
-        stageRows(t0, nt, invD, xs, invd, [&](int i) { return pv[i]; }, (X*)nullptr);
-      }
-    } else if (pre.kind == kPreS) {
-      const X alpha = (X)st->alpha;
-      stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] - alpha * vv[i]; }, pv);
+  if (pre.kind == kPreP) {
+    if (st->iter != 0) {
+      const X beta = ((X)st->rho / (X)st->rhoOld) * ((X)st->alpha / (X)st->omega);
+      const X omega = (X)st->omega;
+      stageRows(t0, nt, invD, xs, invd,
+                [&](int i) { return r[i] + beta * (pv[i] - omega * vv[i]); }, pv);
     } else {
-      stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
+      stageRows(t0, nt, invD, xs, invd, [&](int i) { return pv[i]; }, (X*)nullptr);
     }
-    if (lane == 0) xs[0] = (X)0;  // the zero slot
+  } else if (pre.kind == kPreS) {
+    const X alpha = (X)st->alpha;
+    stageRows(t0, nt, invD, xs, invd, [&](int i) { return r[i] - alpha * vv[i]; }, pv);
+  } else {
+    stageRows(t0, nt, invD, xs, invd, [&](int i) { return rhs[i]; }, (X*)nullptr);
   }
+  if (lane == 0) xs[0] = (X)0;  // the zero slot
   __syncthreads();
-  if (nSmax == 0) return;
+  if (nS == 0) return;
 
-  // this lane: tile tl of the wave, half `half` of record rl's entries
-  const int tl = TPW == 2 ? (lane >> 4) & 1 : 0;
-  const int rl = lane & (kRecs - 1), half = lane >> 5;
-  unsigned char* reg = smem + (size_t)tl * L.total;
-  X* xs = reinterpret_cast<X*>(reg + L.xs);
-  const X* invd = reinterpret_cast<const X*>(reg + L.invd);
-  const uint32_t* meta = reinterpret_cast<const uint32_t*>(reg + L.meta);
-  const uint32_t* hdr = reinterpret_cast<const uint32_t*>(reg + L.hdr);
-  // the wave's streams (and the read-ahead past their end) as buffers
-  const int tileEnd = tileBase + TPW < p.T ? tileBase + TPW : p.T;
-  const uint32_t e0 = p.elemOff[tileBase];
-  const uint32_t nElem = p.elemOff[tileEnd] - e0 + kRowsTail;
+  // this lane: lane row lr, column c; its record word is 2c (rows 0-1) or
+  // 2c + 1 (rows 2-3); its chunk of a short record is lr & 1, of a long
+  // record (rows in the order 0, 1, 3, 2) lr ^ (lr >> 1)
+  const int lr = lane >> 4;
+  const int midx = 2 * (lane & 15) + (lane >> 5);
+  const int chShort = lr & 1, chLong = lr ^ (lr >> 1);
+  const bool lastShort = (lr & 1) != 0, lastLong = lr == 2;
+  // the tile's streams (and the read-ahead past their end) as buffers
+  const uint32_t e0 = p.elemOff[tile];
+  const uint32_t nElem = p.elemOff[tile + 1] - e0 + kRowsTail;
   const __amdgpu_buffer_rsrc_t vrs = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<FT*>(static_cast<const FT*>(p.val) + e0), 0, (int)(nElem * sizeof(FT)),
       kBufferFlags);
@@ -239,20 +251,24 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   const int dilu = p.dilu;
   Words4 vr[D][NV], cr[D];
   uint32_t mr[D], hr[D];
-  X carry = (X)0;  // partial sum of a row split over consecutive steps
-  int ldRow = 0;   // first record word of the next step to load
-  // its first element, from the wave's first stream element
-  uint32_t ldElem = (tileBase + tl < p.T ? p.elemOff[tileBase + tl] : p.elemOff[tileEnd]) - e0;
-  auto header = [&](int k) {
-    return TPW == 1 ? __builtin_amdgcn_readfirstlane(hdr[k]) : hdr[k];
-  };
-  // loads of a step's records into ring slot u; lanes past the step's
-  // records take an offset outside the buffer: their loads return zeros
-  // without a memory access
-  auto issue = [&](int u, uint32_t h, uint32_t m) {
-    const int R = (int)(h & 127u);
-    const bool has = rl < R && kHalf * half < (int)((m >> 11) & 31u);
-    const uint32_t e = has ? ldElem + (m & 0x7ffu) + (uint32_t)(kHalf * half) : 0x8000000u;
+  X carry = (X)0;   // partial sum of a row continued in the next step
+  int ldRow = 0;    // first record word of the next step to load
+  uint32_t ldElem = 0;  // its first element
+  auto header = [&](int k) { return __builtin_amdgcn_readfirstlane(hdr[k]); };
+  // loads of a step's chunks into ring slot u: the step stores the chunks
+  // in lane order, so a lane's chunk follows those of the lanes below it
+  // (a wave-wide load reads one contiguous range); lanes without a chunk
+  // take an offset outside the buffer: their loads return zeros without a
+  // memory access
+  auto issue = [&](int u, uint32_t h, uint32_t mw) {
+    const int R = (int)((h >> 12) & 63u);
+    const uint32_t m = midx < R ? mw : 0u;
+    const int ch = (m & kRowsLongBit) ? chLong : chShort;
+    const bool has = ch < (int)((m >> 10) & 7u);
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(has);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    const uint32_t e = has ? ldElem + (uint32_t)kRowsChunk * rank : 0x8000000u;
     const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
 #pragma unroll
     for (int i = 0; i < NV; i++) vr[u][i] = bufLoad16(vrs, vo, 16 * i);
@@ -260,29 +276,29 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     mr[u] = m;
     hr[u] = h;
     ldRow += R;
-    ldElem += (h >> 17) & 0x7ffu;
+    ldElem += h & 0xfffu;
   };
   // first fill, slot by slot in the steady state's order (the compiler's
   // load waits then hold for every pass)
 #pragma unroll
   for (int u = 0; u < D; u++) {
-    issue(u, header(u), meta[ldRow + rl]);
+    issue(u, header(u), meta[ldRow + midx]);
     __builtin_amdgcn_sched_barrier(0);
   }
-  for (int k0 = 0; k0 < nSmax; k0 += D) {
+  for (int k0 = 0; k0 < nS; k0 += D) {
 #pragma unroll
     for (int u = 0; u < D; u++) {
       // descriptors of the step that refills this slot
       const uint32_t hn = header(k0 + u + D);
-      const uint32_t mn = meta[ldRow + rl];
+      const uint32_t mn = meta[ldRow + midx];
       const uint32_t h = hr[u], m = mr[u];
-      const int R = (int)(h & 127u);
-      const bool bwd = (h >> 31) != 0;
-      const int loc = (int)(m >> 18);  // the row's slot
-      const bool cont = (m >> 16) & 1u, partial = (m >> 17) & 1u;
-      X xv[kHalf];
+      const bool bwd = (h >> 31) != 0, anyLong = ((h >> 30) & 1u) != 0;
+      const int loc = (int)(m >> 16);  // the row's slot (0: no record)
+      const bool lng = (m & kRowsLongBit) != 0, cont = (m & kRowsContBit) != 0;
+      const bool partial = (m & kRowsPartialBit) != 0;
+      X xv[kRowsChunk];
 #pragma unroll
-      for (int j = 0; j < kHalf; j++) {
+      for (int j = 0; j < kRowsChunk; j++) {
         const uint32_t w = cr[u].w[j >> 1];
         const uint32_t bo = (j & 1) ? w >> 16 : w & 0xffffu;  // 4 * slot
         xv[j] = *reinterpret_cast<const X*>(xb + bo * (sizeof(X) / 4));
@@ -290,48 +306,51 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       const X pX = xs[loc], pD = invd[loc];
       // all reads issued before the first product waits on one of them
       __builtin_amdgcn_sched_barrier(0);
-      X prod[kHalf];
+      X prod[kRowsChunk];
 #pragma unroll
-      for (int j = 0; j < kHalf; j++) prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
-      // entries 0-7 in the low lanes, then 8-15 in the high lanes, in order
+      for (int j = 0; j < kRowsChunk; j++) prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
+      // chunk 0 (rows 0, 2), chunk 1 (rows 1, 3), in entry order
       X acc = cont ? carry : bwd ? (X)0 : pX;
 #pragma unroll
-      for (int j = 0; j < kHalf; j++) acc = acc - prod[j];
-      acc = lowToHigh(acc);
+      for (int j = 0; j < kRowsChunk; j++) acc = acc - prod[j];
+      acc = rowUp(acc);
 #pragma unroll
-      for (int j = 0; j < kHalf; j++) acc = acc - prod[j];
+      for (int j = 0; j < kRowsChunk; j++) acc = acc - prod[j];
+      if (anyLong) {
+        // long records: chunk 2 (row 3), chunk 3 (row 2)
+        X t = lowToHigh(acc);
+#pragma unroll
+        for (int j = 0; j < kRowsChunk; j++) t = t - prod[j];
+        if (lng && lr == 3) acc = t;
+        X v = rowDown(acc);
+#pragma unroll
+        for (int j = 0; j < kRowsChunk; j++) v = v - prod[j];
+        if (lng && lr == 2) acc = v;
+        carry = highToLow(acc);  // row 2 to row 0 for a continued row
+      }
       X res;
       if (bwd)
         res = dilu ? pX + acc * pD : (pX + acc) * pD;
       else
         res = dilu ? acc * pD : acc;
-      carry = highToLow(acc);
-      if (half && rl < R && !partial) xs[loc] = res;
+      if ((lng ? lastLong : lastShort) && loc != 0 && !partial) xs[loc] = res;
       issue(u, hn, mn);
     }
   }
   __syncthreads();
-#pragma unroll
-  for (int tt = 0; tt < TPW; tt++) {
-    const int tile = tileBase + tt;
-    if (!live(tile)) continue;
-    const X* xt = reinterpret_cast<const X*>(smem + (size_t)tt * L.total + L.xs);
-    const int t0 = p.tileStart[tile], nt = p.tileStart[tile + 1] - t0;
-    for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xt[q + 1];
-  }
+  for (int q = lane; q < nt; q += kRowsThreads) out[t0 + q] = xs[q + 1];
 }
 
 template <typename FT, typename X>
 static void rowsTyped(const RowsArgs& p, const void* invD, const void* rhs, void* out,
                       const PreOp& pre, DevState* st, int checkDone, hipStream_t s) {
-  const int tpw = p.tilesPerWave == 2 ? 2 : 1;
-  const size_t shm = (size_t)tpw * rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
-  auto k = tpw == 2 ? iluRowsKernel<FT, X, 2> : iluRowsKernel<FT, X, 1>;
+  const size_t shm = rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
+  auto k = iluRowsKernel<FT, X>;
   if (shm > 64 * 1024)
     GPH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
-  hipLaunchKernelGGL(k, dim3((p.T + tpw - 1) / tpw), dim3(kRowsThreads), shm, s, p,
-                     static_cast<const FT*>(invD), static_cast<const X*>(rhs),
+  hipLaunchKernelGGL(k, dim3(p.T), dim3(kRowsThreads), shm, s, p,
+                     static_cast<const InvDiag<FT>*>(invD), static_cast<const X*>(rhs),
                      static_cast<X*>(out), pre, st, checkDone);
   GPH_HIP(hipGetLastError());
 }

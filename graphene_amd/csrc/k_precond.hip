@@ -114,7 +114,7 @@ void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
 // ---------------------------------------------------------------------------
 template <typename FT, typename X, bool USE_LDS>
 __global__ __launch_bounds__(kTileThreads) void iluApplyKernel(
-    TileArgs t, const FT* __restrict__ lu, const FT* __restrict__ invD,
+    TileArgs t, const FT* __restrict__ lu, const InvDiag<FT>* __restrict__ invD,
     const X* __restrict__ rhs, X* __restrict__ out, PreOp pre, DevState* st,
     int checkDone, int dilu) {
   extern __shared__ unsigned char smemRaw[];
@@ -205,14 +205,14 @@ static void iluApplyTyped(const TileArgs& t, const void* lu, const void* invD,
                                   (int)shm));
     hipLaunchKernelGGL(k, dim3(t.T), dim3(kTileThreads), shm, s, t,
                        static_cast<const FT*>(lu),
-                       static_cast<const FT*>(invD),
+                       static_cast<const InvDiag<FT>*>(invD),
                        static_cast<const X*>(rhs), static_cast<X*>(out), pre,
                        st, checkDone, dilu);
   } else {
     hipLaunchKernelGGL((iluApplyKernel<FT, X, false>), dim3(t.T),
                        dim3(kTileThreads), 0, s, t,
                        static_cast<const FT*>(lu),
-                       static_cast<const FT*>(invD),
+                       static_cast<const InvDiag<FT>*>(invD),
                        static_cast<const X*>(rhs), static_cast<X*>(out), pre,
                        st, checkDone, dilu);
   }

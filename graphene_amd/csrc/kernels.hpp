@@ -197,13 +197,13 @@ size_t levelSharedBytes(const LevelArgs& p, DType xt);
 void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
                     DType xt, const PreOp& pre, DevState* st, int checkDone,
                     hipStream_t s);
-// Row-per-lane ILU(0) substitution (k_ilu_rows.hip): one wave per tile,
-// one lane per row of the current level, each lane's factor row loaded
-// into a register ring several steps ahead (stream format in ilu_pack.hpp).
+// Row-record ILU(0) substitution (k_ilu_rows.hip): one wave per tile, up to
+// 8 factor entries of a record per lane, the lanes of a record handing the
+// row's partial sum on in entry order; each lane's entries are loaded into
+// a register ring several steps ahead (stream format in ilu_pack.hpp).
 struct RowsArgs {
   int32_t T = 0, maxRows = 0, maxSteps = 0, maxRecs = 0;
   int32_t dilu = 0;
-  int32_t tilesPerWave = 1;  // 1, or 2 (lanes 0-15 / 32-47 and 16-31 / 48-63)
   const int32_t* tileStart = nullptr;  // T+1
   const uint32_t* elemOff = nullptr;   // T+1 element offsets of the tiles' streams
   const int32_t* hdrOff = nullptr;     // T+1 offsets of the tiles' step headers
@@ -218,8 +218,13 @@ struct RowsArgs {
   const int32_t* tileSys = nullptr;
 };
 constexpr int kRowsThreads = 64;
-constexpr int kRowsMaxLen = 16;  // entries loaded per record (longest record)
+constexpr int kRowsMaxLen = 32;  // entries of a record in one step (long record)
+constexpr int kRowsChunk = 8;    // entries per lane
 constexpr int kRowsTail = 64;    // padding elements after the streams
+// record word flags
+constexpr uint32_t kRowsLongBit = 1u << 13;
+constexpr uint32_t kRowsContBit = 1u << 14;
+constexpr uint32_t kRowsPartialBit = 1u << 15;
 size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt);
 void launchIluRows(const RowsArgs& p, const void* invD, DType ft, const void* rhs,
                    void* out, DType xt, const PreOp& pre, DevState* st,

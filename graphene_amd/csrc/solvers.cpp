@@ -104,10 +104,8 @@ class IluSolver : public Solver {
     const void* lu = lu_.get();
     const void* inv = invD_.get();
     DType ft = ft_;
-    if (ft_ == DType::F16) {
-      lu = lu16_.get();
-      inv = invD16_.get();
-    }
+    // float16 factors: float16 off-diagonals, float32 inverse diagonal
+    if (ft_ == DType::F16) lu = lu16_.get();
     if (rows_ && gph::rowsSharedBytes(rargs_, ft, out.type) <= kMaxPackedShared) {
       gph::launchIluRows(rargs_, inv, ft, rhs.ptr, out.ptr, out.type, pre, st, st ? 1 : 0, s);
       return;
@@ -161,7 +159,7 @@ class IluSolver : public Solver {
   }
 
   size_t factorBytes() const {
-    return lu_.bytes() + invD_.bytes() + lu16_.bytes() + invD16_.bytes() +
+    return lu_.bytes() + invD_.bytes() + lu16_.bytes() +
            fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes() +
            rcol_.bytes() + rval_.bytes() + rmeta_.bytes();
   }
@@ -179,9 +177,7 @@ class IluSolver : public Solver {
                          cfg_->diagonalBased, s);
     if (ft_ == DType::F16) {
       lu16_.alloc((size_t)std::max<int64_t>(nnz, 1) * 2);
-      invD16_.alloc((size_t)n * 2);
       gph::launchCast(lu_.get(), work, lu16_.get(), DType::F16, nnz, s);
-      gph::launchCast(invD_.get(), work, invD16_.get(), DType::F16, n, s);
     }
     if (cfg_->substitution == "level") buildLevel(s);
     if (cfg_->substitution == "rows") buildRows(s);
@@ -192,12 +188,15 @@ class IluSolver : public Solver {
 
   // Level records of the factors for the lane-per-row kernel (built on the
   // host from the device factors).
+  // (Not for float16 factors, whose float32 inverse diagonal the records'
+  // factor-sized 1/diag field cannot hold: they run in the packed kernel.)
   void buildLevel(hipStream_t s) {
+    if (ft_ == DType::F16) return;
     const int64_t nnz = A_.nnzOff(), n = A_.n();
     const size_t es = gph::dtypeSize(ft_);
     std::vector<uint8_t> lu((size_t)std::max<int64_t>(nnz, 1) * es), inv((size_t)n * es);
     const void* luD = ft_ == DType::F16 ? lu16_.get() : lu_.get();
-    const void* invDv = ft_ == DType::F16 ? invD16_.get() : invD_.get();
+    const void* invDv = invD_.get();
     GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
     GPH_HIP(hipMemcpyAsync(inv.data(), invDv, (size_t)n * es, hipMemcpyDeviceToHost, s));
     GPH_HIP(hipStreamSynchronize(s));
@@ -220,15 +219,13 @@ class IluSolver : public Solver {
     level_ = true;
   }
 
-  // One tile per wave: with two, a 128^3 / 2048-tile apply takes 106 us
-  // instead of 94 (one wave per SIMD leaves each step's latency uncovered).
-  int autoTilesPerWave() const { return 1; }
-
-  // Row-per-lane stream of the factors (built on the host from the device
+  // Row-record stream of the factors (built on the host from the device
   // factors).
   void buildRows(hipStream_t s) {
-    static_assert(kRowsMaxLen == gph::kRowsMaxLen && kRowsTail == gph::kRowsTail,
-                  "record length and padding of stream and kernel");
+    static_assert(kRowsMaxLen == gph::kRowsMaxLen && kRowsTail == gph::kRowsTail &&
+                      kRowsPad == gph::kRowsChunk && kRowsLong == gph::kRowsLongBit &&
+                      kRowsCont == gph::kRowsContBit && kRowsPartial == gph::kRowsPartialBit,
+                  "record format of stream and kernel");
     const int64_t nnz = A_.nnzOff(), n = A_.n();
     const size_t es = gph::dtypeSize(ft_);
     std::vector<uint8_t> lu((size_t)std::max<int64_t>(nnz, 1) * es);
@@ -236,8 +233,7 @@ class IluSolver : public Solver {
     GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
     GPH_HIP(hipStreamSynchronize(s));
     (void)n;
-    const int tpw = cfg_->tilesPerWave > 0 ? cfg_->tilesPerWave : autoTilesPerWave();
-    IluRowStream R = packIluRows(A_.hostMatrix(), lu.data(), (int)es, 32 / tpw);
+    IluRowStream R = packIluRows(A_.hostMatrix(), lu.data(), (int)es);
     if (!R.ok) return;
     rcol_.upload(R.col.data(), R.col.size(), s);
     rval_.upload(R.val.data(), R.val.size(), s);
@@ -254,7 +250,6 @@ class IluSolver : public Solver {
     rargs_.maxRecs = (int32_t)R.maxRecs;
     rargs_.metaOff = rmoff_.as<int64_t>();
     rargs_.dilu = cfg_->diagonalBased ? 1 : 0;
-    rargs_.tilesPerWave = tpw;
     rargs_.elemOff = relem_.as<uint32_t>();
     rargs_.hdrOff = rhoff_.as<int32_t>();
     rargs_.hdr = rhdr_.as<uint32_t>();
@@ -317,7 +312,7 @@ class IluSolver : public Solver {
   std::vector<int64_t> groupRowStart_;
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
-  DevBuf lu_, invD_, lu16_, invD16_;
+  DevBuf lu_, invD_, lu16_;
   bool packed_ = false, level_ = false, rows_ = false;
   gph::RowsArgs rargs_;
   DevBuf rcol_, rval_, relem_, rhoff_, rhdr_, rmeta_, rmoff_, tileSys_;
