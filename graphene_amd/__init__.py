@@ -100,6 +100,7 @@ _SIGS = {
     "gph_residual": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P], C.c_int),
     "gph_vector_norm": ([_P, C.c_int, _P, C.POINTER(C.c_double)], C.c_int),
     "gph_vector_norm_noalloc": ([_P, C.c_int, _P, C.POINTER(C.c_double)], C.c_int),
+    "gph_vector_norm_kind": ([_P, C.c_int, C.c_int, _P, C.POINTER(C.c_double)], C.c_int),
     "gph_bench_spmv": ([_P, C.c_int, C.c_int, _P, _P, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     "gph_solver_create": ([_P, C.c_char_p, C.POINTER(_P)], C.c_int),
     "gph_solver_destroy": ([_P], None),
@@ -514,9 +515,15 @@ class Matrix:
                                   r.type, r.ptr, inter, r64.ptr if r64 else None))
         return r
 
-    def vectorNorm(self, x):
+    def vectorNorm(self, x, norm="L2"):
+        """Matrix::vectorNorm (lg/matrix/Matrix.hpp:115-117): "L2", "L1"
+        (sum of magnitudes) or "LINF" (largest magnitude)."""
         out = C.c_double()
-        _check(lib().gph_vector_norm(self._A, x.type, x.ptr, C.byref(out)))
+        if norm == "L2":
+            _check(lib().gph_vector_norm(self._A, x.type, x.ptr, C.byref(out)))
+        else:
+            code = {"L1": 1, "LINF": 2}[norm]
+            _check(lib().gph_vector_norm_kind(self._A, code, x.type, x.ptr, C.byref(out)))
         return out.value
 
     def bench_spmv(self, x, y, iters=50, value_type=FLOAT32):

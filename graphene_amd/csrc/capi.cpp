@@ -478,6 +478,20 @@ int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
     *out = A->A->norm2(Vec{const_cast<void*>(x), dt(t)}, A->A->stream());
   });
 }
+int gph_vector_norm_kind(const gph_matrix* A, gph_norm norm, gph_type t,
+                         const void* x, double* out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(x, "x");
+    need(out, "out");
+    if (norm != GPH_NORM_L2 && norm != GPH_NORM_L1 && norm != GPH_NORM_LINF)
+      gph::fail(gph::kInvalid, "invalid norm " + std::to_string((int)norm));
+    static_assert((int)GPH_NORM_L1 == gph::kNormL1 && (int)GPH_NORM_LINF == gph::kNormLinf &&
+                      (int)GPH_NORM_L2 == gph::kNormL2,
+                  "norm codes of the C-ABI and the kernels");
+    *out = A->A->norm(Vec{const_cast<void*>(x), dt(t)}, (int)norm, A->A->stream());
+  });
+}
 int gph_vector_norm_noalloc(const gph_matrix* A, gph_type t, const void* x,
                             double* out) {
   return guard([&] {

@@ -396,6 +396,22 @@ void DeviceMatrix::residual(const Vec& x, const Vec& b, const Vec& r,
                   gph::kRedNone, s);
 }
 
+double DeviceMatrix::norm(const Vec& x, int kind, hipStream_t s) const {
+  if (kind == gph::kNormL2) return norm2(x, s);
+  gph::VecArgs v;
+  v.n = n_;
+  v.a = x.ptr;
+  v.state = scratchState();
+  v.partials = scratchPartials();
+  v.fin = gph::kFinNormAuxV;
+  gph::launchNormKind(v, x.type, kind, s);
+  double out = 0;
+  GPH_HIP(hipMemcpyAsync(&out, &scratchState()->aux, sizeof(double),
+                         hipMemcpyDeviceToHost, s));
+  GPH_HIP(hipStreamSynchronize(s));
+  return out;
+}
+
 double DeviceMatrix::norm2(const Vec& x, hipStream_t s) const {
   gph::VecArgs v;
   v.n = n_;

@@ -55,6 +55,8 @@ class DeviceMatrix {
                 gph::DType vt, hipStream_t s, double* r64 = nullptr) const;
   // ||x||_2 (MatrixBase::vectorNorm), synchronous.
   double norm2(const Vec& x, hipStream_t s) const;
+  // ||x|| in the norm `kind` (gph::NormKind), synchronous.
+  double norm(const Vec& x, int kind, hipStream_t s) const;
 
   // scratch for standalone reductions
   gph::DevState* scratchState() const { return scratchState_.as<gph::DevState>(); }

@@ -33,7 +33,28 @@ enum FinOp : int32_t {
   kFinAlpha = 5,      // alpha = rho / sum0
   kFinOmega = 6,      // omega = sum1 / sum0, singularity check
   kFinUpdate = 7,     // end of iteration: ||r||, convergence, next rho
+  // The same steps for a norm other than L2, in two kernels: the vector
+  // kernel's reduction updates rho (kFinUpdateRho), then a norm kernel
+  // hands over the finished norm value (the ...V ops take it as sum0).
+  kFinUpdateRho = 8,    // rhoOld = rho, iteration + 1, rho = sum1
+  kFinUpdateNormV = 9,  // rF = norm, convergence, singularity, done
+  kFinInitNormV = 10,   // r0n = rF = norm (after kFinInit)
+  kFinBNormV = 11,      // bNorm = norm
+  kFinNormAuxV = 12,    // aux = norm
 };
+
+// Vector norms (lg/matrix/Norm.hpp): L2 = sqrt(sum x^2), L1 = sum |x|,
+// LINF = max |x|.
+enum NormKind : int32_t { kNormL2 = 0, kNormL1 = 1, kNormLinf = 2 };
+
+// Combination of partial results: sums, or maxima (LINF).
+template <int OP, typename R>
+__device__ __forceinline__ R combine(R a, R b) {
+  if constexpr (OP == 1)
+    return a > b ? a : b;
+  else
+    return a + b;
+}
 
 // Singularity threshold Traits<float>::vsmall() (lg/common/Traits.hpp:38-46).
 constexpr double kVSmall = 1e-37;
@@ -85,14 +106,15 @@ __device__ __forceinline__ int xcdRemap(int b, int nb) {
 
 // Deterministic block sum of K values over a 1-D block of NT threads
 // (NT multiple of 64). Result valid in thread 0.
-template <typename R, int K, int NT>
+// OP = 1: maxima instead of sums.
+template <typename R, int K, int NT, int OP = 0>
 __device__ __forceinline__ void blockSum(R (&v)[K], R* lds /* K*NT/64 */) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; k++) {
     R s = v[k];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s = s + __shfl_down(s, off, 64);
+    for (int off = 32; off > 0; off >>= 1) s = combine<OP>(s, __shfl_down(s, off, 64));
     if (lane == 0) lds[k * (NT / 64) + wid] = s;
   }
   __syncthreads();
@@ -100,7 +122,7 @@ __device__ __forceinline__ void blockSum(R (&v)[K], R* lds /* K*NT/64 */) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
       R s = lds[k * (NT / 64)];
-      for (int w = 1; w < NT / 64; w++) s = s + lds[k * (NT / 64) + w];
+      for (int w = 1; w < NT / 64; w++) s = combine<OP>(s, lds[k * (NT / 64) + w]);
       v[k] = s;
     }
   }
@@ -137,11 +159,11 @@ constexpr size_t kArriveBytes = (size_t)kArriveSlots * kArriveStride * 4;
 // issues all of its loads before the first add (a device-scope load takes
 // about a microsecond); the adds take the partials in the same fixed order
 // as a plain strided loop.
-template <typename R, int K, int NT>
+template <typename R, int K, int NT, int OP = 0>
 __device__ bool gridReduce(R (&v)[K], void* scratch, uint32_t* counter,
                            R (&tot)[K], R* lds) {
   __shared__ int isLast;
-  blockSum<R, K, NT>(v, lds);
+  blockSum<R, K, NT, OP>(v, lds);
   const int nb = gridDim.x;
   R* partials = reinterpret_cast<R*>(static_cast<char*>(scratch) + kArriveBytes);
   if (threadIdx.x == 0) {
@@ -190,9 +212,9 @@ __device__ bool gridReduce(R (&v)[K], void* scratch, uint32_t* counter,
     for (int k = 0; k < K; k++)
 #pragma unroll
       for (int j = 0; j < kB; j++)
-        if (b0 + j * NT < nb) acc[k] = acc[k] + w[k][j];
+        if (b0 + j * NT < nb) acc[k] = combine<OP>(acc[k], w[k][j]);
   }
-  blockSum<R, K, NT>(acc, lds);
+  blockSum<R, K, NT, OP>(acc, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; k++) tot[k] = acc[k];
@@ -274,6 +296,33 @@ __device__ void finalize(DevState* s, int op, double sum0, double sum1) {
       s->done = term ? 1 : 0;
       break;
     }
+    case kFinUpdateRho:
+      s->rhoOld = s->rho;
+      s->iter = s->iter + 1;
+      s->rho = sum1;
+      break;
+    case kFinUpdateNormV: {
+      // the rest of kFinUpdate, with the norm handed over
+      s->rF = sum0;
+      s->converged = convergedTest(s, sum0) ? 1 : 0;
+      bool term = (s->converged && s->iter >= s->minIt) || s->iter >= s->maxIt;
+      if (!term && fabs(s->rho) < kVSmall) {
+        s->singular = 1;
+        term = true;
+      }
+      s->done = term ? 1 : 0;
+      break;
+    }
+    case kFinInitNormV:
+      s->r0n = sum0;
+      s->rF = sum0;
+      break;
+    case kFinBNormV:
+      s->bNorm = sum0;
+      break;
+    case kFinNormAuxV:
+      s->aux = sum0;
+      break;
     default:
       break;
   }

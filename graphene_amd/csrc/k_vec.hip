@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kVecThreads) void updateXRKernel(VecArgs v,
   if (gridReduce<X, 2, kVecThreads>(acc, v.partials,
                                     &st->counter[0], tot, lds)) {
     if (threadIdx.x == 0)
-      finalize<X>(st, kFinUpdate, (double)tot[0], (double)tot[1]);
+      finalize<X>(st, v.fin ? v.fin : kFinUpdate, (double)tot[0], (double)tot[1]);
   }
 }
 
@@ -130,6 +130,26 @@ __global__ __launch_bounds__(kVecThreads) void norm2Kernel(VecArgs v) {
   T tot[1];
   if (gridReduce<T, 1, kVecThreads>(acc, v.partials,
                                     &st->counter[0], tot, lds)) {
+    if (threadIdx.x == 0) finalize<T>(st, v.fin, (double)tot[0], 0.0);
+  }
+}
+
+// sum |x| (L1, OP = 0) or max |x| (LINF, OP = 1) in the operand type,
+// handed to finalize() as the finished norm. The reference's L1 is the
+// signed sum (Tensor.cpp:430-431) and its LINF reduction is not available
+// (Expression.cpp:502-521); these are the textbook norms.
+template <typename T, int OP>
+__global__ __launch_bounds__(kVecThreads) void normKindKernel(VecArgs v) {
+  __shared__ T lds[kVecThreads / 64];
+  DevState* st = v.state;
+  if (v.checkDone && st->done) return;
+  const T* a = static_cast<const T*>(v.a);
+  T acc[1] = {0};
+  for (int i = blockIdx.x * kVecThreads + threadIdx.x; i < v.n;
+       i += gridDim.x * kVecThreads)
+    acc[0] = combine<OP>(acc[0], a[i] < (T)0 ? -a[i] : a[i]);
+  T tot[1];
+  if (gridReduce<T, 1, kVecThreads, OP>(acc, v.partials, &st->counter[0], tot, lds)) {
     if (threadIdx.x == 0) finalize<T>(st, v.fin, (double)tot[0], 0.0);
   }
 }
@@ -226,6 +246,28 @@ void launchNorm2(const VecArgs& v, DType t, hipStream_t s) {
                        v);
   else
     fail(kUnsupported, "norm: type must be float32 or float64");
+  GPH_HIP(hipGetLastError());
+}
+
+void launchNormKind(const VecArgs& v, DType t, int kind, hipStream_t s) {
+  if (kind == kNormL2) {
+    fail(kInvalid, "launchNormKind: the L2 norm has its own kernel");
+  }
+  const int g = vecGrid(v.n);
+  const bool mx = kind == kNormLinf;
+  if (t == DType::F32) {
+    if (mx)
+      hipLaunchKernelGGL((normKindKernel<float, 1>), dim3(g), dim3(kVecThreads), 0, s, v);
+    else
+      hipLaunchKernelGGL((normKindKernel<float, 0>), dim3(g), dim3(kVecThreads), 0, s, v);
+  } else if (t == DType::F64) {
+    if (mx)
+      hipLaunchKernelGGL((normKindKernel<double, 1>), dim3(g), dim3(kVecThreads), 0, s, v);
+    else
+      hipLaunchKernelGGL((normKindKernel<double, 0>), dim3(g), dim3(kVecThreads), 0, s, v);
+  } else {
+    fail(kUnsupported, "norm: type must be float32 or float64");
+  }
   GPH_HIP(hipGetLastError());
 }
 

@@ -90,6 +90,9 @@ void launchUpdateP(const VecArgs& v, DType x, hipStream_t s);
 void launchUpdateS(const VecArgs& v, DType x, hipStream_t s);
 // sum of squares of a (type t), finalised with v.fin into the state
 void launchNorm2(const VecArgs& v, DType t, hipStream_t s);
+// sum |a| (kNormL1) or max |a| (kNormLinf), finalised with v.fin (one of the
+// ...V ops, which take the finished norm)
+void launchNormKind(const VecArgs& v, DType t, int kind, hipStream_t s);
 void launchFill(void* x, DType t, int64_t n, double value, hipStream_t s);
 // ||x||_2 with the reduction scratch in a static device array instead of an
 // allocation (k_noalloc.hip); synchronous on stream s.

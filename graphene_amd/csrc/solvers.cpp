@@ -49,6 +49,20 @@ size_t Workspace::bytes() const {
   return b;
 }
 
+// gph::NormKind of a configured norm (lg/matrix/Norm.hpp). L1 here is the
+// sum of magnitudes and LINF the largest magnitude; the reference's L1 sums
+// signed values (Tensor.cpp:430-431) and its LINF reduction is unavailable.
+static int normKind(VectorNorm n) {
+  switch (n) {
+    case VectorNorm::L1:
+      return gph::kNormL1;
+    case VectorNorm::LINF:
+      return gph::kNormLinf;
+    default:
+      return gph::kNormL2;
+  }
+}
+
 static double nowSeconds() {
   return std::chrono::duration<double>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -330,10 +344,7 @@ class GaussSeidelSolver : public Solver {
  public:
   GaussSeidelSolver(const DeviceMatrix& A,
                     std::shared_ptr<gaussseidel::Configuration> c)
-      : Solver(A), cfg_(std::move(c)) {
-    if (cfg_->norm != VectorNorm::L2)
-      fail(kUnsupported, "only the L2 norm is supported by this build");
-  }
+      : Solver(A), cfg_(std::move(c)) {}
   std::string name() const override { return cfg_->solverName(); }
   bool usesInitialGuess() const override { return true; }
   bool supportsDeviceApply() const override { return true; }
@@ -394,12 +405,13 @@ class GaussSeidelSolver : public Solver {
       st.seconds = nowSeconds() - t0;
       return st;
     }
+    const int kind = normKind(cfg_->norm);
     double bNorm = 0;
-    if (cfg_->relTolerance > 0) bNorm = A_.norm2(b, s);
+    if (cfg_->relTolerance > 0) bNorm = A_.norm(b, kind, s);
     Vec r = ws_.get(1, x.type, A_.n());
     auto resNorm = [&]() {
       A_.residual(x, b, r, x.type, DType::F32, s);
-      return A_.norm2(r, s);
+      return A_.norm(r, kind, s);
     };
     st.initialResidual = resNorm();
     st.finalResidual = st.initialResidual;
@@ -436,9 +448,7 @@ class PBiCGStabSolver : public Solver {
  public:
   PBiCGStabSolver(const DeviceMatrix& A,
                   std::shared_ptr<pbicgstab::Configuration> c)
-      : Solver(A), cfg_(std::move(c)) {
-    if (cfg_->norm != VectorNorm::L2)
-      fail(kUnsupported, "only the L2 norm is supported by this build");
+      : Solver(A), cfg_(std::move(c)), kind_(normKind(cfg_->norm)) {
     if (cfg_->preconditioner) {
       pre_ = Solver::createSolver(A, cfg_->preconditioner);
       if (!pre_->supportsDeviceApply())
@@ -501,8 +511,13 @@ class PBiCGStabSolver : public Solver {
       v.a = cfg_->relToleranceFromB ? b.ptr : x.ptr;
       v.state = st;
       v.partials = partials_.get();
-      v.fin = gph::kFinBNorm;
-      gph::launchNorm2(v, X, s);
+      if (kind_ == gph::kNormL2) {
+        v.fin = gph::kFinBNorm;
+        gph::launchNorm2(v, X, s);
+      } else {
+        v.fin = gph::kFinBNormV;
+        gph::launchNormKind(v, X, kind_, s);
+      }
     }
     // r = b - A x ; r0 = p = r ; ||r|| and rho (pbicgstab/Solver.cpp:66-83)
     {
@@ -516,6 +531,15 @@ class PBiCGStabSolver : public Solver {
       a.partials = partials_.get();
       a.fin = gph::kFinInit;
       gph::launchSpmv(a, V_, X, X, X, X, gph::kModeResidual, gph::kRedSelf, s);
+      if (kind_ != gph::kNormL2) {  // r0n = rF = ||r|| in the configured norm
+        gph::VecArgs v;
+        v.n = (int32_t)n;
+        v.a = r_.ptr;
+        v.state = st;
+        v.partials = partials_.get();
+        v.fin = gph::kFinInitNormV;
+        gph::launchNormKind(v, X, kind_, s);
+      }
     }
 
     int chunk = cfg_->iterationsPerSync > 0 ? cfg_->iterationsPerSync : 4;
@@ -684,11 +708,25 @@ class PBiCGStabSolver : public Solver {
       v.o2 = r_.ptr;
       v.state = st;
       v.partials = partials_.get();
+      // L2: the kernel's r.r gives ||r||; other norms: rho only, then a
+      // norm kernel finishes the iteration
+      if (kind_ != gph::kNormL2) v.fin = gph::kFinUpdateRho;
       gph::launchUpdateXR(v, r0_.ptr, X_, s);
+      if (kind_ != gph::kNormL2) {
+        gph::VecArgs w;
+        w.n = n;
+        w.a = r_.ptr;
+        w.state = st;
+        w.partials = partials_.get();
+        w.fin = gph::kFinUpdateNormV;
+        w.checkDone = 1;
+        gph::launchNormKind(w, X_, kind_, s);
+      }
     }
   }
 
   std::shared_ptr<pbicgstab::Configuration> cfg_;
+  int kind_;  // gph::NormKind of cfg_->norm
   std::unique_ptr<Solver> pre_;
   Workspace ws_;
   DevBuf state_, partials_;
@@ -722,8 +760,6 @@ class IterativeRefinementSolver : public Solver {
   IterativeRefinementSolver(const DeviceMatrix& A,
                             std::shared_ptr<iterativerefinement::Configuration> c)
       : Solver(A), cfg_(std::move(c)) {
-    if (cfg_->norm != VectorNorm::L2)
-      fail(kUnsupported, "only the L2 norm is supported by this build");
     inner_ = Solver::createSolver(A, cfg_->innerSolver);
     inner_->setDeferredSync(true);  // every outer step ends in a synchronizing read
     state_.alloc(sizeof(DevState));
@@ -754,8 +790,11 @@ class IterativeRefinementSolver : public Solver {
     stats.solverName = name();
     stats.norm = cfg_->norm;
     stats.numTiles = A_.numTiles();
+    const int kind = normKind(cfg_->norm);
     double bNorm = 0;
-    if (cfg_->relTolerance > 0) bNorm = A_.norm2(b, s);
+    if (cfg_->relTolerance > 0) bNorm = A_.norm(b, kind, s);
+    // the unrounded residual, for norms other than L2
+    Vec r64 = kind != gph::kNormL2 ? ws_.get(3, E, n) : Vec{};
 
     // r = fl_W(b - A phi) accumulated in E, norm of the unrounded residual
     auto residual = [&]() {
@@ -766,7 +805,17 @@ class IterativeRefinementSolver : public Solver {
       a.state = st;
       a.partials = partials_.get();
       a.fin = gph::kFinNormAux;
+      if (r64.ptr) a.out64 = static_cast<double*>(r64.ptr);
       gph::launchSpmv(a, V, E, b.type, W, E, gph::kModeResidual, gph::kRedSelf, s);
+      if (r64.ptr) {
+        gph::VecArgs v;
+        v.n = (int32_t)n;
+        v.a = r64.ptr;
+        v.state = st;
+        v.partials = partials_.get();
+        v.fin = gph::kFinNormAuxV;
+        gph::launchNormKind(v, E, kind, s);
+      }
       double out = 0;
       GPH_HIP(hipMemcpyAsync(&out, &st->aux, sizeof(double),
                              hipMemcpyDeviceToHost, s));

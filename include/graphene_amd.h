@@ -209,6 +209,15 @@ int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
 /* ||x||_2 (Matrix::vectorNorm, lg/matrix/Matrix.hpp:115-117). */
 int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,
                     double* out);
+/* Norms of lg/matrix/Norm.hpp:24-34 (names "L1", "L2", "LINF" in solver
+ * configurations). L1 is the sum of magnitudes and LINF the largest
+ * magnitude (the reference's L1 sums signed values, Tensor.cpp:430-431, and
+ * its LINF reduction is unavailable). */
+typedef enum { GPH_NORM_L2 = 0, GPH_NORM_L1 = 1, GPH_NORM_LINF = 2 } gph_norm;
+/* ||x|| in the given norm (Matrix::vectorNorm(norm, x),
+ * lg/matrix/Matrix.hpp:115-117). */
+int gph_vector_norm_kind(const gph_matrix* A, gph_norm norm, gph_type t,
+                         const void* x, double* out);
 /* No reference counterpart: ||x||_2 as gph_vector_norm (same kernel, same
  * result bits), for callers that cannot allocate device memory (code under
  * graph capture, tight serving loops). It calls no hipMalloc/hipFree: the

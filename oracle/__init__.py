@@ -39,14 +39,27 @@ class PBiCGStabCfg(C.Structure):
                 ("minIterations", C.c_int32), ("working64", C.c_int32),
                 ("precond", C.c_int32), ("factor64", C.c_int32),
                 ("gsSweeps", C.c_int32), ("gsSymmetric", C.c_int32),
-                ("bNormFromX", C.c_int32)]
+                ("bNormFromX", C.c_int32), ("norm", C.c_int32)]
 
 
 class MPIRCfg(C.Structure):
     _fields_ = [("absTolerance", C.c_float), ("relTolerance", C.c_float),
                 ("relResidual", C.c_float), ("maxIterations", C.c_int32),
                 ("minIterations", C.c_int32), ("normFromRounded", C.c_int32),
-                ("inner", PBiCGStabCfg)]
+                ("inner", PBiCGStabCfg), ("norm", C.c_int32)]
+
+
+# norm codes of the oracle (L1: sum of magnitudes, LINF: largest magnitude)
+NORMS = {"L2": 0, "L1": 1, "LINF": 2}
+
+
+def vec_norm(v, tile_start, norm="L2"):
+    """The oracle's vector norm (float32 or float64, in the reference's
+    reduction order for L2 and L1)."""
+    ts = np.ascontiguousarray(tile_start, np.int64)
+    v = np.ascontiguousarray(v)
+    f = lib().orc_vec_norm_f32 if v.dtype == np.float32 else lib().orc_vec_norm_f64
+    return f(_p(v), _i64(len(v)), C.c_int32(len(ts) - 1), _p(ts), C.c_int32(NORMS[norm]))
 
 
 class Stats(C.Structure):
@@ -73,6 +86,8 @@ def lib():
         L.orc_pbicgstab_f64.restype = C.c_int32
         L.orc_mpir.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
+        L.orc_vec_norm_f32.restype = C.c_float
+        L.orc_vec_norm_f64.restype = C.c_double
         _lib = L
     return _lib
 
@@ -331,7 +346,7 @@ class System:
     @staticmethod
     def pbicgstab_cfg(relResidual=0.0, maxIterations=1000, precond="ilu", working64=False,
                       factor64=None, gsSweeps=1, gsSymmetric=False, absTolerance=0.0,
-                      relTolerance=0.0, minIterations=0, bNormFromX=True):
+                      relTolerance=0.0, minIterations=0, bNormFromX=True, norm="L2"):
         c = PBiCGStabCfg()
         c.absTolerance = absTolerance
         c.relTolerance = relTolerance
@@ -344,6 +359,7 @@ class System:
         c.gsSweeps = gsSweeps
         c.gsSymmetric = int(gsSymmetric)
         c.bNormFromX = int(bNormFromX)
+        c.norm = NORMS[norm]
         return c
 
     def pbicgstab(self, x, b, cfg):
@@ -355,8 +371,9 @@ class System:
         return x, st
 
     def mpir(self, b32, relResidual=1e-10, maxIterations=100, inner=None,
-             normFromRounded=False, phi0=None):
+             normFromRounded=False, phi0=None, norm="L2"):
         c = MPIRCfg()
+        c.norm = NORMS[norm]
         c.relResidual = relResidual
         c.maxIterations = maxIterations
         c.normFromRounded = int(normFromRounded)

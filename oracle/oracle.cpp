@@ -804,7 +804,29 @@ struct OrcPBiCGStabCfg {
   int32_t gsSweeps;       // numFixedIterations of the GS preconditioner
   int32_t gsSymmetric;    // forward + backward per sweep
   int32_t bNormFromX;     // reference quirk (pbicgstab/Solver.cpp:69-70)
+  int32_t norm;           // 0 L2, 1 L1 (sum of magnitudes), 2 LINF (largest magnitude)
 };
+
+// Vector norm of kind `norm` in the element type R (Tensor::norm,
+// lg/dsl/tensor/Tensor.cpp:425-439): L2 squares in the reference's
+// reduction order and takes the root; L1 sums magnitudes in the same order
+// (the reference sums signed values, Tensor.cpp:430-431; the textbook norm
+// is the build's documented choice); LINF is the largest magnitude (exact,
+// order-free).
+template <typename R>
+static R vec_norm(const R* u, int64_t n, int32_t T, const int64_t* tsp, int32_t norm) {
+  if (norm == 2) {
+    R m = 0;
+    for (int64_t i = 0; i < n; i++) m = std::max(m, (R)std::fabs(u[i]));
+    return m;
+  }
+  if (norm == 1) {
+    std::vector<R> a(n);
+    for (int64_t i = 0; i < n; i++) a[i] = (R)std::fabs(u[i]);
+    return reduce_add<R>(a.data(), T, tsp, 0);
+  }
+  return (R)std::sqrt(reduce_add<R>(u, T, tsp, 1));
+}
 struct OrcStats {
   double initialResidual, finalResidual;
   int32_t iterations, converged, singular, pad;
@@ -917,9 +939,7 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
     for (int64_t i = 0; i < n; i++) t[i] = u[i] * v[i];
     return reduce_add<X>(t.data(), T, tsp, 0);
   };
-  auto norm = [&](const X* u) {
-    return (X)std::sqrt(reduce_add<X>(u, T, tsp, 1));
-  };
+  auto norm = [&](const X* u) { return vec_norm<X>(u, n, T, tsp, c->norm); };
   std::vector<X> r(n), r0(n), p(n), v(n), y(n), z(n), sv(n), t(n);
   for (int64_t i = 0; i < n; i++)
     r[i] = b[i] - row_sum<float, X, X>(i, s->rowPtr.data(), s->col.data(),
@@ -972,6 +992,15 @@ static void pbicgstab(OrcSystem* s, const OrcPBiCGStabCfg* c, X* x,
   }
 }
 
+ORC_API float orc_vec_norm_f32(const float* u, int64_t n, int32_t T, const int64_t* tsp,
+                               int32_t norm) {
+  return vec_norm<float>(u, n, T, tsp, norm);
+}
+ORC_API double orc_vec_norm_f64(const double* u, int64_t n, int32_t T, const int64_t* tsp,
+                                int32_t norm) {
+  return vec_norm<double>(u, n, T, tsp, norm);
+}
+
 // Threads of the parallel loops (the results do not depend on it).
 ORC_API void orc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
 ORC_API int32_t orc_max_threads(void) { return omp_get_max_threads(); }
@@ -997,6 +1026,7 @@ struct OrcMPIRCfg {
   int32_t maxIterations, minIterations;
   int32_t normFromRounded;
   OrcPBiCGStabCfg inner;
+  int32_t norm;  // as OrcPBiCGStabCfg::norm
 };
 ORC_API int32_t orc_mpir(void* h, const OrcMPIRCfg* c, double* phi, const float* b,
                  OrcStats* st, int32_t* innerIterations) {
@@ -1009,14 +1039,13 @@ ORC_API int32_t orc_mpir(void* h, const OrcMPIRCfg* c, double* phi, const float*
   auto resid = [&]() {
     orc_residual_mixed(n, s->rowPtr.data(), s->col.data(), s->off.data(),
                        s->diag.data(), phi, b, r32.data(), r64.data());
-    if (c->normFromRounded)
-      return (double)std::sqrt(reduce_add<float>(r32.data(), T, tsp, 1));
-    return std::sqrt(reduce_add<double>(r64.data(), T, tsp, 1));
+    if (c->normFromRounded) return (double)vec_norm<float>(r32.data(), n, T, tsp, c->norm);
+    return vec_norm<double>(r64.data(), n, T, tsp, c->norm);
   };
   double bNorm = 0;
   if (c->relTolerance > 0) {
     std::vector<double> bd(b, b + n);
-    bNorm = std::sqrt(reduce_add<double>(bd.data(), T, tsp, 1));
+    bNorm = vec_norm<double>(bd.data(), n, T, tsp, c->norm);
   }
   st->initialResidual = resid();
   st->finalResidual = st->initialResidual;

@@ -157,3 +157,63 @@ def test_restarter_against_oracle_loop(sysm):
                        "innerSolver": cfg("float64", relResidual=1e-6)}).solve(
         x2, dev(b.astype(np.float64)))
     assert st2.converged and st2.innerIterations == st2.iterations
+
+
+# --- norms (lg/matrix/Norm.hpp): L1 = sum of magnitudes, LINF = largest
+# magnitude (the reference's L1 sums signed values and its LINF reduction is
+# unavailable: documented deviations) ------------------------------------------
+@pytest.mark.parametrize("norm", ["L1", "LINF"])
+def test_vector_norm_kinds(sysm, norm):
+    h, L, A, _ = sysm
+    for dt in (np.float32, np.float64):
+        x = (o.uniform_pm1(40, h.nrows) * 3).astype(dt)
+        got = A.vectorNorm(dev(x), norm)
+        ref = o.vec_norm(x, L["tile_start"], norm)
+        if norm == "LINF":
+            assert got == float(ref)
+        else:
+            assert abs(got - ref) <= (1e-5 if dt == np.float32 else 1e-12) * ref
+
+
+@pytest.mark.parametrize("working", ["float32", "float64"])
+@pytest.mark.parametrize("norm", ["L1", "LINF"])
+def test_pbicgstab_norms_against_oracle(sysm, norm, working):
+    h, L, _, S = sysm
+    _, b = rhs_for(h, L, seed=41)
+    st, sto, x, xo = both(sysm, cfg(working, relResidual=1e-6, norm=norm),
+                          ocfg(S, working, relResidual=1e-6, norm=norm), b,
+                          np.zeros(h.nrows), working)
+    assert sto.converged and st.converged and close(st.iterations, sto.iterations)
+    assert abs(st.initialResidual - sto.initialResidual) <= 1e-5 * sto.initialResidual
+    assert st.finalResidual < 1e-6 * st.initialResidual
+    # the norm of the returned residual, recomputed on the host
+    r = o.residual_mixed(L, x.astype(np.float64), b)[1]
+    rn = np.sum(np.abs(r)) if norm == "L1" else np.max(np.abs(r))
+    assert rn <= 10e-6 * sto.initialResidual
+
+
+def test_mpir_and_gauss_seidel_norms(sysm):
+    h, L, A, S = sysm
+    _, b = rhs_for(h, L, seed=42)
+    inner = cfg("float32", relResidual=1e-3, maxIterations=100, norm="L1")
+    mcfg = {"type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 30,
+            "extendedPrecisionType": "float64", "workingType": "float32", "norm": "LINF",
+            "innerSolver": inner}
+    phi = dev(np.zeros(h.nrows))
+    st = g.Solver(A, mcfg).solve(phi, dev(b))
+    _, sto, inn = S.mpir(b, 1e-10, 30, inner=ocfg(S, "float32", relResidual=1e-3,
+                                                    maxIterations=100, norm="L1"),
+                         norm="LINF")
+    assert st.converged and sto.converged and abs(st.iterations - sto.iterations) <= 1
+    r = o.residual_mixed(L, phi.numpy(), b)[1]
+    assert np.max(np.abs(r)) <= 1e-10 * st.initialResidual * 1.0001
+    assert abs(st.initialResidual - np.max(np.abs(o.residual_mixed(
+        L, np.zeros(h.nrows), b)[1]))) <= 1e-12 * st.initialResidual
+    # Gauss-Seidel iterated to a relative L1 residual
+    gcfg = {"type": "GaussSeidel", "relResidual": 1e-4, "maxIterations": 500,
+            "numSweeps": 1, "norm": "L1"}
+    x = dev(np.zeros(h.nrows, np.float32))
+    sg = g.Solver(A, gcfg).solve(x, dev(b))
+    assert sg.converged and sg.finalResidual < 1e-4 * sg.initialResidual
+    rr = o.residual_mixed(L, x.numpy().astype(np.float64), b)[1]
+    assert abs(np.sum(np.abs(rr)) - sg.finalResidual) <= 1e-3 * sg.finalResidual

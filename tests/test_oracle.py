@@ -555,3 +555,20 @@ def _gs_global(A, b, x0):
     for i in range(len(b)):
         x[i] = (b[i] - A[i, :i] @ x[:i] - A[i, i + 1:] @ x[i + 1:]) / A[i, i]
     return x
+
+
+# --- vector norms -------------------------------------------------------------
+def test_vector_norms_vs_numpy():
+    """L2 and L1 (sum of magnitudes) in the reference's reduction order,
+    LINF (largest magnitude) exactly; a plausible slip (signed sum, missing
+    root) fails these."""
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(5000)
+    ts = np.array([0, 1200, 3100, 5000])
+    assert o.vec_norm(v, ts, "LINF") == np.max(np.abs(v))
+    assert np.isclose(o.vec_norm(v, ts, "L1"), np.sum(np.abs(v)), rtol=1e-13)
+    assert np.isclose(o.vec_norm(v, ts, "L2"), np.linalg.norm(v), rtol=1e-13)
+    v32 = v.astype(np.float32)
+    assert o.vec_norm(v32, ts, "LINF") == np.max(np.abs(v32))
+    assert np.isclose(o.vec_norm(v32, ts, "L1"), np.sum(np.abs(v.astype(np.float32)),
+                                                          dtype=np.float64), rtol=1e-5)
