@@ -112,6 +112,7 @@ _SIGS = {
     "gph_server_create": ([C.c_int32, C.c_size_t, C.POINTER(_P)], C.c_int),
     "gph_batch_create": ([_P, C.c_int32, _P, C.c_char_p, C.POINTER(_P)], C.c_int),
     "gph_batch_destroy": ([_P], None),
+    "gph_batch_create_rhs": ([_P, C.c_int32, C.c_char_p, C.POINTER(_P)], C.c_int),
     "gph_batch_info": ([_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                         C.POINTER(C.c_int32)], C.c_int),
     "gph_batch_solve": ([_P, C.c_int, _P, _P, _P, C.POINTER(C.c_double)], C.c_int),
@@ -542,11 +543,18 @@ class Batch:
     consecutive tile ranges of one block-diagonal device matrix
     (system_tile_start, S+1 entries), solved together from x = 0."""
 
-    def __init__(self, matrix, system_tile_start, config):
-        st = np.ascontiguousarray(system_tile_start, np.int32)
+    def __init__(self, matrix, system_tile_start, config, n_rhs=None):
+        """n_rhs: instead of system tile ranges, n_rhs right-hand sides of the
+        one system `matrix` (vectors of n_rhs * nrows elements, block j =
+        right-hand side j; gph_batch_create_rhs)."""
         h = _P()
-        _check(lib().gph_batch_create(matrix._A, len(st) - 1, _ptr(st),
-                                      _cfg_text(config).encode(), C.byref(h)))
+        if n_rhs is not None:
+            _check(lib().gph_batch_create_rhs(matrix._A, int(n_rhs),
+                                              _cfg_text(config).encode(), C.byref(h)))
+        else:
+            st = np.ascontiguousarray(system_tile_start, np.int32)
+            _check(lib().gph_batch_create(matrix._A, len(st) - 1, _ptr(st),
+                                          _cfg_text(config).encode(), C.byref(h)))
         self._h = h
         self._A = matrix
         n, ref, rpw = C.c_int32(), C.c_int32(), C.c_int32()

@@ -34,6 +34,7 @@ SOURCES = [
     "k_ilu_rows.hip",
     "k_batch.hip",
     "k_noalloc.hip",
+    "k_multi.hip",
 ]
 
 COMMON = [

@@ -34,29 +34,10 @@ gph::DevState stateFor(float absTol, float relRes, int minIt, int maxIt) {
 }
 }  // namespace
 
-BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_t>& sysTile,
-                         const std::string& configJson)
-    : A_(A) {
-  const auto& h = A.hostMatrix();
-  S_ = (int32_t)sysTile.size() - 1;
-  if (S_ < 1) fail(gph::kInvalid, "a batch needs at least one system");
-  if (sysTile.front() != 0 || sysTile.back() != h.numTiles)
-    fail(gph::kInvalid, "system tile ranges must cover the matrix's tiles");
-  for (int32_t s = 0; s < S_; s++)
-    if (sysTile[(size_t)s + 1] <= sysTile[(size_t)s])
-      fail(gph::kInvalid, "every system needs at least one tile");
-  sysRow_.resize((size_t)S_ + 1);
-  for (int32_t s = 0; s <= S_; s++) sysRow_[(size_t)s] = h.tileStart[(size_t)sysTile[(size_t)s]];
-  // block-diagonal: every entry of a system's rows lies in the system
-  for (int32_t s = 0; s < S_; s++)
-    for (int64_t i = sysRow_[(size_t)s]; i < sysRow_[(size_t)s + 1]; i++)
-      for (int64_t a = h.rowPtr[(size_t)i]; a < h.rowPtr[(size_t)i + 1]; a++) {
-        const int64_t j = h.col[(size_t)a];
-        if (j < sysRow_[(size_t)s] || j >= sysRow_[(size_t)s + 1])
-          fail(gph::kInvalid, "a matrix entry couples two systems of the batch");
-      }
-
-  // configuration: MPIR(PBiCGStab(ILU)) or PBiCGStab(ILU)
+// Configuration: MPIR(PBiCGStab(ILU)) or PBiCGStab(ILU); sets the outer and
+// inner initial states and returns the ILU configuration.
+std::shared_ptr<solver::ilu::Configuration> BatchSolver::parseConfig(
+    const std::string& configJson) {
   auto cfg = solver::Configuration::fromJSON(configJson);
   std::shared_ptr<solver::pbicgstab::Configuration> pb;
   if (auto ir = std::dynamic_pointer_cast<solver::iterativerefinement::Configuration>(cfg)) {
@@ -80,6 +61,12 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   if (!ilu) fail(gph::kUnsupported, "batched PBiCGStab needs an ILU preconditioner");
   innerInit_ = stateFor(pb->absTolerance, pb->relResidual, pb->minIterations, pb->maxIterations);
 
+  return ilu;
+}
+
+// Chunks (consecutive rows of one system, at most kBatchChunk each), states,
+// vectors of sysRow_[S] rows and the reduction scratch.
+void BatchSolver::setupChunks(hipStream_t st) {
   // chunks: consecutive rows of one system, at most kBatchChunk each
   std::vector<int32_t> chunkSys, chunkRow, sysChunk((size_t)S_ + 1);
   for (int32_t s = 0; s < S_; s++) {
@@ -92,7 +79,6 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   sysChunk[(size_t)S_] = (int32_t)chunkSys.size();
   chunkRow.push_back((int32_t)sysRow_[(size_t)S_]);
   C_ = (int32_t)chunkSys.size();
-  hipStream_t st = A.stream();
   chunkSys_.upload(chunkSys.data(), chunkSys.size(), st);
   chunkRow_.upload(chunkRow.data(), chunkRow.size(), st);
   sysChunk_.upload(sysChunk.data(), sysChunk.size(), st);
@@ -100,7 +86,7 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   outer_.alloc((size_t)S_ * sizeof(gph::DevState));
   glob_.alloc(sizeof(gph::DevState));
   partials_.alloc((size_t)2 * C_ * sizeof(double));
-  const size_t n = (size_t)A.n();
+  const size_t n = (size_t)sysRow_[(size_t)S_];
   for (gph::DevBuf* b : {&r_, &r0_, &p_, &v_, &y_, &s_, &z_, &t_, &d_, &rw_}) b->alloc(n * 4);
   if (mpir_) r64_.alloc(n * 8);
   args_.S = S_;
@@ -114,6 +100,34 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   GPH_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(gph::DevState), 0));
   gph::DevState zero{};
   GPH_HIP(hipMemcpyAsync(glob_.get(), &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+}
+
+BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_t>& sysTile,
+                         const std::string& configJson)
+    : A_(A) {
+  const auto& h = A.hostMatrix();
+  S_ = (int32_t)sysTile.size() - 1;
+  if (S_ < 1) fail(gph::kInvalid, "a batch needs at least one system");
+  if (sysTile.front() != 0 || sysTile.back() != h.numTiles)
+    fail(gph::kInvalid, "system tile ranges must cover the matrix's tiles");
+  for (int32_t s = 0; s < S_; s++)
+    if (sysTile[(size_t)s + 1] <= sysTile[(size_t)s])
+      fail(gph::kInvalid, "every system needs at least one tile");
+  sysRow_.resize((size_t)S_ + 1);
+  for (int32_t s = 0; s <= S_; s++) sysRow_[(size_t)s] = h.tileStart[(size_t)sysTile[(size_t)s]];
+  // block-diagonal: every entry of a system's rows lies in the system
+  for (int32_t s = 0; s < S_; s++)
+    for (int64_t i = sysRow_[(size_t)s]; i < sysRow_[(size_t)s + 1]; i++)
+      for (int64_t a = h.rowPtr[(size_t)i]; a < h.rowPtr[(size_t)i + 1]; a++) {
+        const int64_t j = h.col[(size_t)a];
+        if (j < sysRow_[(size_t)s] || j >= sysRow_[(size_t)s + 1])
+          fail(gph::kInvalid, "a matrix entry couples two systems of the batch");
+      }
+
+  auto ilu = parseConfig(configJson);
+  hipStream_t st = A.stream();
+  setupChunks(st);
+  const size_t n = (size_t)A.n();
 
   // ILU substitution kernel: the configured one, or the fastest on this
   // union (three timed applications each) of the row-per-lane kernel and
@@ -185,10 +199,55 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
   GPH_HIP(hipStreamSynchronize(st));
 }
 
+BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, int32_t nrhs,
+                         const std::string& configJson,
+                         std::shared_ptr<solver::Solver> ilu)
+    : A_(A), multi_(true) {
+  if (nrhs < 1 || nrhs > 8)
+    fail(gph::kInvalid, "1 to 8 right-hand sides, not " + std::to_string(nrhs));
+  if (A.sellArgs(DType::F32).numSlices <= 0)
+    fail(gph::kUnsupported, "several right-hand sides need the sliced matrix layout");
+  S_ = nrhs;
+  const int64_t n = A.n();
+  sysRow_.resize((size_t)S_ + 1);
+  for (int32_t j = 0; j <= S_; j++) sysRow_[(size_t)j] = j * n;
+  if ((int64_t)S_ * n >= (int64_t)INT32_MAX)
+    fail(gph::kInvalid, "right-hand sides times rows exceed the int32 row range");
+  auto iluCfg = parseConfig(configJson);
+  GPH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  setupChunks(stream_);
+  if (ilu) {
+    ilu_ = std::move(ilu);
+  } else {
+    ilu_ = solver::Solver::createSolver(A, iluCfg);
+  }
+  substitution_ = iluCfg->substitution;
+  // warm-up of the launch paths outside any capture
+  const Vec vy{y_.get(), DType::F32}, vv{v_.get(), DType::F32};
+  gph::launchFill(y_.get(), DType::F32, (int64_t)S_ * n, 0.0, stream_);
+  gph::SpmvArgs a = A.sellArgs(DType::F32);
+  a.x = y_.get();
+  a.out = v_.get();
+  gph::launchSpmvMulti(a, S_, n, nullptr, nullptr, stream_);
+  ilu_->applyDeviceMulti(vv, vy, S_, n, nullptr, nullptr, stream_);
+  GPH_HIP(hipStreamSynchronize(stream_));
+}
+
+size_t BatchSolver::deviceBytes() const {
+  size_t b = 0;
+  for (const gph::DevBuf* q : {&chunkSys_, &chunkRow_, &sysChunk_, &inner_, &outer_, &glob_,
+                               &partials_, &sliceSys_, &r_, &r0_, &p_, &v_, &y_, &s_, &z_, &t_,
+                               &d_, &rw_, &r64_})
+    b += q->bytes();
+  return b;
+}
+
 BatchSolver::~BatchSolver() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   if (pinned_) (void)hipHostFree(pinned_);
+  if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 // One PBiCGStab iteration for every running system (pbicgstab/Solver.cpp:91-180).
@@ -196,7 +255,15 @@ void BatchSolver::enqueueIteration(hipStream_t s) {
   const Vec vp{p_.get(), DType::F32}, vy{y_.get(), DType::F32}, vv{v_.get(), DType::F32};
   const Vec vs{s_.get(), DType::F32}, vz{z_.get(), DType::F32}, vt{t_.get(), DType::F32};
   gph::DevState* glob = glob_.as<gph::DevState>();
+  const int64_t n = A_.n();
   auto spmv = [&](const Vec& x, const Vec& y) {
+    if (multi_) {  // every right-hand side in one pass over the matrix
+      gph::SpmvArgs a = A_.sellArgs(DType::F32);
+      a.x = x.ptr;
+      a.out = y.ptr;
+      gph::launchSpmvMulti(a, S_, n, args_.st, glob, s);
+      return;
+    }
     gph::SpmvArgs a = A_.spmvArgs(DType::F32);
     a.x = x.ptr;
     a.out = y.ptr;
@@ -211,12 +278,18 @@ void BatchSolver::enqueueIteration(hipStream_t s) {
   };
   float* r = r_.as<float>();
   gph::launchBatchUpdateP(args_, r, p_.as<float>(), v_.as<float>(), s);
-  ilu_->applyDevice(vp, vy, gph::PreOp{}, glob, s);
+  auto ilu = [&](const Vec& in, const Vec& out) {
+    if (multi_)
+      ilu_->applyDeviceMulti(in, out, S_, n, args_.st, glob, s);
+    else
+      ilu_->applyDevice(in, out, gph::PreOp{}, glob, s);
+  };
+  ilu(vp, vy);
   spmv(vy, vv);
   gph::launchBatchDot1(args_, r0_.get(), v_.get(), s);
   gph::launchBatchFin(args_, gph::kFinAlpha, 1, s);
   gph::launchBatchUpdateS(args_, r, s_.as<float>(), v_.as<float>(), s);
-  ilu_->applyDevice(vs, vz, gph::PreOp{}, glob, s);
+  ilu(vs, vz);
   spmv(vz, vt);
   gph::launchBatchDot2(args_, t_.get(), t_.get(), t_.get(), s_.get(), s);
   gph::launchBatchFin(args_, gph::kFinOmega, 2, s);
@@ -235,7 +308,7 @@ void BatchSolver::captureGraph(hipStream_t s) {
 // Batched PBiCGStab from x = 0 on the systems whose inner state is not
 // finished; adds each system's iteration count to innerIts.
 void BatchSolver::innerSolve(const void* rhs, void* x, std::vector<int64_t>& innerIts) {
-  hipStream_t s = A_.stream();
+  hipStream_t s = stream();
   xInner_ = static_cast<float*>(x);  // always d_ (captured in the graph)
   gph::launchBatchStart(args_, static_cast<const float*>(rhs), static_cast<float*>(x),
                         r_.as<float>(), r0_.as<float>(), p_.as<float>(), s);
@@ -264,15 +337,16 @@ void BatchSolver::innerSolve(const void* rhs, void* x, std::vector<int64_t>& inn
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   std::vector<gph::DevState> st((size_t)S_);
-  GPH_HIP(hipMemcpy(st.data(), inner_.get(), st.size() * sizeof(gph::DevState),
-                    hipMemcpyDeviceToHost));
+  GPH_HIP(hipMemcpyAsync(st.data(), inner_.get(), st.size() * sizeof(gph::DevState),
+                         hipMemcpyDeviceToHost, s));
+  GPH_HIP(hipStreamSynchronize(s));
   for (int32_t k = 0; k < S_; k++) innerIts[(size_t)k] += st[(size_t)k].iter;
 }
 
 std::vector<SystemStats> BatchSolver::solve(const Vec& x, const Vec& b) {
   const double t0 = now();
-  hipStream_t s = A_.stream();
-  const int64_t n = A_.n();
+  hipStream_t s = stream();
+  const int64_t n = sysRow_[(size_t)S_];  // rows of all systems (right-hand sides)
   if (b.type != DType::F32) fail(gph::kUnsupported, "batched solve: b must be float32");
   if (x.type != (mpir_ ? DType::F64 : DType::F32))
     fail(gph::kUnsupported, mpir_ ? "batched refinement: x must be float64"
@@ -286,8 +360,9 @@ std::vector<SystemStats> BatchSolver::solve(const Vec& x, const Vec& b) {
     innerSolve(b.ptr, d_.get(), innerIts);
     GPH_HIP(hipMemcpyAsync(x.ptr, d_.get(), (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     std::vector<gph::DevState> st((size_t)S_);
-    GPH_HIP(hipMemcpy(st.data(), inner_.get(), st.size() * sizeof(gph::DevState),
-                      hipMemcpyDeviceToHost));
+    GPH_HIP(hipMemcpyAsync(st.data(), inner_.get(), st.size() * sizeof(gph::DevState),
+                           hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipStreamSynchronize(s));
     for (int32_t k = 0; k < S_; k++) {
       const auto& q = st[(size_t)k];
       out[(size_t)k] = SystemStats{q.iter, 0, q.r0n, q.rF, q.converged != 0, q.singular != 0};
@@ -302,13 +377,17 @@ std::vector<SystemStats> BatchSolver::solve(const Vec& x, const Vec& b) {
   gph::launchFill(x.ptr, DType::F64, n, 0.0, s);
   gph::DevState* outer = outer_.as<gph::DevState>();
   auto residual = [&](int first) {
-    gph::SpmvArgs a = A_.spmvArgs(DType::F32);
-    a.x = x.ptr;
-    a.b = b.ptr;
-    a.out = rw_.get();
-    a.out64 = r64_.as<double>();
-    gph::launchSpmv(a, DType::F32, DType::F64, DType::F32, DType::F32, DType::F64,
-                    gph::kModeResidual, gph::kRedNone, s);
+    // the matrix's own rows once (a batch), or once per right-hand side
+    const int64_t m = A_.n();
+    for (int32_t j = 0; j < (multi_ ? S_ : 1); j++) {
+      gph::SpmvArgs a = A_.spmvArgs(DType::F32);
+      a.x = static_cast<double*>(x.ptr) + j * m;
+      a.b = static_cast<const float*>(b.ptr) + j * m;
+      a.out = rw_.as<float>() + j * m;
+      a.out64 = r64_.as<double>() + j * m;
+      gph::launchSpmv(a, DType::F32, DType::F64, DType::F32, DType::F32, DType::F64,
+                      gph::kModeResidual, gph::kRedNone, s);
+    }
     gph::launchBatchOuter(args_, outer, r64_.as<double>(), first, s);
   };
   residual(1);

@@ -41,7 +41,19 @@ class BatchSolver {
   // PBiCGStab inner solver, or PBiCGStab, each with an ILU preconditioner.
   BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_t>& systemTileStart,
               const std::string& configJson);
+  // nrhs (1..8) right-hand sides of the one system A: the vectors are nrhs
+  // blocks of A.n() rows (block j = right-hand side j), the matrix streams
+  // are read once per product for all of them and every tile's ILU(0)
+  // substitution runs for each of them in one launch. Runs on a stream of
+  // its own; `ilu` (optional) is a factorized ILU preconditioner of A shared
+  // with other solvers (its data is only read).
+  BatchSolver(const matrix::DeviceMatrix& A, int32_t nrhs, const std::string& configJson,
+              std::shared_ptr<matrix::solver::Solver> ilu);
   ~BatchSolver();
+
+  // device memory held by this solver (vectors, states, scratch)
+  size_t deviceBytes() const;
+  hipStream_t stream() const { return multi_ ? stream_ : A_.stream(); }
 
   int32_t numSystems() const { return S_; }
   int64_t systemRowStart(int32_t s) const { return sysRow_[(size_t)s]; }
@@ -56,6 +68,8 @@ class BatchSolver {
   int64_t lastGraphLaunches() const { return launches_; }
 
  private:
+  std::shared_ptr<matrix::solver::ilu::Configuration> parseConfig(const std::string& json);
+  void setupChunks(hipStream_t st);
   void innerSolve(const void* rhs, void* x, std::vector<int64_t>& innerIts);
   void enqueueIteration(hipStream_t s);
   void captureGraph(hipStream_t s);
@@ -67,7 +81,9 @@ class BatchSolver {
   int32_t rowsPerWave_ = 0;
   std::string substitution_;
   gph::DevState innerInit_{}, outerInit_{};
-  std::unique_ptr<matrix::solver::Solver> ilu_;
+  std::shared_ptr<matrix::solver::Solver> ilu_;
+  bool multi_ = false;  // nrhs right-hand sides of one system
+  hipStream_t stream_ = nullptr;
   gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_, sliceSys_;
   gph::DevBuf r_, r0_, p_, v_, y_, s_, z_, t_, d_, rw_, r64_;
   gph::BatchArgs args_;

@@ -642,6 +642,22 @@ int gph_batch_create(const gph_matrix* A, int32_t n_systems,
     *out = b;
   });
 }
+int gph_batch_create_rhs(const gph_matrix* A, int32_t n_rhs, const char* json,
+                         gph_batch** out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(json, "json");
+    need(out, "out");
+    auto* b = new gph_batch;
+    try {
+      b->B = std::make_unique<graphene::batch::BatchSolver>(*A->A, n_rhs, json, nullptr);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
 void gph_batch_destroy(gph_batch* b) { delete b; }
 int gph_batch_info(const gph_batch* b, int32_t* n_systems, int32_t* refinement,
                    int32_t* rows_per_wave) {

@@ -308,6 +308,24 @@ const void* DeviceMatrix::crsDiag(DType vt) const {
   return nullptr;
 }
 
+gph::SpmvArgs DeviceMatrix::sellArgs(DType vt) const {
+  gph::SpmvArgs a;
+  a.n = n_;
+  a.diag = crsDiag(vt);
+  const Sell& sl = fmt_ == MatrixFormat::CRS ? sellCrs_ : sellLdu_;
+  if (sl.numSlices > 0) {
+    a.numSlices = sl.numSlices;
+    a.sliceOff = sl.sliceOff.as<int32_t>();
+    a.rowLen = sl.rowLen.as<uint16_t>();
+    a.perm = sl.perm ? sl.perm.as<int32_t>() : nullptr;
+    a.sliceBase = sl.sliceBase.as<int32_t>();
+    a.sliceColW = sl.sliceColW.as<uint32_t>();
+    a.scol = sl.col.as<uint16_t>();
+    a.sval = sellVal(sl, vt);
+  }
+  return a;
+}
+
 gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
   gph::SpmvArgs a;
   a.n = n_;

@@ -41,6 +41,9 @@ class DeviceMatrix {
   // Row-gather arguments for values stored in `vt` (copies in float16 /
   // float64 are made on first use).
   gph::SpmvArgs spmvArgs(gph::DType vt) const;
+  // The sliced-ELL form of the matrix's row streams, whichever form
+  // spmvArgs prefers (numSlices = 0 if the matrix has none).
+  gph::SpmvArgs sellArgs(gph::DType vt) const;
   gph::TileArgs tileArgs() const;
   const void* crsOff(gph::DType vt) const;   // CRS off-diagonal values
   const void* crsDiag(gph::DType vt) const;

@@ -192,7 +192,17 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   constexpr int NV = (kRowsChunk * (int)sizeof(FT)) / 16;  // 16-byte value words per lane
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
-  const int tile = blockIdx.x;
+  int tile = blockIdx.x;
+  if (p.nrhs > 1) {
+    // (tile, right-hand side) pairs, the pairs of a tile on one XCD (its
+    // factor streams are read from that XCD's L2 after the first)
+    const int lb = xcdRemap(blockIdx.x, gridDim.x);
+    tile = lb / p.nrhs;
+    const int j = lb - tile * p.nrhs;
+    if (p.rhsState && p.rhsState[j].done) return;
+    rhs += j * p.rhsStride;
+    out += j * p.rhsStride;
+  }
   // batched solves: the tile's system has finished
   if (p.tileSys && p.sysState[p.tileSys[tile]].done) return;
   const RowsLds L = rowsLayout(p, sizeof(X), D);
@@ -349,7 +359,9 @@ static void rowsTyped(const RowsArgs& p, const void* invD, const void* rhs, void
   if (shm > 64 * 1024)
     GPH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));
-  hipLaunchKernelGGL(k, dim3(p.T), dim3(kRowsThreads), shm, s, p,
+  if (p.nrhs > 1 && pre.kind != kPreNone)
+    fail(kUnsupported, "several right-hand sides: no fused vector update");
+  hipLaunchKernelGGL(k, dim3(p.T * (p.nrhs > 1 ? p.nrhs : 1)), dim3(kRowsThreads), shm, s, p,
                      static_cast<const InvDiag<FT>*>(invD), static_cast<const X*>(rhs),
                      static_cast<X*>(out), pre, st, checkDone);
   GPH_HIP(hipGetLastError());

@@ -64,6 +64,11 @@ enum RedKind : int32_t { kRedNone = 0, kRedSelf = 1, kRedW = 2, kRedSelfW = 3 };
 // (residual); ot: output type; at: accumulation type.
 void launchSpmv(const SpmvArgs& a, DType vt, DType xt, DType bt, DType ot,
                 DType at, int mode, int red, hipStream_t s);
+// y_j = A x_j for k (1..8) float32 vectors of the sliced layout at
+// x/out + j*stride, the matrix streams read once (k_multi.hip); vector j is
+// skipped when states[j].done, every vector when glob->done (both optional).
+void launchSpmvMulti(const SpmvArgs& a, int k, int64_t stride, const DevState* states,
+                     const DevState* glob, hipStream_t s);
 
 // --- vector kernels -------------------------------------------------------
 struct VecArgs {
@@ -219,6 +224,11 @@ struct RowsArgs {
   // has finished (sysState[tileSys[t]].done)
   const DevState* sysState = nullptr;
   const int32_t* tileSys = nullptr;
+  // several right-hand sides of the same matrix in one launch: vector j is
+  // rhs/out + j * rhsStride; it is skipped when rhsState[j].done (optional)
+  int32_t nrhs = 1;
+  int64_t rhsStride = 0;
+  const DevState* rhsState = nullptr;
 };
 constexpr int kRowsThreads = 64;
 constexpr int kRowsMaxLen = 32;  // entries of a record in one step (long record)

@@ -33,6 +33,18 @@ void Solver::applyDevice(const Vec&, const Vec&, const gph::PreOp&, DevState*,
   fail(kUnsupported, name() + " cannot be used as a preconditioner");
 }
 
+void Solver::applyDeviceMulti(const Vec& rhs, const Vec& out, int k, int64_t stride,
+                              const DevState* states, DevState* st, hipStream_t s) {
+  (void)states;
+  const size_t es = gph::dtypeSize(out.type);
+  for (int j = 0; j < k; j++) {
+    const Vec r{static_cast<char*>(rhs.ptr) + (size_t)j * stride * gph::dtypeSize(rhs.type),
+                rhs.type};
+    const Vec o{static_cast<char*>(out.ptr) + (size_t)j * stride * es, out.type};
+    applyDevice(r, o, gph::PreOp{}, st, s);
+  }
+}
+
 Vec Workspace::get(int slot, DType t, int64_t n) {
   if ((int)e_.size() <= slot) e_.resize((size_t)slot + 1);
   Entry& e = e_[(size_t)slot];
@@ -135,6 +147,21 @@ class IluSolver : public Solver {
     }
     gph::launchIluApply(A_.tileArgs(), lu, inv, ft, rhs.ptr, out.ptr, out.type,
                         pre, st, st ? 1 : 0, cfg_->diagonalBased, s);
+  }
+
+  void applyDeviceMulti(const Vec& rhs, const Vec& out, int k, int64_t stride,
+                        const DevState* states, DevState* st, hipStream_t s) override {
+    DType ft = ft_;
+    if (rows_ && gph::rowsSharedBytes(rargs_, ft, out.type) <= kMaxPackedShared) {
+      gph::RowsArgs a = rargs_;
+      a.nrhs = k;
+      a.rhsStride = stride;
+      a.rhsState = states;
+      gph::launchIluRows(a, invD_.get(), ft, rhs.ptr, out.ptr, out.type, gph::PreOp{}, st,
+                         st ? 1 : 0, s);
+      return;
+    }
+    Solver::applyDeviceMulti(rhs, out, k, stride, states, st, s);
   }
 
   void setSystemSkip(const DevState* states,

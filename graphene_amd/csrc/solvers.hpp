@@ -61,6 +61,13 @@ class Solver {
   }
   virtual void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,
                            gph::DevState* st, hipStream_t s);
+  // Several right-hand sides of the same matrix: out_j = M^-1 rhs_j for
+  // j < k, vectors at rhs/out + j * stride elements; vector j may be skipped
+  // when states[j].done, and every vector when st->done. The solver's own
+  // data is only read, so concurrent calls on different streams are safe.
+  virtual void applyDeviceMulti(const Vec& rhs, const Vec& out, int k, int64_t stride,
+                                const gph::DevState* states, gph::DevState* st,
+                                hipStream_t s);
 
   static std::unique_ptr<Solver> createSolver(
       const DeviceMatrix& A, std::shared_ptr<Configuration> config);

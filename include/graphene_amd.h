@@ -304,6 +304,16 @@ typedef struct gph_batch gph_batch;
 int gph_batch_create(const gph_matrix* A, int32_t n_systems,
                      const int32_t* system_tile_start, const char* json,
                      gph_batch** out);
+/* No reference counterpart: n_rhs (1..8) right-hand sides of one system A,
+ * solved together as a batch whose systems are the right-hand sides: x and b
+ * hold n_rhs blocks of A's rows (block j = right-hand side j, tile-major
+ * order). Each product reads the matrix streams once for all right-hand
+ * sides and each tile's ILU(0) substitution runs for all of them in one
+ * launch; every right-hand side keeps its own scalars and convergence. The
+ * matrix needs its sliced layout (every CRS and LDU device matrix has one).
+ * Solve, info and destroy as for gph_batch_create. */
+int gph_batch_create_rhs(const gph_matrix* A, int32_t n_rhs, const char* json,
+                         gph_batch** out);
 void gph_batch_destroy(gph_batch* b);
 /* refinement: 1 for IterativeRefinement (x float64), 0 for PBiCGStab (x
  * float32); rows_per_wave: the packed ILU kernel's rows per wave chosen at
