@@ -48,34 +48,43 @@ __global__ __launch_bounds__(kBatchThreads) void batchDotKernel(BatchArgs b, Bat
   }
 }
 
-// One thread per system: adds the system's chunk partials in chunk order
-// and applies the scalar epilogue `op` (the single-system finalize). Then
-// the block counts the systems still running and sets glob->done when none
-// is (one block; the counts are integers, so the result is order-free).
+// One wave per system: lane l adds the system's chunk partials l, l + 64,
+// ... in chunk order, the lanes' sums are combined in a fixed shuffle tree
+// (the order depends only on the system's own chunk count), and lane 0
+// applies the scalar epilogue `op` (the single-system finalize). Then the
+// block counts the systems still running and sets glob->done when none is
+// (one block; the counts are integers, so the result is order-free).
 template <typename X>
 __global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int K) {
   __shared__ int nActive;
   if (op != kFinInit && b.glob->done) return;
   if (threadIdx.x == 0) nActive = 0;
   __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  const X* part = static_cast<const X*>(b.partials);
   int mine = 0;
-  for (int s = threadIdx.x; s < b.S; s += blockDim.x) {
+  for (int s = wave; s < b.S; s += nw) {
     DevState* st = b.st + s;
-    if (st->done && op != kFinInit) continue;
+    if (st->done && op != kFinInit) continue;  // the same for every lane of the wave
     double sum[2] = {0.0, 0.0};
     for (int k = 0; k < K; k++) {
       X acc = (X)0;
-      for (int c = b.sysChunk[s]; c < b.sysChunk[s + 1]; c++)
-        acc = acc + static_cast<const X*>(b.partials)[k * b.C + c];
-      sum[k] = (double)acc;
+      for (int c = b.sysChunk[s] + lane; c < b.sysChunk[s + 1]; c += 64)
+        acc = acc + part[k * b.C + c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+      sum[k] = (double)acc;  // lane 0's is the total
     }
-    if (op == kFinInit && st->done) {
-      // a system excluded from this solve (its outer iteration has
-      // converged) stays finished
-    } else {
-      finalize<X>(st, op, sum[0], sum[1]);
+    if (lane == 0) {
+      if (op == kFinInit && st->done) {
+        // a system excluded from this solve (its outer iteration has
+        // converged) stays finished
+      } else {
+        finalize<X>(st, op, sum[0], sum[1]);
+      }
+      mine += st->done ? 0 : 1;
     }
-    mine += st->done ? 0 : 1;
   }
   if (op == kFinInit || op == kFinUpdate || op == kFinOmega) {
     atomicAdd(&nActive, mine);
@@ -176,13 +185,19 @@ __global__ __launch_bounds__(kBatchThreads) void batchAccumulateKernel(BatchArgs
 // loop condition of iterative-refinement/Solver.cpp:89-112. The inner state
 // of a system that has finished refining is marked finished, so the next
 // inner solve skips it.
+// One wave per system, the chunk partials summed as in batchFinKernel.
 __global__ __launch_bounds__(1024) void batchOuterKernel(BatchArgs b, DevState* outer,
                                                          int first) {
-  for (int s = threadIdx.x; s < b.S; s += blockDim.x) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  for (int s = wave; s < b.S; s += nw) {
     DevState* o = outer + s;
     double acc = 0.0;
-    for (int c = b.sysChunk[s]; c < b.sysChunk[s + 1]; c++)
+    for (int c = b.sysChunk[s] + lane; c < b.sysChunk[s + 1]; c += 64)
       acc = acc + static_cast<const double*>(b.partials)[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+    if (lane != 0) continue;
     const double rF = sqrt(acc);
     if (first) {
       o->r0n = rF;
