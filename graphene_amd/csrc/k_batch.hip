@@ -48,12 +48,32 @@ __global__ __launch_bounds__(kBatchThreads) void batchDotKernel(BatchArgs b, Bat
   }
 }
 
-// One wave per system: lane l adds the system's chunk partials l, l + 64,
-// ... in chunk order, the lanes' sums are combined in a fixed shuffle tree
-// (the order depends only on the system's own chunk count), and lane 0
-// applies the scalar epilogue `op` (the single-system finalize). Then the
-// block counts the systems still running and sets glob->done when none is
-// (one block; the counts are integers, so the result is order-free).
+// Sum of one system's chunk partials: up to kSerialChunks chunks one thread
+// adds them in chunk order; more chunks, one wave: lane l adds chunks l,
+// l + 64, ... in order and the lanes' sums are combined in a fixed shuffle
+// tree. Either way the order depends only on the system's own chunk count.
+constexpr int kSerialChunks = 32;
+
+template <typename X>
+__device__ __forceinline__ X chunkSumSerial(const X* part, int c0, int c1) {
+  X acc = (X)0;
+#pragma unroll 8
+  for (int c = c0; c < c1; c++) acc = acc + part[c];
+  return acc;
+}
+template <typename X>
+__device__ __forceinline__ X chunkSumWave(const X* part, int c0, int c1, int lane) {
+  X acc = (X)0;
+  for (int c = c0 + lane; c < c1; c += 64) acc = acc + part[c];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
+  return acc;  // lane 0's is the total
+}
+
+// The chunk partials of every system summed (chunkSum*) and the scalar
+// epilogue `op` applied (the single-system finalize); then the block counts
+// the systems still running and sets glob->done when none is (one block;
+// the counts are integers, so the result is order-free).
 template <typename X>
 __global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int K) {
   __shared__ int nActive;
@@ -64,27 +84,34 @@ __global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int 
   const int nw = (int)(blockDim.x >> 6);
   const X* part = static_cast<const X*>(b.partials);
   int mine = 0;
+  auto finish = [&](DevState* st, const double (&sum)[2]) {
+    if (op == kFinInit && st->done) {
+      // a system excluded from this solve (its outer iteration has
+      // converged) stays finished
+    } else {
+      finalize<X>(st, op, sum[0], sum[1]);
+    }
+    mine += st->done ? 0 : 1;
+  };
+  // systems with few chunks: one thread each
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x) {
+    const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
+    if (c1 - c0 > kSerialChunks) continue;
+    DevState* st = b.st + s;
+    if (st->done && op != kFinInit) continue;
+    double sum[2] = {0.0, 0.0};
+    for (int k = 0; k < K; k++) sum[k] = (double)chunkSumSerial(part + k * b.C, c0, c1);
+    finish(st, sum);
+  }
+  // the others: one wave each
   for (int s = wave; s < b.S; s += nw) {
+    const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
+    if (c1 - c0 <= kSerialChunks) continue;
     DevState* st = b.st + s;
     if (st->done && op != kFinInit) continue;  // the same for every lane of the wave
     double sum[2] = {0.0, 0.0};
-    for (int k = 0; k < K; k++) {
-      X acc = (X)0;
-      for (int c = b.sysChunk[s] + lane; c < b.sysChunk[s + 1]; c += 64)
-        acc = acc + part[k * b.C + c];
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
-      sum[k] = (double)acc;  // lane 0's is the total
-    }
-    if (lane == 0) {
-      if (op == kFinInit && st->done) {
-        // a system excluded from this solve (its outer iteration has
-        // converged) stays finished
-      } else {
-        finalize<X>(st, op, sum[0], sum[1]);
-      }
-      mine += st->done ? 0 : 1;
-    }
+    for (int k = 0; k < K; k++) sum[k] = (double)chunkSumWave(part + k * b.C, c0, c1, lane);
+    if (lane == 0) finish(st, sum);
   }
   if (op == kFinInit || op == kFinUpdate || op == kFinOmega) {
     atomicAdd(&nActive, mine);
@@ -179,39 +206,46 @@ __global__ __launch_bounds__(kBatchThreads) void batchAccumulateKernel(BatchArgs
     phi[i] = phi[i] + (double)d[i];
 }
 
-// Outer (MPIR) residual norms, per system: the norm of the unrounded FP64
-// residual; on the first call the initial norm (r0n), afterwards one
+// One system's outer (MPIR) step from the norm rF of its unrounded FP64
+// residual: on the first call the initial norm (r0n), afterwards one
 // refinement step is counted and convergence tested against it, with the
 // loop condition of iterative-refinement/Solver.cpp:89-112. The inner state
 // of a system that has finished refining is marked finished, so the next
 // inner solve skips it.
-// One wave per system, the chunk partials summed as in batchFinKernel.
+__device__ __forceinline__ void outerStep(BatchArgs& b, DevState* o, int s, double rF,
+                                          int first) {
+  if (first) {
+    o->r0n = rF;
+    o->rF = rF;
+    o->iter = 0;
+    o->converged = 0;
+    o->done = o->maxIt <= 0 ? 1 : 0;
+  } else if (!o->done) {
+    o->iter = o->iter + 1;
+    o->rF = rF;
+    o->converged = convergedTest(o, rF) ? 1 : 0;
+    o->done = (o->converged && o->iter >= o->minIt) || o->iter >= o->maxIt ? 1 : 0;
+  }
+  b.st[s].done = o->done;
+}
+
+// Outer residual norms of every system, the chunk partials summed as in
+// batchFinKernel (one thread per system with few chunks, one wave per
+// system with more).
 __global__ __launch_bounds__(1024) void batchOuterKernel(BatchArgs b, DevState* outer,
                                                          int first) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (int)(blockDim.x >> 6);
-  for (int s = wave; s < b.S; s += nw) {
-    DevState* o = outer + s;
-    double acc = 0.0;
-    for (int c = b.sysChunk[s] + lane; c < b.sysChunk[s + 1]; c += 64)
-      acc = acc + static_cast<const double*>(b.partials)[c];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc = acc + __shfl_down(acc, off, 64);
-    if (lane != 0) continue;
-    const double rF = sqrt(acc);
-    if (first) {
-      o->r0n = rF;
-      o->rF = rF;
-      o->iter = 0;
-      o->converged = 0;
-      o->done = o->maxIt <= 0 ? 1 : 0;
-    } else if (!o->done) {
-      o->iter = o->iter + 1;
-      o->rF = rF;
-      o->converged = convergedTest(o, rF) ? 1 : 0;
-      o->done = (o->converged && o->iter >= o->minIt) || o->iter >= o->maxIt ? 1 : 0;
+  const double* part = static_cast<const double*>(b.partials);
+  for (int q = 0; q < 2; q++) {  // pass 0: thread per system; pass 1: wave per system
+    for (int s = q == 0 ? (int)threadIdx.x : wave; s < b.S;
+         s += q == 0 ? (int)blockDim.x : nw) {
+      const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
+      if ((c1 - c0 > kSerialChunks) != (q == 1)) continue;
+      const double acc = q == 0 ? chunkSumSerial(part, c0, c1) : chunkSumWave(part, c0, c1, lane);
+      if (q == 1 && lane != 0) continue;
+      outerStep(b, outer + s, s, sqrt(acc), first);
     }
-    b.st[s].done = o->done;
   }
 }
 

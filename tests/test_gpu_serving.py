@@ -93,3 +93,25 @@ def test_request_errors(systems):
     sid = small.add_system(systems[0][0], MPIR)
     with pytest.raises(g.GrapheneError):
         small.run([sid], [1], [0.0], [1])
+
+
+def test_requests_on_one_system_overlap_and_equal_alone(systems):
+    """Concurrent requests on the same system run on slots of their own
+    (block solves of their right-hand sides) and overlap on the device; each
+    result equals the same request served alone."""
+    S, ids = make_server(systems[1:2], workers=4)
+    rows = systems[1][2]
+    alone = {}
+    for sd in (11, 12, 13, 14):
+        _, sol = S.run([ids[0]], [3], [0.0], [sd], keep_solution=[(0, rows)])
+        alone[sd] = sol[0]
+    res, sol = S.run([ids[0]] * 4, [3] * 4, [0.0] * 4, [11, 12, 13, 14],
+                     keep_solution=[(i, rows) for i in range(4)])
+    assert np.all(res["converged"] == 1) and np.all(res["rel_residual"] <= 1e-10)
+    for i, sd in enumerate((11, 12, 13, 14)):
+        assert np.array_equal(sol[i], alone[sd])
+    fin = res["latency"]
+    start = fin - res["service"]
+    # at least two services overlap in time
+    order = np.argsort(start)
+    assert np.any(start[order][1:] < fin[order][:-1])
