@@ -119,6 +119,7 @@ _SIGS = {
     "gph_server_destroy": ([_P], None),
     "gph_server_add_system": ([_P, _P, C.c_char_p, C.POINTER(C.c_int32)], C.c_int),
     "gph_server_footprint": ([_P, C.c_int32, C.POINTER(C.c_size_t)], C.c_int),
+    "gph_server_resident_bytes": ([_P, C.POINTER(C.c_size_t)], C.c_int),
     "gph_server_run": ([_P, C.c_int64, _P, _P, _P, _P, _P, C.POINTER(_ReqResult)], C.c_int),
 }
 
@@ -624,6 +625,11 @@ class Server:
     def footprint(self, sid):
         b = C.c_size_t()
         _check(lib().gph_server_footprint(self._h, int(sid), C.byref(b)))
+        return b.value
+
+    def resident_bytes(self):
+        b = C.c_size_t()
+        _check(lib().gph_server_resident_bytes(self._h, C.byref(b)))
         return b.value
 
     def run(self, system, n_rhs, arrival, seed, keep_solution=None):

@@ -724,6 +724,13 @@ int gph_server_footprint(const gph_server* s, int32_t system_id, size_t* bytes) 
     *bytes = s->S->footprint(system_id);
   });
 }
+int gph_server_resident_bytes(const gph_server* s, size_t* bytes) {
+  return guard([&] {
+    need(s, "server");
+    need(bytes, "bytes");
+    *bytes = s->S->residentBytes();
+  });
+}
 int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
                    const int32_t* n_rhs, const double* arrival_s,
                    const uint64_t* seed, double* const* x0_out,

@@ -260,10 +260,16 @@ int gph_config_check(const char* json, char* name_buf, size_t len);
  * (a device matrix + a PBiCGStab or IterativeRefinement configuration) are
  * registered once and stay resident with their factors and workspace;
  * requests (system, right-hand sides, seed) arrive on a schedule and are
- * admitted by `workers` host threads when their system is idle and the
- * footprint of the requests in flight stays within `hbm_budget_bytes`; their
- * kernels overlap on the device. Each right-hand side is b = fl(A x_true),
- * x_true ~ U[-1,1) from the seed, solved from x = 0. */
+ * admitted in arrival order by `workers` host threads. A request runs on a
+ * slot: the Krylov workspace and buffers of its 1-8 right-hand sides, solved
+ * as one block on a stream of the slot's own (ILU-preconditioned
+ * configurations; others run their right-hand sides in turn, one request
+ * per system at a time), so requests on one system overlap and share its
+ * factors. Slots stay resident for later requests of the same system and
+ * right-hand-side count; every resident slot counts against
+ * `hbm_budget_bytes`, and a request that does not fit evicts idle slots,
+ * least recently used first, before it waits. Each right-hand side is
+ * b = fl(A x_true), x_true ~ U[-1,1) from the seed, solved from x = 0. */
 typedef struct gph_server gph_server;
 typedef struct {
   double latency;      /* finish - arrival, seconds */
@@ -280,7 +286,10 @@ void gph_server_destroy(gph_server* s);
 /* The matrix must outlive the server. */
 int gph_server_add_system(gph_server* s, const gph_matrix* A, const char* json,
                           int32_t* system_id);
+/* Device bytes of a slot for one right-hand side of the system. */
 int gph_server_footprint(const gph_server* s, int32_t system_id, size_t* bytes);
+/* Device bytes of the slots resident now (between runs). */
+int gph_server_resident_bytes(const gph_server* s, size_t* bytes);
 /* Runs n requests; arrival_s[i] = seconds after the start; x0_out (nullable,
  * or an array of n nullable pointers to n_rows doubles) receives the solution
  * of request i's first right-hand side in the matrix's tile-major order. */
