@@ -35,6 +35,7 @@ SOURCES = [
     "k_batch.hip",
     "k_noalloc.hip",
     "k_multi.hip",
+    "k_ilu_block.hip",
 ]
 
 COMMON = [

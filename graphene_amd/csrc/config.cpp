@@ -326,9 +326,10 @@ ilu::Configuration::Configuration(const Json& c) {
   if (const Json* j = c.get("substitution")) {
     substitution = asText(*j);
     if (substitution != "level" && substitution != "packed" && substitution != "tile" &&
-        substitution != "rows")
+        substitution != "rows" && substitution != "block")
       fail(kInvalid, "Invalid value for substitution: " + substitution +
-                         ". Allowed values are \"packed\", \"rows\", \"level\" or \"tile\"");
+                         ". Allowed values are \"packed\", \"rows\", \"level\", \"block\" "
+                         "or \"tile\"");
   }
 }
 

@@ -111,8 +111,9 @@ struct Configuration : solver::Configuration {
   int rowsPerWave = 1024;
   // extension: substitution kernel: "rows" (lanes per row, factor rows in
   // registers; the packed kernel where its formats do not fit), "packed"
-  // (product slabs), "level" (lane per row over level records), "tile" (row
-  // blocks)
+  // (product slabs), "level" (lane per row over level records), "block"
+  // (a workgroup per tile, any tile size; also where the one-wave kernels do
+  // not fit), "tile" (a workgroup per tile, unpacked factors)
   std::string substitution = "rows";
   explicit Configuration(const Json& c);
   std::string solverName() const override {

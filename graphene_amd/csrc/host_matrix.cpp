@@ -831,6 +831,31 @@ std::shared_ptr<HostMatrix> buildHostMatrix(TripletMatrix&& coo,
   return m;
 }
 
+void forwardLevelRows(const HostMatrix& m, std::vector<int32_t>& list,
+                      std::vector<int64_t>& levelOff) {
+  const int32_t T = (int32_t)m.numColors.size();
+  int32_t nl = 0;
+  for (int32_t t = 0; t < T; t++) nl = std::max(nl, m.numColors[(size_t)t]);
+  levelOff.assign((size_t)nl + 1, 0);
+  for (int32_t t = 0; t < T; t++) {
+    const int64_t* cs = m.colorStart.data() + m.colorStartOffset[(size_t)t];
+    for (int32_t L = 0; L < m.numColors[(size_t)t]; L++) levelOff[(size_t)L + 1] += cs[L + 1] - cs[L];
+  }
+  for (int32_t L = 0; L < nl; L++) levelOff[(size_t)L + 1] += levelOff[(size_t)L];
+  list.assign(2 * (size_t)levelOff[(size_t)nl], 0);
+  std::vector<int64_t> at(levelOff.begin(), levelOff.end() - 1);
+  for (int32_t t = 0; t < T; t++) {
+    const int64_t t0 = m.tileStart[(size_t)t];
+    const int64_t* cs = m.colorStart.data() + m.colorStartOffset[(size_t)t];
+    for (int32_t L = 0; L < m.numColors[(size_t)t]; L++)
+      for (int64_t q = cs[L]; q < cs[L + 1]; q++) {
+        const size_t r = (size_t)at[(size_t)L]++;
+        list[2 * r] = (int32_t)(t0 + m.colorSortAddr[(size_t)(t0 + q)]);
+        list[2 * r + 1] = t;
+      }
+  }
+}
+
 LDUHost buildLDU(const HostMatrix& m) {
   const int64_t n = m.nrows;
   LDUHost L;

@@ -124,6 +124,12 @@ std::shared_ptr<HostMatrix> buildHostMatrix(TripletMatrix&& coo,
                                             int32_t numTiles,
                                             const int32_t* rowToTile);
 
+// Rows of the forward levels of all tiles, level by level (level 0 of every
+// tile, then level 1, ...): list[2r] = row (tile-major), list[2r+1] = its
+// tile; levelOff[L]..levelOff[L+1]-1 are level L's entries.
+void forwardLevelRows(const HostMatrix& m, std::vector<int32_t>& list,
+                      std::vector<int64_t>& levelOff);
+
 // LDU addressing over the tile-major numbering (LDUAddressing.hpp:30-63).
 struct LDUHost {
   int64_t numFaces = 0;

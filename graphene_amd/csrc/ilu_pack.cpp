@@ -528,4 +528,127 @@ PackedILU packILU(const host::HostMatrix& m, int64_t targetRows) {
   return p;
 }
 
+// The workgroup-per-tile stream: the entries of every level piece in the
+// substitution's order (pass 1), then the records at the common width.
+namespace {
+struct BlockPiece {
+  int32_t tile = 0;
+  int64_t q0 = 0, R = 0;  // rows [q0, q0 + R) of the tile's level order
+  int32_t W = 0;
+  std::vector<int64_t> entOff;  // R + 1
+  std::vector<int64_t> ents;    // CRS positions
+};
+}  // namespace
+
+static std::vector<BlockPiece> blockPieces(const host::HostMatrix& m, bool bwd, int32_t& maxW,
+                                           std::vector<int32_t>& lvlBase) {
+  std::vector<BlockPiece> out;
+  const int32_t T = m.numTiles;
+  lvlBase.assign((size_t)T + 1, 0);
+  for (int32_t t = 0; t < T; t++) {
+    lvlBase[(size_t)t] = (int32_t)out.size();
+    const int64_t t0 = m.tileStart[(size_t)t], nt = m.tileRows(t);
+    const int64_t* cs = bwd ? m.upperColorStart.data() + m.upperColorStartOffset[(size_t)t]
+                            : m.colorStart.data() + m.colorStartOffset[(size_t)t];
+    const int32_t nl = bwd ? m.numUpperColors[(size_t)t] : m.numColors[(size_t)t];
+    const int32_t* rows = bwd ? m.upperLevelRows.data() : m.colorSortAddr.data();
+    for (int32_t L = 0; L < nl; L++)
+      for (int64_t q0 = cs[L]; q0 < cs[L + 1]; q0 += kBlockSlots) {
+        BlockPiece p;
+        p.tile = t;
+        p.q0 = q0;
+        p.R = std::min<int64_t>(kBlockSlots, cs[L + 1] - q0);
+        p.entOff.push_back(0);
+        for (int64_t q = 0; q < p.R; q++) {
+          const int32_t li = rows[(size_t)(t0 + q0 + q)];
+          const int64_t i = t0 + li;
+          if (!bwd) {
+            for (int64_t a = m.rowPtr[(size_t)i]; a < m.rowPtr[(size_t)i + 1]; a++) {
+              const int32_t c = m.colLocal[(size_t)a];
+              if (c >= nt || c > li) break;  // halo or upper part reached
+              p.ents.push_back(a);
+            }
+          } else {
+            for (int64_t a = m.rowPtr[(size_t)i + 1] - 1; a >= m.rowPtr[(size_t)i]; a--) {
+              const int32_t c = m.colLocal[(size_t)a];
+              if (c >= nt) continue;  // halo
+              if (c < li) break;
+              p.ents.push_back(a);
+            }
+          }
+          p.entOff.push_back((int64_t)p.ents.size());
+          p.W = std::max<int32_t>(p.W, (int32_t)(p.entOff[(size_t)q + 1] - p.entOff[(size_t)q]));
+        }
+        maxW = std::max(maxW, p.W);
+        out.push_back(std::move(p));
+      }
+  }
+  lvlBase[(size_t)T] = (int32_t)out.size();
+  return out;
+}
+
+static bool emitBlockDirection(const host::HostMatrix& m, const std::vector<BlockPiece>& pieces,
+                               const uint8_t* lu, int es, const uint8_t* inv, int ies, int width,
+                               bool bwd, IluBlockDirection& d) {
+  const size_t rs = ies == 8 ? 16 : 8;  // row record
+  const size_t er = es == 8 ? 16 : 8;   // entry record
+  const int32_t* rows = bwd ? m.upperLevelRows.data() : m.colorSortAddr.data();
+  int64_t slots = 0, ents = 0;
+  for (const BlockPiece& p : pieces) {
+    if (slots + p.R > INT32_MAX || ents + (int64_t)width * p.R > INT32_MAX) return false;
+    d.info.insert(d.info.end(), {(int32_t)slots, (int32_t)p.R, (int32_t)ents, p.W});
+    slots += p.R;
+    ents += (int64_t)width * p.R;
+  }
+  d.rows.assign((size_t)std::max<int64_t>(slots, 1) * rs, 0);
+  d.ents.assign((size_t)std::max<int64_t>(ents, 1) * er, 0);
+  for (size_t k = 0; k < pieces.size(); k++) {
+    const BlockPiece& p = pieces[k];
+    const int64_t t0 = m.tileStart[(size_t)p.tile];
+    const uint32_t zero = (uint32_t)m.tileRows(p.tile);
+    const int64_t slot0 = d.info[4 * k], ent0 = d.info[4 * k + 2];
+    for (int64_t q = 0; q < p.R; q++) {
+      const int32_t li = rows[(size_t)(t0 + p.q0 + q)];
+      uint8_t* rr = &d.rows[(size_t)(slot0 + q) * rs];
+      std::memcpy(rr, &li, 4);
+      std::memcpy(rr + rs - (size_t)ies, inv + (size_t)(t0 + li) * (size_t)ies, (size_t)ies);
+      const int64_t e0 = p.entOff[(size_t)q], n = p.entOff[(size_t)q + 1] - e0;
+      for (int64_t j = 0; j < width; j++) {
+        uint8_t* er_ = &d.ents[(size_t)(ent0 + j * p.R + q) * er];
+        if (j < n) {
+          const int64_t a = p.ents[(size_t)(e0 + j)];
+          const uint32_t c = (uint32_t)m.colLocal[(size_t)a];
+          std::memcpy(er_, &c, 4);
+          std::memcpy(er_ + er - (size_t)(es == 8 ? 8 : 4), lu + (size_t)a * (size_t)es, (size_t)es);
+        } else {
+          std::memcpy(er_, &zero, 4);  // the zero slot, value +0
+        }
+      }
+    }
+  }
+  if (d.info.empty()) d.info.assign(4, 0);
+  return true;
+}
+
+IluBlockStream packIluBlock(const host::HostMatrix& m, const void* lu, int es, const void* invD,
+                            int ies) {
+  IluBlockStream o;
+  for (int32_t t = 0; t < m.numTiles; t++)
+    o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
+  int32_t maxW = 0;
+  const std::vector<BlockPiece> f = blockPieces(m, false, maxW, o.fwd.lvlBase);
+  const std::vector<BlockPiece> b = blockPieces(m, true, maxW, o.bwd.lvlBase);
+  if (maxW > kBlockMaxWidth) return o;
+  o.width = maxW <= 4 ? 4 : maxW <= 8 ? 8 : 16;
+  const uint8_t* luB = static_cast<const uint8_t*>(lu);
+  const uint8_t* invB = static_cast<const uint8_t*>(invD);
+  o.ok = emitBlockDirection(m, f, luB, es, invB, ies, o.width, false, o.fwd) &&
+         emitBlockDirection(m, b, luB, es, invB, ies, o.width, true, o.bwd);
+  for (int32_t t = 0; o.ok && t < m.numTiles; t++)
+    o.maxLevels = std::max<int32_t>(
+        o.maxLevels, o.fwd.lvlBase[(size_t)t + 1] - o.fwd.lvlBase[(size_t)t] +
+                         o.bwd.lvlBase[(size_t)t + 1] - o.bwd.lvlBase[(size_t)t]);
+  return o;
+}
+
 }  // namespace graphene::matrix

@@ -149,6 +149,46 @@ struct IluRowStream {
 
 IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es);
 
+// Level-major stream of the ILU(0) factors for the workgroup-per-tile
+// kernel (tiles of any size; x in LDS when it fits).
+//
+// Per direction (forward: strictly lower in-tile entries in stored,
+// ascending order; backward: strictly upper in-tile entries in descending
+// order, as ilu/Solver.cpp:380-442 visits them) and tile, the tile's levels
+// one after the other, a level of more than kBlockSlots rows split into
+// pieces of at most that many (consecutive "levels" of the stream). A level
+// of R rows: info {first slot, R, first entry, W (its longest row)}; slot s
+// is a record (local row id, the row's 1/d); entry j < width of slot s is
+// record first entry + j*R + s (local column, factor value). Every level has
+// `width` entries per slot (the largest W of the stream rounded up to 4, 8
+// or 16); entries past a row's end refer to the zero slot (local id = the
+// tile's row count, where the kernel keeps x = +0) with value +0, and
+// subtracting their +0 products leaves every partial sum unchanged.
+// Records are 8 bytes (32-bit id and a float32 value or float16 value bits)
+// or 16 bytes (id, 0, float64 value) for float64 factors.
+constexpr int kBlockMaxWidth = 16;
+constexpr int kBlockSlots = 1024;  // rows of a level piece (one per thread)
+
+struct IluBlockDirection {
+  std::vector<int32_t> lvlBase;  // T+1: each tile's first level
+  std::vector<int32_t> info;     // 4 per level
+  std::vector<uint8_t> rows;     // per slot: (row, 1/d) record
+  std::vector<uint8_t> ents;     // per entry: (column, value) record
+};
+
+struct IluBlockStream {
+  IluBlockDirection fwd, bwd;
+  int32_t width = 0;      // entries per slot (4, 8 or 16)
+  int32_t maxRows = 0;    // largest tile
+  int32_t maxLevels = 0;  // most levels of one tile, both directions together
+  bool ok = false;        // false: a level wider than kBlockMaxWidth, or too large
+};
+
+// lu: factor values (es bytes each, CRS order); invD: 1/d per row (ies
+// bytes: 8 for float64 factors, else 4).
+IluBlockStream packIluBlock(const host::HostMatrix& m, const void* lu, int es, const void* invD,
+                            int ies);
+
 // Device order of a 16-bit slot array: slabs 2p and 2p+1 are interleaved so
 // that lane l loads both of its slots with one 32-bit load,
 // out[p*128 + 2l + (s&1)] = in[s*64 + l].

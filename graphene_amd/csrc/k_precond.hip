@@ -21,86 +21,219 @@ size_t tileSharedBytes(const TileArgs& t, DType xt) {
 static size_t maxDynamicShared() { return 160 * 1024 - 4096; }
 
 // ---------------------------------------------------------------------------
-// ILU(0) factorization (Saad Alg. 10.4, IKJ), per tile, level by level.
+// ILU(0) factorization (Saad Alg. 10.4, IKJ), per tile.
 // Row i: for each k = col(ik) < i in-tile: a_ik /= a_kk; a_ij -= a_ik*a_kj
 // for the later in-tile entries j of row i (a_kj = 0 if absent); then
 // a_ii -= a_ik*a_ki. ilu/Solver.cpp:167-218. Afterwards diag = 1/diag
 // (:290-291). DILU (diagonalBased, :191-216): the off-diagonals stay A's
 // and a_ii -= (a_ik*a_ki)/a_kk.
+//
+// A row's entries are its in-tile columns ascending, then its halo columns;
+// the scans stop at the first halo column, so only that in-tile prefix
+// takes part. Rows of one forward level depend only on rows of lower
+// levels, whose factor rows are final: one launch per level over the
+// level's rows of every tile (a (row, tile) list), one wave per row. Lane e holds row i's entry e. For a batch of lower entries the wave
+// loads the rows k at once (lane q: entry q of row k), finds every a_kj by
+// a binary search of row k's in-tile columns in LDS and a shuffle, and a_ki
+// by a ballot; then it applies the updates of the batch's entries in order
+// (the subtraction order of the sequential loop, a_kj = 0 subtracted too).
+// Rows longer than 64 entries, or with a row k that is, run the sequential
+// loop on one lane.
 // ---------------------------------------------------------------------------
+constexpr int kFactorWaves = 4;  // rows (waves) per block
+constexpr int kFactorBatch = 8;  // rows k loaded together
+
 template <typename FT>
-__global__ __launch_bounds__(kTileThreads) void iluFactorKernel(TileArgs t,
-                                                                FT* off,
-                                                                FT* diag,
-                                                                int dilu) {
-  const int tile = blockIdx.x;
-  const int t0 = t.tileStart[tile], t1 = t.tileStart[tile + 1];
-  const int* lp = t.lvlPtr + t.lvlPtrOff[tile];
-  const int nL = t.lvlPtrOff[tile + 1] - t.lvlPtrOff[tile] - 1;
-  const int32_t* __restrict__ rowPtr = t.rowPtr;
-  const int32_t* __restrict__ col = t.col;
-  for (int L = 0; L < nL; L++) {
-    for (int q = lp[L] + threadIdx.x; q < lp[L + 1]; q += kTileThreads) {
-      const int i = t0 + t.lvlRows[t0 + q];
-      const int iEnd = rowPtr[i + 1];
-      for (int ik = rowPtr[i]; ik < iEnd; ik++) {
-        const int k = col[ik];
-        if (k >= i || k < t0) break;  // upper part or halo reached
-        const FT a_kk = diag[k];
-        const int kEnd = rowPtr[k + 1];
-        if (dilu) {
-          FT a_ki = 0;
-          for (int qq = rowPtr[k]; qq < kEnd; qq++) {
-            const int c = col[qq];
-            if (c < t0 || c >= t1 || c > i) break;
-            if (c == i) a_ki = off[qq];
-          }
-          diag[i] = diag[i] - off[ik] * a_ki / a_kk;
-          continue;
-        }
-        const FT a_ik = off[ik] / a_kk;
-        off[ik] = a_ik;
-        int qk = rowPtr[k];
-        for (int ij = ik + 1; ij < iEnd; ij++) {
-          const int j = col[ij];
-          if (j < t0 || j >= t1) break;  // stop at halo coefficients
-          while (qk < kEnd) {
-            const int c = col[qk];
-            if (c < t0 || c >= t1 || c >= j) break;
-            qk++;
-          }
-          const FT a_kj = (qk < kEnd && col[qk] == j) ? off[qk] : (FT)0;
-          off[ij] = off[ij] - a_ik * a_kj;
-        }
-        FT a_ki = 0;
-        for (int qq = rowPtr[k]; qq < kEnd; qq++) {
-          const int c = col[qq];
-          if (c < t0 || c >= t1 || c > i) break;
-          if (c == i) a_ki = off[qq];
-        }
-        diag[i] = diag[i] - a_ik * a_ki;
-      }
-    }
-    __syncthreads();
+__device__ __forceinline__ FT readLane(FT v, int l) {
+  if constexpr (sizeof(FT) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  } else {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
   }
-  for (int i = t0 + threadIdx.x; i < t1; i += kTileThreads)
-    diag[i] = (FT)1 / diag[i];
 }
 
-void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
-                     bool dilu, hipStream_t s) {
-  if (t.T <= 0) return;
+__device__ __forceinline__ uint64_t waveBallot(bool p) { return __ballot(p); }
+// Orders the wave's LDS writes before its later LDS reads (and reads before
+// later writes) across lanes.
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int firstSet(uint64_t m) { return m ? __builtin_ctzll(m) : 64; }
+
+// The sequential row loop (one lane).
+template <typename FT>
+__device__ void factorRowSerial(const int32_t* __restrict__ rowPtr, const int32_t* __restrict__ col,
+                                FT* off, FT* diag, int t0, int t1, int i, int dilu) {
+  const int iEnd = rowPtr[i + 1];
+  for (int ik = rowPtr[i]; ik < iEnd; ik++) {
+    const int k = col[ik];
+    if (k >= i || k < t0) break;  // upper part or halo reached
+    const FT a_kk = diag[k];
+    const int kEnd = rowPtr[k + 1];
+    if (dilu) {
+      FT a_ki = 0;
+      for (int qq = rowPtr[k]; qq < kEnd; qq++) {
+        const int c = col[qq];
+        if (c < t0 || c >= t1 || c > i) break;
+        if (c == i) a_ki = off[qq];
+      }
+      diag[i] = diag[i] - off[ik] * a_ki / a_kk;
+      continue;
+    }
+    const FT a_ik = off[ik] / a_kk;
+    off[ik] = a_ik;
+    int qk = rowPtr[k];
+    for (int ij = ik + 1; ij < iEnd; ij++) {
+      const int j = col[ij];
+      if (j < t0 || j >= t1) break;  // stop at halo coefficients
+      while (qk < kEnd) {
+        const int c = col[qk];
+        if (c < t0 || c >= t1 || c >= j) break;
+        qk++;
+      }
+      const FT a_kj = (qk < kEnd && col[qk] == j) ? off[qk] : (FT)0;
+      off[ij] = off[ij] - a_ik * a_kj;
+    }
+    FT a_ki = 0;
+    for (int qq = rowPtr[k]; qq < kEnd; qq++) {
+      const int c = col[qq];
+      if (c < t0 || c >= t1 || c > i) break;
+      if (c == i) a_ki = off[qq];
+    }
+    diag[i] = diag[i] - a_ik * a_ki;
+  }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(64 * kFactorWaves) void iluFactorLevelKernel(FactorArgs f, FT* off,
+                                                                          FT* diag, int dilu,
+                                                                          int64_t base,
+                                                                          int count) {
+  __shared__ int kcols[kFactorWaves][kFactorBatch][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int w = (int)blockIdx.x * kFactorWaves + wave;
+  if (w >= count) return;  // the wave's exit; the kernel has no block barrier
+  const int i = f.list[2 * (base + w)], tile = f.list[2 * (base + w) + 1];
+  const int t0 = f.tileStart[tile], t1 = f.tileStart[tile + 1];
+  const int32_t* __restrict__ rowPtr = f.rowPtr;
+  const int32_t* __restrict__ col = f.col;
+  const int rb = rowPtr[i], n = rowPtr[i + 1] - rb;
+  // lower in-tile entries lead the row; lane m < mL holds row k = c's start,
+  // length and diagonal
+  const int c = lane < n ? col[rb + lane] : -1;
+  FT v = lane < n ? off[rb + lane] : (FT)0;
+  const bool low = lane < n && c >= t0 && c < i;
+  const int mL = firstSet(waveBallot(!low));
+  const int kb = low ? rowPtr[c] : 0;
+  const int nk = low ? rowPtr[c + 1] - kb : 0;
+  const FT dk = low ? diag[c] : (FT)1;
+  if (n > 64 || waveBallot(nk > 64)) {
+    if (lane == 0) factorRowSerial(rowPtr, col, off, diag, t0, t1, i, dilu);
+    return;
+  }
+  const bool inT = lane < n && c >= t0 && c < t1;
+  const int P = firstSet(waveBallot(!inT));  // in-tile prefix of row i
+  FT di = diag[i];
+  int* kc = &kcols[wave][0][0];
+  for (int m0 = 0; m0 < mL; m0 += kFactorBatch) {
+    const int nb = min(kFactorBatch, mL - m0);
+    FT vk[kFactorBatch];
+    {
+      int ck[kFactorBatch];
+#pragma unroll
+      for (int b = 0; b < kFactorBatch; b++) {
+        ck[b] = -1;
+        vk[b] = (FT)0;
+        if (b < nb) {
+          const int k0 = __builtin_amdgcn_readlane(kb, m0 + b);
+          const int len = __builtin_amdgcn_readlane(nk, m0 + b);
+          if (lane < len) {
+            ck[b] = col[k0 + lane];
+            vk[b] = off[k0 + lane];
+          }
+        }
+      }
+      // row k's in-tile prefix in LDS, ascending, padded with INT_MAX
+#pragma unroll
+      for (int b = 0; b < kFactorBatch; b++) {
+        const int Pk = firstSet(waveBallot(!(ck[b] >= t0 && ck[b] < t1)));
+        kc[b * 64 + lane] = lane < Pk ? ck[b] : INT_MAX;
+      }
+    }
+    waveSync();
+    int pos[kFactorBatch];
+#pragma unroll
+    for (int b = 0; b < kFactorBatch; b++) pos[b] = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+#pragma unroll
+      for (int b = 0; b < kFactorBatch; b++)
+        if (kc[b * 64 + pos[b] + s - 1] < c) pos[b] += s;
+    }
+    FT akj[kFactorBatch], aki[kFactorBatch];
+#pragma unroll
+    for (int b = 0; b < kFactorBatch; b++) {
+      const bool hit = kc[b * 64 + pos[b]] == c;
+      const FT g = __shfl(vk[b], pos[b], 64);
+      akj[b] = hit ? g : (FT)0;
+      const int li = firstSet(waveBallot(kc[b * 64 + lane] == i));
+      aki[b] = li < 64 ? readLane(vk[b], li) : (FT)0;
+    }
+    waveSync();  // LDS reads before the next batch's writes
+#pragma unroll
+    for (int b = 0; b < kFactorBatch; b++) {
+      const int m = m0 + b;
+      const FT akk = readLane(dk, m0 + min(b, nb - 1));
+      if (b < nb && dilu) {
+        di = di - readLane(v, m) * aki[b] / akk;
+      } else if (b < nb) {
+        const FT aik = readLane(v, m) / akk;
+        if (lane == m) v = aik;
+        if (lane > m && lane < P) v = v - aik * akj[b];
+        di = di - aik * aki[b];
+      }
+    }
+  }
+  if (!dilu && lane < P) off[rb + lane] = v;
+  if (lane == 0) diag[i] = di;
+}
+
+template <typename FT>
+__global__ void invertDiagKernel(FT* diag, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) diag[i] = (FT)1 / diag[i];
+}
+
+template <typename FT>
+static void factorLevels(const FactorArgs& f, const std::vector<int64_t>& levelOff, int64_t n,
+                         FT* off, FT* diag, bool dilu, hipStream_t s) {
+  for (size_t L = 0; L + 1 < levelOff.size(); L++) {
+    const int64_t count = levelOff[L + 1] - levelOff[L];
+    if (count <= 0) continue;
+    const int64_t blocks = (count + kFactorWaves - 1) / kFactorWaves;
+    hipLaunchKernelGGL(iluFactorLevelKernel<FT>, dim3((unsigned)blocks), dim3(64 * kFactorWaves),
+                       0, s, f, off, diag, dilu ? 1 : 0, levelOff[L], (int)count);
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(invertDiagKernel<FT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       diag, n);
+  GPH_HIP(hipGetLastError());
+}
+
+void launchIluFactor(const FactorArgs& f, const std::vector<int64_t>& levelOff, int64_t n,
+                     void* off, void* diag, DType ft, bool dilu, hipStream_t s) {
+  if (n > (int64_t)INT32_MAX) fail(kUnsupported, "ILU factorization: too many rows");
   if (ft == DType::F32)
-    hipLaunchKernelGGL(iluFactorKernel<float>, dim3(t.T), dim3(kTileThreads),
-                       0, s, t, static_cast<float*>(off),
-                       static_cast<float*>(diag), dilu ? 1 : 0);
+    factorLevels(f, levelOff, n, static_cast<float*>(off), static_cast<float*>(diag), dilu, s);
   else if (ft == DType::F64)
-    hipLaunchKernelGGL(iluFactorKernel<double>, dim3(t.T), dim3(kTileThreads),
-                       0, s, t, static_cast<double*>(off),
-                       static_cast<double*>(diag), dilu ? 1 : 0);
+    factorLevels(f, levelOff, n, static_cast<double*>(off), static_cast<double*>(diag), dilu, s);
   else
     fail(kUnsupported, "ILU factorization type must be float32 or float64");
-  GPH_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

@@ -135,10 +135,20 @@ struct PreOp {
   const void* v = nullptr;
 };
 
+// Rows of the forward levels of all tiles, level by level: list[2r] = row,
+// list[2r+1] = its tile.
+struct FactorArgs {
+  const int32_t* tileStart = nullptr;
+  const int32_t* rowPtr = nullptr;
+  const int32_t* col = nullptr;
+  const int32_t* list = nullptr;
+};
 // In-place ILU(0) factorization of copies of off/diag (ft = f32 or f64),
 // diagonal inverted at the end. dilu: diagonal-based ILU, only diag changes.
-void launchIluFactor(const TileArgs& t, void* off, void* diag, DType ft,
-                     bool dilu, hipStream_t s);
+// levelOff[L]..levelOff[L+1]-1: the list entries of forward level L; n:
+// rows of the matrix.
+void launchIluFactor(const FactorArgs& f, const std::vector<int64_t>& levelOff, int64_t n,
+                     void* off, void* diag, DType ft, bool dilu, hipStream_t s);
 // out = (LU)^-1 rhs per tile; ft factor type, xt vector type. dilu: the
 // factors are A's off-diagonals and the DILU diagonal.
 void launchIluApply(const TileArgs& t, const void* lu, const void* invD,
@@ -242,6 +252,30 @@ size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt);
 void launchIluRows(const RowsArgs& p, const void* invD, DType ft, const void* rhs,
                    void* out, DType xt, const PreOp& pre, DevState* st,
                    int checkDone, hipStream_t s);
+// Workgroup-per-tile ILU(0) substitution (k_ilu_block.hip) for tiles of any
+// size: 1024 threads walk the tile's levels with a barrier per level, x in
+// LDS when it fits, the level-major factor stream (ilu_pack.hpp,
+// packIluBlock) prefetched several levels ahead into registers.
+constexpr int kBlockThreads = 1024;
+struct BlockDirArgs {
+  const int32_t* lvlBase = nullptr;  // T+1 first level of each tile
+  const int32_t* info = nullptr;     // per level {first slot, rows, first entry, width}
+  const void* rows = nullptr;        // per slot: (local row id, 1/d) record
+  const void* ents = nullptr;        // per entry: (local column, factor value) record
+};
+struct BlockArgs {
+  int32_t T = 0, maxRows = 0, width = 0, dilu = 0;
+  int32_t maxLevels = 0;  // most levels of one tile (both directions)
+  const int32_t* tileStart = nullptr;
+  BlockDirArgs fwd, bwd;
+};
+// LDS of the level tables of one tile
+__host__ __device__ inline size_t blockInfoBytes(int32_t maxLevels) {
+  return (size_t)maxLevels * 16;
+}
+size_t blockSharedBytes(const BlockArgs& a, DType xt);
+void launchIluBlock(const BlockArgs& a, DType ft, const void* rhs, void* out, DType xt,
+                    const PreOp& pre, DevState* st, int checkDone, hipStream_t s);
 // Batched solves over a block-diagonal union of systems (k_batch.hip):
 // chunks of rows, each inside one system; per-system solver states.
 constexpr int kBatchThreads = 256;

@@ -145,6 +145,10 @@ class IluSolver : public Solver {
                            st ? 1 : 0, s);
       return;
     }
+    if (block_) {
+      gph::launchIluBlock(bargs_, ft, rhs.ptr, out.ptr, out.type, pre, st, st ? 1 : 0, s);
+      return;
+    }
     gph::launchIluApply(A_.tileArgs(), lu, inv, ft, rhs.ptr, out.ptr, out.type,
                         pre, st, st ? 1 : 0, cfg_->diagonalBased, s);
   }
@@ -202,7 +206,8 @@ class IluSolver : public Solver {
   size_t factorBytes() const {
     return lu_.bytes() + invD_.bytes() + lu16_.bytes() +
            fcol_.bytes() + fval_.bytes() + bcol_.bytes() + bval_.bytes() + lbytes_.bytes() +
-           rcol_.bytes() + rval_.bytes() + rmeta_.bytes();
+           rcol_.bytes() + rval_.bytes() + rmeta_.bytes() + blk_[2].bytes() + blk_[3].bytes() +
+           blk_[6].bytes() + blk_[7].bytes();
   }
 
  private:
@@ -214,16 +219,35 @@ class IluSolver : public Solver {
     invD_.alloc((size_t)n * gph::dtypeSize(work));
     gph::launchCast(A_.crsOff(DType::F32), DType::F32, lu_.get(), work, nnz, s);
     gph::launchCast(A_.crsDiag(DType::F32), DType::F32, invD_.get(), work, n, s);
-    gph::launchIluFactor(A_.tileArgs(), lu_.get(), invD_.get(), work,
-                         cfg_->diagonalBased, s);
+    std::vector<int32_t> list;
+    std::vector<int64_t> levelOff;
+    host::forwardLevelRows(A_.hostMatrix(), list, levelOff);
+    gph::DevBuf listD;
+    listD.upload(list.data(), std::max<size_t>(list.size(), 1), s);
+    gph::FactorArgs fa;
+    fa.tileStart = A_.tileArgs().tileStart;
+    fa.rowPtr = A_.crsRowPtr();
+    fa.col = A_.crsCol();
+    fa.list = listD.as<int32_t>();
+    gph::launchIluFactor(fa, levelOff, n, lu_.get(), invD_.get(), work, cfg_->diagonalBased, s);
     if (ft_ == DType::F16) {
       lu16_.alloc((size_t)std::max<int64_t>(nnz, 1) * 2);
       gph::launchCast(lu_.get(), work, lu16_.get(), DType::F16, nnz, s);
     }
     if (cfg_->substitution == "level") buildLevel(s);
     if (cfg_->substitution == "rows") buildRows(s);
+    if (cfg_->substitution == "block") buildBlock(s);
     // also the fallback of the other kernels
-    if (!level_ && !rows_ && cfg_->substitution != "tile") buildPacked(s);
+    if (!level_ && !rows_ && !block_ && cfg_->substitution != "tile") buildPacked(s);
+    // tiles too large for the one-wave kernels: the workgroup-per-tile kernel
+    if (cfg_->substitution != "tile" && !block_) {
+      const DType xt = ft_ == DType::F64 ? DType::F64 : DType::F32;
+      const bool fits =
+          (rows_ && gph::rowsSharedBytes(rargs_, ft_, xt) <= kMaxPackedShared) ||
+          (level_ && gph::levelSharedBytes(largs_, xt) <= kMaxPackedShared) ||
+          (packed_ && gph::packedSharedBytes(pargs_, xt) <= kMaxPackedShared);
+      if (!fits) buildBlock(s);
+    }
     GPH_HIP(hipStreamSynchronize(s));
   }
 
@@ -300,6 +324,44 @@ class IluSolver : public Solver {
     rows_ = true;
   }
 
+  // Level-major stream of the factors for the workgroup-per-tile kernel
+  // (built on the host from the device factors).
+  void buildBlock(hipStream_t s) {
+    const int64_t nnz = A_.nnzOff();
+    const size_t es = gph::dtypeSize(ft_);
+    std::vector<uint8_t> lu((size_t)std::max<int64_t>(nnz, 1) * es);
+    const void* luD = ft_ == DType::F16 ? lu16_.get() : lu_.get();
+    GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipStreamSynchronize(s));
+    const size_t ies = ft_ == DType::F64 ? 8 : 4;  // InvDiag of the factor type
+    std::vector<uint8_t> inv((size_t)std::max<int64_t>(A_.n(), 1) * ies);
+    GPH_HIP(hipMemcpyAsync(inv.data(), invD_.get(), (size_t)A_.n() * ies, hipMemcpyDeviceToHost,
+                           s));
+    GPH_HIP(hipStreamSynchronize(s));
+    static_assert(kBlockSlots == gph::kBlockThreads, "block stream format");
+    IluBlockStream B = packIluBlock(A_.hostMatrix(), lu.data(), (int)es, inv.data(), (int)ies);
+    if (!B.ok || gph::blockInfoBytes(B.maxLevels) > kMaxPackedShared) return;
+    auto dir = [&](const IluBlockDirection& d, DevBuf* b, gph::BlockDirArgs& a) {
+      b[0].upload(d.lvlBase.data(), d.lvlBase.size(), s);
+      b[1].upload(d.info.data(), d.info.size(), s);
+      b[2].upload(d.rows.data(), d.rows.size(), s);
+      b[3].upload(d.ents.data(), d.ents.size(), s);
+      a.lvlBase = b[0].as<int32_t>();
+      a.info = b[1].as<int32_t>();
+      a.rows = b[2].get();
+      a.ents = b[3].get();
+    };
+    dir(B.fwd, blk_, bargs_.fwd);
+    dir(B.bwd, blk_ + 4, bargs_.bwd);
+    bargs_.maxLevels = B.maxLevels;
+    bargs_.T = A_.numTiles();
+    bargs_.tileStart = A_.tileArgs().tileStart;
+    bargs_.maxRows = B.maxRows;
+    bargs_.width = B.width;
+    bargs_.dilu = cfg_->diagonalBased ? 1 : 0;
+    block_ = true;
+  }
+
   // Level-major packed copy of the factors for the one-wave-per-tile kernel.
   void buildPacked(hipStream_t s) {
     PackedILU P = packILU(A_.hostMatrix(), std::max(1, cfg_->rowsPerWave));
@@ -354,7 +416,9 @@ class IluSolver : public Solver {
   std::shared_ptr<ilu::Configuration> cfg_;
   DType ft_;
   DevBuf lu_, invD_, lu16_;
-  bool packed_ = false, level_ = false, rows_ = false;
+  bool packed_ = false, level_ = false, rows_ = false, block_ = false;
+  gph::BlockArgs bargs_;
+  DevBuf blk_[8];
   gph::RowsArgs rargs_;
   DevBuf rcol_, rval_, relem_, rhoff_, rhdr_, rmeta_, rmoff_, tileSys_;
   gph::PackedArgs pargs_;

@@ -145,12 +145,12 @@ def _sub(sub):
     return {"substitution": sub}
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level", "tile"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level", "block", "tile"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_ilu0_apply_bit_exact(case, ftype, sub):
     """ILU(0) factorization + substitution equal the oracle's, with each
-    substitution kernel (product slabs, lane-per-row level records, row
-    blocks)."""
+    substitution kernel (product slabs, lane-per-row level records,
+    workgroup per tile over the level stream, row blocks)."""
     name, h, L, A = case
     dt = np.float32 if ftype == "float32" else np.float64
     S = g.Solver(A, {"type": "ILU", "factorType": ftype, **_sub(sub)})
@@ -163,7 +163,7 @@ def test_ilu0_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level", "block"])
 @pytest.mark.parametrize("ftype", ["float32", "float64"])
 def test_dilu_apply_bit_exact(case, ftype, sub):
     """DILU (diagonalBased) factorization + substitution equal the oracle's."""
@@ -180,7 +180,7 @@ def test_dilu_apply_bit_exact(case, ftype, sub):
     assert np.array_equal(x.numpy(), ref), name
 
 
-@pytest.mark.parametrize("sub", ["packed", "rows", "level"])
+@pytest.mark.parametrize("sub", ["packed", "rows", "level", "block"])
 def test_ilu0_fp16_factors_bit_exact(case, sub):
     """FP16-stored factors: the FP32 off-diagonal factors rounded to FP16,
     the FP32 inverse diagonal, applied in FP32 (the level-record kernel has
@@ -194,6 +194,39 @@ def test_ilu0_fp16_factors_bit_exact(case, sub):
     lu16 = lu.astype(np.float16).astype(np.float32)
     ref = o.ilu0_apply(L, lu16, d, b)
     assert np.array_equal(x.numpy(), ref), name
+
+
+@pytest.fixture(scope="module", params=[32, 48])
+def one_tile(request):
+    """Poisson-7 on an m^3 grid as one tile (32^3: 32,768 rows, x fits in
+    LDS in float32; 48^3: 110,592 rows, levels of more than 1,024 rows and x
+    in global memory)."""
+    m = request.param
+    h, L = poisson_case(m, m, m)
+    assert h.num_tiles == 1
+    return m, h, L, h.copyToTile()
+
+
+@pytest.mark.parametrize("variant", ["float32", "float64", "dilu", "float16"])
+def test_ilu0_one_large_tile_bit_exact(one_tile, variant):
+    """A single tile too large for the one-wave kernels runs the
+    workgroup-per-tile kernel by default; factors and substitution equal
+    the oracle's."""
+    m, h, L, A = one_tile
+    ft = "float32" if variant in ("dilu", "float16") else variant
+    dt = np.float64 if ft == "float64" else np.float32
+    cfg = {"type": "ILU", "factorType": "float16" if variant == "float16" else ft}
+    if variant == "dilu":
+        cfg["diagonalBased"] = True
+    S = g.Solver(A, cfg)
+    b = o.uniform_pm1(31, h.nrows).astype(dt)
+    x = dev(np.zeros(h.nrows, dt))
+    S.solve(x, dev(b))
+    lu, d = o.ilu0_factor(L, dt, diagonalBased=variant == "dilu")
+    if variant == "float16":
+        lu = lu.astype(np.float16).astype(np.float32)
+    ref = o.ilu0_apply(L, lu, d, b, diagonalBased=variant == "dilu")
+    assert np.array_equal(x.numpy(), ref), (m, variant)
 
 
 @pytest.mark.parametrize("symmetric", [False, True])
