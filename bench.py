@@ -167,6 +167,43 @@ def bench_batched(g, S=512, seed=3):
     return out
 
 
+def bench_one_tile(g, m=32, reps=5):
+    """Poisson-7 on a 32^3 grid as ONE tile (the whole matrix is the ILU(0)
+    block): PBiCGStab + ILU(0), FP32, relative residual 1e-6, from x = 0;
+    b = fl32(A x_true). The tile is too large for the one-wave substitution
+    kernels and runs the workgroup-per-tile kernel."""
+    h = g.generate3DPoissonMatrix(m, m, m)
+    A = h.copyToTile()
+    N = h.nrows
+    xt = h.to_local(g.uniform_pm1(0, N)).astype(np.float64)
+    b = g.DeviceVector.from_numpy(A.spmv(g.DeviceVector.from_numpy(xt)).numpy().astype(np.float32))
+    t = time.perf_counter()
+    S = g.Solver(A, {"type": "PBiCGStab", "relResidual": 1e-6, "maxIterations": 1000,
+                     "workingType": "float32",
+                     "preconditioner": {"type": "ILU", "factorType": "float32"}})
+    setup = time.perf_counter() - t
+    x = g.DeviceVector(N, np.float32)
+    times, st = [], None
+    for _ in range(reps):
+        x.fill(0)
+        t = time.perf_counter()
+        st = S.solve(x, b)
+        times.append(time.perf_counter() - t)
+    P = g.Solver(A, {"type": "ILU", "factorType": "float32"})
+    y = g.DeviceVector(N, np.float32)
+    apply_us = 1e3 * P.bench_apply(g.DeviceVector.from_numpy(xt.astype(np.float32)), y, iters=20)
+    r = A.residual(g.DeviceVector.from_numpy(x.numpy().astype(np.float64)), b,
+                   destType=g.FLOAT64, intermediateType=g.FLOAT64)
+    rel = float(np.linalg.norm(r.numpy()) / np.linalg.norm(b.numpy().astype(np.float64)))
+    out = {"workload": f"poisson7 {m}^3, 1 tile, PBiCGStab+ILU(0) f32, rel. residual 1e-6",
+           "rows": N, "tiles": h.num_tiles, "ms": 1e3 * float(np.median(times)),
+           "iterations": st.iterations, "converged": st.converged, "true_rel_residual": rel,
+           "ilu_apply_us": apply_us, "solver_setup_s": setup}
+    log(f"one-tile {m}^3: {out['ms']:.2f} ms per solve ({st.iterations} iterations), "
+        f"ILU apply {apply_us:.1f} us, true rel. residual {rel:.2e}")
+    return out
+
+
 def bench_powerlaw(g, n=4194304, tiles=4096):
     """Unstructured SpMV + solve: power-law rows (lengths 5..64, exponent
     1.85, 90 % of columns within +-2000), seed 2, contiguous tiles; CRS SpMV
@@ -577,6 +614,7 @@ def main():
         del L
     del e
 
+    one_tile = bench_one_tile(g)
     powerlaw = None if args.skip_powerlaw else bench_powerlaw(g)
     batched = None if args.skip_batched else bench_batched(g)
     serving = None if args.skip_serve else bench_serving(g, args.serve_requests)
@@ -601,6 +639,7 @@ def main():
         "roofline_other": other,
         "spmv": spmv,
         "ilu_apply": ilu,
+        "one_tile": one_tile,
         "powerlaw": powerlaw,
         "batched": batched,
         "serving": serving,

@@ -26,6 +26,7 @@ SOURCES = [
     "server.cpp",
     "batch.cpp",
     "ilu_pack.cpp",
+    "gs_pack.cpp",
     "k_spmv.hip",
     "k_vec.hip",
     "k_precond.hip",
@@ -36,6 +37,7 @@ SOURCES = [
     "k_noalloc.hip",
     "k_multi.hip",
     "k_ilu_block.hip",
+    "k_gs.hip",
 ]
 
 COMMON = [

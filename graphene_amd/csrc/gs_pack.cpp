@@ -1,0 +1,69 @@
+// Jagged-diagonal level stream for the Gauss-Seidel sweep (gs_pack.hpp).
+#include "gs_pack.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace graphene::matrix {
+
+GsStream packGs(const host::HostMatrix& m, bool backward) {
+  GsStream o;
+  const int32_t T = m.numTiles;
+  o.lvlBase.assign((size_t)T + 1, 0);
+  for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
+  if (o.maxRows >= (1 << 24)) return o;
+  o.code.reserve((size_t)m.nnzOff());
+  o.src.reserve((size_t)m.nnzOff());
+  std::vector<int32_t> order;
+  for (int32_t t = 0; t < T; t++) {
+    o.lvlBase[(size_t)t] = (int32_t)(o.lvl.size() / 4);
+    const int64_t t0 = m.tileStart[(size_t)t], nt = m.tileRows(t);
+    const int64_t* cs = backward ? m.upperColorStart.data() + m.upperColorStartOffset[(size_t)t]
+                                 : m.colorStart.data() + m.colorStartOffset[(size_t)t];
+    const int32_t nl = backward ? m.numUpperColors[(size_t)t] : m.numColors[(size_t)t];
+    const int32_t* rows = backward ? m.upperLevelRows.data() : m.colorSortAddr.data();
+    auto len = [&](int32_t li) {
+      return m.rowPtr[(size_t)(t0 + li) + 1] - m.rowPtr[(size_t)(t0 + li)];
+    };
+    for (int32_t L = 0; L < nl; L++) {
+      order.assign(rows + t0 + cs[L], rows + t0 + cs[L + 1]);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int32_t a, int32_t b) { return len(a) > len(b); });
+      const int64_t R = (int64_t)order.size();
+      const int64_t W = R ? len(order[0]) : 0;
+      if (W > 255) return o;
+      if ((int64_t)o.slot.size() + R > INT32_MAX ||
+          (int64_t)o.code.size() + W * R > INT32_MAX)
+        return o;
+      o.lvl.insert(o.lvl.end(), {(int32_t)o.slot.size(), (int32_t)R, (int32_t)o.diag.size(),
+                                 (int32_t)W});
+      for (int64_t q = 0; q < R; q++)
+        o.slot.push_back(((uint32_t)order[(size_t)q] << 8) | (uint32_t)len(order[(size_t)q]));
+      for (int64_t j = 0; j < W; j++) {
+        o.diag.push_back((int32_t)o.code.size());
+        for (int64_t q = 0; q < R; q++) {
+          const int32_t li = order[(size_t)q];
+          if (len(li) <= j) break;  // sorted: the rows after are shorter too
+          const int64_t a = m.rowPtr[(size_t)(t0 + li)] + j;
+          const int32_t cl = m.colLocal[(size_t)a];
+          const bool fresh = cl < nt && (backward ? cl > li : cl < li);
+          o.code.push_back(fresh ? (kGsFresh | (uint32_t)cl) : (uint32_t)m.col[(size_t)a]);
+          o.src.push_back((int32_t)a);
+        }
+      }
+      o.diag.push_back((int32_t)o.code.size());
+    }
+  }
+  o.lvlBase[(size_t)T] = (int32_t)(o.lvl.size() / 4);
+  if (o.lvl.empty()) o.lvl.assign(4, 0);
+  if (o.diag.empty()) o.diag.assign(1, 0);
+  if (o.slot.empty()) o.slot.assign(1, 0);
+  if (o.code.empty()) {
+    o.code.assign(1, kGsFresh);
+    o.src.assign(1, -1);
+  }
+  o.ok = true;
+  return o;
+}
+
+}  // namespace graphene::matrix

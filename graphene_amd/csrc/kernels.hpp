@@ -276,6 +276,24 @@ __host__ __device__ inline size_t blockInfoBytes(int32_t maxLevels) {
 size_t blockSharedBytes(const BlockArgs& a, DType xt);
 void launchIluBlock(const BlockArgs& a, DType ft, const void* rhs, void* out, DType xt,
                     const PreOp& pre, DevState* st, int checkDone, hipStream_t s);
+// Gauss-Seidel sweep over the jagged-diagonal level stream (k_gs.hip,
+// stream format in gs_pack.hpp): a workgroup per tile, a wave per 64 rows
+// of a level, the sweep's x in LDS.
+constexpr int kGsThreads = 256;
+constexpr uint32_t kGsFreshBit = 0x80000000u;
+struct GsArgs {
+  int32_t T = 0, maxRows = 0;
+  const int32_t* tileStart = nullptr;
+  const int32_t* lvlBase = nullptr;    // T+1
+  const int32_t* lvl = nullptr;        // per level: first slot, rows, first diagonal, width
+  const int32_t* diagStart = nullptr;  // per level: width + 1 diagonal starts
+  const uint32_t* slot = nullptr;      // per slot: tile-local row << 8 | length
+  const uint32_t* code = nullptr;      // per entry: x source
+  const float* val = nullptr;          // per entry: matrix value
+};
+size_t gsStreamSharedBytes(const GsArgs& a, DType xt);
+void launchGsStream(const GsArgs& a, const void* diag, const void* b, const void* xin,
+                    void* xout, DType xt, DevState* st, int checkDone, hipStream_t s);
 // Batched solves over a block-diagonal union of systems (k_batch.hip):
 // chunks of rows, each inside one system; per-system solver states.
 constexpr int kBatchThreads = 256;

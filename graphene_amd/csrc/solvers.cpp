@@ -1,6 +1,7 @@
 // Solver tree implementation (see solvers.hpp).
 #include "solvers.hpp"
 
+#include "gs_pack.hpp"
 #include "ilu_pack.hpp"
 
 #include <chrono>
@@ -435,7 +436,10 @@ class GaussSeidelSolver : public Solver {
  public:
   GaussSeidelSolver(const DeviceMatrix& A,
                     std::shared_ptr<gaussseidel::Configuration> c)
-      : Solver(A), cfg_(std::move(c)) {}
+      : Solver(A), cfg_(std::move(c)) {
+    buildStream(false, fwd_, fbuf_);
+    if (cfg_->symmetric) buildStream(true, bwd_, bbuf_);
+  }
   std::string name() const override { return cfg_->solverName(); }
   bool usesInitialGuess() const override { return true; }
   bool supportsDeviceApply() const override { return true; }
@@ -448,12 +452,17 @@ class GaussSeidelSolver : public Solver {
     const void* off = A_.crsOff(DType::F32);
     const void* dg = A_.crsDiag(DType::F32);
     const int chk = st ? 1 : 0;
+    auto sweep = [&](bool bwd, const void* xin, void* xout) {
+      const gph::GsArgs& g = bwd ? bwd_ : fwd_;
+      if (g.T > 0 && gph::gsStreamSharedBytes(g, x.type) <= kMaxGsShared)
+        gph::launchGsStream(g, dg, b.ptr, xin, xout, x.type, st, chk, s);
+      else
+        gph::launchGsSweep(tiles, off, dg, DType::F32, b.ptr, xin, xout, x.type, bwd, st, chk, s);
+    };
     for (int k = 0; k < count; k++) {
-      gph::launchGsSweep(tiles, off, dg, DType::F32, b.ptr, x.ptr, tmp.ptr,
-                         x.type, false, st, chk, s);
+      sweep(false, x.ptr, tmp.ptr);
       if (cfg_->symmetric) {
-        gph::launchGsSweep(tiles, off, dg, DType::F32, b.ptr, tmp.ptr, x.ptr,
-                           x.type, true, st, chk, s);
+        sweep(true, tmp.ptr, x.ptr);
       } else {
         gph::launchCast(tmp.ptr, x.type, x.ptr, x.type, A_.n(), s);
       }
@@ -523,8 +532,42 @@ class GaussSeidelSolver : public Solver {
   }
 
  private:
+  // Jagged-diagonal level stream of one sweep direction (gs_pack.hpp),
+  // values gathered from the device's float32 matrix values; none when a
+  // tile's x does not fit in LDS (the row-block sweep kernel runs then).
+  void buildStream(bool backward, gph::GsArgs& a, DevBuf* buf) {
+    if ((size_t)(A_.tileArgs().maxTileRows + 1) * 8 > kMaxGsShared) return;
+    static_assert(kGsFresh == gph::kGsFreshBit, "stream format");
+    GsStream g = packGs(A_.hostMatrix(), backward);
+    if (!g.ok) return;
+    hipStream_t s = A_.stream();
+    buf[0].upload(g.lvlBase.data(), g.lvlBase.size(), s);
+    buf[1].upload(g.lvl.data(), g.lvl.size(), s);
+    buf[2].upload(g.diag.data(), g.diag.size(), s);
+    buf[3].upload(g.slot.data(), g.slot.size(), s);
+    buf[4].upload(g.code.data(), g.code.size(), s);
+    DevBuf idx;
+    idx.upload(g.src.data(), g.src.size(), s);
+    buf[5].alloc(g.src.size() * sizeof(float));
+    gph::launchGather(A_.crsOff(DType::F32), DType::F32, idx.as<int32_t>(), buf[5].get(),
+                      (int64_t)g.src.size(), s);
+    GPH_HIP(hipStreamSynchronize(s));
+    a.T = A_.numTiles();
+    a.maxRows = g.maxRows;
+    a.tileStart = A_.tileArgs().tileStart;
+    a.lvlBase = buf[0].as<int32_t>();
+    a.lvl = buf[1].as<int32_t>();
+    a.diagStart = buf[2].as<int32_t>();
+    a.slot = buf[3].as<uint32_t>();
+    a.code = buf[4].as<uint32_t>();
+    a.val = buf[5].as<float>();
+  }
+
+  static constexpr size_t kMaxGsShared = 156 * 1024;
   std::shared_ptr<gaussseidel::Configuration> cfg_;
   Workspace ws_;
+  gph::GsArgs fwd_, bwd_;
+  DevBuf fbuf_[6], bbuf_[6];
 };
 
 // ---------------------------------------------------------------------------
