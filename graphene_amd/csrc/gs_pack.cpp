@@ -52,6 +52,8 @@ GsStream packGs(const host::HostMatrix& m, bool backward) {
         }
       }
       o.diag.push_back((int32_t)o.code.size());
+      o.maxLevelEntries = std::max<int32_t>(
+          o.maxLevelEntries, o.diag.back() - o.diag[(size_t)o.lvl[o.lvl.size() - 2]]);
     }
   }
   o.lvlBase[(size_t)T] = (int32_t)(o.lvl.size() / 4);

@@ -32,9 +32,10 @@ struct GsStream {
   std::vector<int32_t> lvl;      // per level: first slot, rows, first diagonal, width
   std::vector<int32_t> diag;     // per level: width + 1 diagonal starts
   std::vector<uint32_t> slot;    // per slot: local row << 8 | length
-  std::vector<uint32_t> code;    // per entry
+  std::vector<uint32_t> code;    // per entry: x source
   std::vector<int32_t> src;      // per entry: CRS position of its value
   int32_t maxRows = 0;           // largest tile
+  int32_t maxLevelEntries = 0;   // entries of the largest level
   bool ok = false;  // false: rows longer than 255 or tiles over 2^24 rows, or over 32-bit offsets
 };
 

@@ -5,22 +5,23 @@
 // per row x_i = (b_i - sum_j a_ij x_j) / a_ii over the row's entries in
 // stored order, x_j the value this sweep has already written for in-tile
 // j < i (forward) or j > i (backward), the previous x otherwise (halo
-// included). One workgroup per tile walks the tile's levels with a barrier
-// per level; its waves take groups of 64 of the level's rows (sorted by
-// length; stream format in gs_pack.hpp), lane = row, so the entries a wave
-// loads for one entry index are adjacent, and a group runs as many entry
-// steps as its longest row. A row runs in steps of 16 entries whose codes
-// and values are loaded one step ahead, so a step waits on one round of x
-// reads. The sweep's own x values sit in LDS; the result is written to xout
-// at the end (xout != xin).
+// included). One workgroup per tile, the values this sweep writes in LDS.
+// Per level (stream format in gs_pack.hpp) two phases: every thread takes
+// consecutive entries of the level's stream and stores its product
+// a_ij * x_j in LDS (x_j from LDS or the previous x; the stream loads are
+// coalesced and no lane idles on a short row); then one thread per row
+// subtracts its products from b_i in the row's stored order and writes
+// x_i. The result is written to xout at the end (xout != xin).
 #include "kernels.hpp"
 
 namespace gph {
 
 static size_t gsMaxShared() { return 160 * 1024 - 4096; }
 
+// LDS: the x values this sweep writes, the level's products, the level's
+// diagonal starts (kGsMaxWidth + 1 ints)
 size_t gsStreamSharedBytes(const GsArgs& a, DType xt) {
-  return (size_t)(a.maxRows + 1) * dtypeSize(xt);
+  return (size_t)(a.maxRows + a.maxLevelEntries) * dtypeSize(xt) + (kGsMaxWidth + 1) * 4;
 }
 
 template <typename X>
@@ -33,67 +34,71 @@ __global__ __launch_bounds__(kGsThreads) void gsChunkKernel(GsArgs a,
   extern __shared__ __align__(16) unsigned char smemRaw[];
   if (checkDone && st->done) return;
   X* xs = reinterpret_cast<X*>(smemRaw);
+  X* prod = xs + a.maxRows;
+  int* doff = reinterpret_cast<int*>(prod + a.maxLevelEntries);
   // consecutive tiles on one XCD: their x windows overlap in its L2
   const int tile = xcdRemap(blockIdx.x, gridDim.x);
   const int t0 = a.tileStart[tile], n = a.tileStart[tile + 1] - t0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int nw = kGsThreads / 64;
+  const int tid = threadIdx.x;
   const float* __restrict__ val = a.val;
   const uint32_t* __restrict__ code = a.code;
   const int32_t* __restrict__ dg = a.diagStart;
   const int l0 = a.lvlBase[tile], nL = a.lvlBase[tile + 1] - l0;
-  auto xOf = [&](uint32_t c) { return (c & kGsFreshBit) ? xs[c & ~kGsFreshBit] : xin[c]; };
   for (int L = 0; L < nL; L++) {
     const int4 lv = reinterpret_cast<const int4*>(a.lvl)[l0 + L];
-    for (int g0 = 64 * wave; g0 < lv.y; g0 += 64 * nw) {
-      const bool own = g0 + lane < lv.y;
-      const uint32_t sl = a.slot[lv.x + g0 + (own ? lane : 0)];
-      const int len = own ? (int)(sl & 255u) : 0;
-      const int r = (int)(sl >> 8);
-      const int W = __builtin_amdgcn_readfirstlane(len);  // the group's longest row
-      const int i = t0 + r;
-      X s = b[i];
-      // entry j of the lane's row: diagonal j's start + the row's slot;
-      // lanes whose row is shorter (or past the level) read the diagonal's
-      // first entry and do not subtract it
-      constexpr int S = 16;
-      uint32_t cA[S], cB[S];
-      float vA[S], vB[S];
-      auto ld = [&](int j0, uint32_t (&cd)[S], float (&v)[S]) {
+    const int e0 = dg[lv.z], e1 = dg[lv.z + lv.w];
+    // the first row of this thread (loads independent of the products)
+    uint32_t sl = 0;
+    X bi = 0, di = 1;
+    if (tid < lv.y) {
+      sl = a.slot[lv.x + tid];
+      const int i = t0 + (int)(sl >> 8);
+      bi = b[i];
+      di = cvt<X>(diag[i]);
+    }
+    if (tid <= lv.w) doff[tid] = dg[lv.z + tid] - e0;
+    // 4 entries per thread and step: the code and value loads, the x
+    // reads, then the products (indices past the level read its last entry)
+    for (int e = e0 + tid; e < e1; e += 4 * kGsThreads) {
+      uint32_t c[4];
+      float v[4];
 #pragma unroll
-        for (int k = 0; k < S; k++) {
-          const int j = j0 + k;
-          const uint32_t d = (uint32_t)dg[lv.z + min(j, W - 1)];
-          const uint32_t e = j < len ? d + (uint32_t)(g0 + lane) : d;
-          cd[k] = code[e];
-          v[k] = val[e];
-        }
-      };
-      auto step = [&](int j0, const uint32_t (&cd)[S], const float (&v)[S]) {
-        X xv[S];
-#pragma unroll
-        for (int k = 0; k < S; k++) xv[k] = xOf(cd[k]);
-#pragma unroll
-        for (int k = 0; k < S; k++) {
-          const X p = cvt<X>(v[k]) * xv[k];
-          s = j0 + k < len ? s - p : s;
-        }
-      };
-      if (W > 0) {
-        ld(0, cA, vA);
-        for (int j = 0; j < W; j += 2 * S) {
-          ld(j + S, cB, vB);
-          step(j, cA, vA);
-          if (j + S >= W) break;
-          ld(j + 2 * S, cA, vA);
-          step(j + S, cB, vB);
-        }
+      for (int k = 0; k < 4; k++) {
+        const int ek = min(e + k * kGsThreads, e1 - 1);
+        c[k] = code[ek];
+        v[k] = val[ek];
       }
-      if (own) xs[r] = s / cvt<X>(diag[i]);
+      X xv[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        xv[k] = (c[k] & kGsFreshBit) ? xs[c[k] & ~kGsFreshBit] : xin[c[k]];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (e + k * kGsThreads < e1) prod[e + k * kGsThreads - e0] = cvt<X>(v[k]) * xv[k];
+    }
+    __syncthreads();
+    for (int q = tid; q < lv.y; q += kGsThreads) {
+      if (q != tid) {
+        sl = a.slot[lv.x + q];
+        const int i = t0 + (int)(sl >> 8);
+        bi = b[i];
+        di = cvt<X>(diag[i]);
+      }
+      const int len = (int)(sl & 255u);
+      X s = bi;
+      // 8 products per step, read together, subtracted in order
+      for (int j = 0; j < len; j += 8) {
+        X p[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[k] = prod[doff[min(j + k, len - 1)] + q];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s = j + k < len ? s - p[k] : s;
+      }
+      xs[sl >> 8] = s / di;
     }
     __syncthreads();
   }
-  for (int q = threadIdx.x; q < n; q += kGsThreads) xout[t0 + q] = xs[q];
+  for (int q = tid; q < n; q += kGsThreads) xout[t0 + q] = xs[q];
 }
 
 void launchGsStream(const GsArgs& a, const void* diag, const void* b, const void* xin,

@@ -277,12 +277,14 @@ size_t blockSharedBytes(const BlockArgs& a, DType xt);
 void launchIluBlock(const BlockArgs& a, DType ft, const void* rhs, void* out, DType xt,
                     const PreOp& pre, DevState* st, int checkDone, hipStream_t s);
 // Gauss-Seidel sweep over the jagged-diagonal level stream (k_gs.hip,
-// stream format in gs_pack.hpp): a workgroup per tile, a wave per 64 rows
-// of a level, the sweep's x in LDS.
+// stream format in gs_pack.hpp): a workgroup per tile, the sweep's x in
+// LDS, per level the products entry-parallel, then the row sums.
 constexpr int kGsThreads = 256;
+constexpr int kGsMaxWidth = 255;  // longest row of the stream
 constexpr uint32_t kGsFreshBit = 0x80000000u;
 struct GsArgs {
   int32_t T = 0, maxRows = 0;
+  int32_t maxLevelEntries = 0;         // entries of the largest level
   const int32_t* tileStart = nullptr;
   const int32_t* lvlBase = nullptr;    // T+1
   const int32_t* lvl = nullptr;        // per level: first slot, rows, first diagonal, width

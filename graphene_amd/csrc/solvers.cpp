@@ -534,7 +534,8 @@ class GaussSeidelSolver : public Solver {
  private:
   // Jagged-diagonal level stream of one sweep direction (gs_pack.hpp),
   // values gathered from the device's float32 matrix values; none when a
-  // tile's x does not fit in LDS (the row-block sweep kernel runs then).
+  // tile's x and a level's products do not fit in LDS (the row-block sweep
+  // kernel runs then).
   void buildStream(bool backward, gph::GsArgs& a, DevBuf* buf) {
     if ((size_t)(A_.tileArgs().maxTileRows + 1) * 8 > kMaxGsShared) return;
     static_assert(kGsFresh == gph::kGsFreshBit, "stream format");
@@ -561,6 +562,9 @@ class GaussSeidelSolver : public Solver {
     a.slot = buf[3].as<uint32_t>();
     a.code = buf[4].as<uint32_t>();
     a.val = buf[5].as<float>();
+    a.maxLevelEntries = g.maxLevelEntries;
+    static_assert(gph::kGsMaxWidth == 255, "stream row length field");
+    if (gph::gsStreamSharedBytes(a, DType::F32) > kMaxGsShared) a.T = 0;  // row-block kernel
   }
 
   static constexpr size_t kMaxGsShared = 156 * 1024;
