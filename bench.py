@@ -207,7 +207,8 @@ def bench_one_tile(g, m=32, reps=5):
 def bench_powerlaw(g, n=4194304, tiles=4096):
     """Unstructured SpMV + solve: power-law rows (lengths 5..64, exponent
     1.85, 90 % of columns within +-2000), seed 2, contiguous tiles; CRS SpMV
-    throughput, then PBiCGStab with symmetric Gauss-Seidel (2 sweeps) to
+    and Gauss-Seidel sweep throughput (algorithmic bytes 8 nnz_off + 16 N
+    each), then PBiCGStab with symmetric Gauss-Seidel (2 sweeps) to
     relative residual 1e-8 in FP32, and the same FP32 inner solve under FP64
     refinement to a true relative residual of 1e-8; forward errors against
     the FP64 solve to 1e-14."""
@@ -253,7 +254,16 @@ def bench_powerlaw(g, n=4194304, tiles=4096):
             "extendedPrecisionType": "float64", "workingType": "float32",
             "innerSolver": dict(cfg, relResidual=1e-3, maxIterations=200)}
     msm, stm, relm, xm = solve(mcfg, np.float64)
+    # one symmetric Gauss-Seidel iteration as a preconditioner application:
+    # x = 0, a forward and a backward sweep (the x = 0 fill is a few us)
+    G = g.Solver(A, {"type": "GaussSeidel", "numFixedIterations": 1, "symmetric": True})
+    sgs_us = 1e3 * G.bench_apply(x, y, iters=20)
+    del G
+    sweep_bytes = 8 * h.nnz_off + 16 * n
     out = {"rows": n, "nnz_off": h.nnz_off, "tiles": tiles,
+           "gs_sweep_us": sgs_us / 2, "gs_sweep_GBs": sweep_bytes / (sgs_us / 2) / 1e3,
+           "gs_sweep_frac": sweep_bytes / (sgs_us / 2) / 1e3 / HBM_PEAK_GBS,
+           "gs_sweep_algorithmic_bytes": sweep_bytes,
            "spmv_us": ms * 1e3, "spmv_GBs": byt / ms / 1e6,
            "spmv_frac": byt / ms / 1e6 / HBM_PEAK_GBS, "spmv_algorithmic_bytes": byt,
            "solve_ms": ms32, "iterations": st.iterations,
@@ -263,6 +273,7 @@ def bench_powerlaw(g, n=4194304, tiles=4096):
                     "forward_error": fwd(xm)},
            "forward_error_reference": {"iterations": st_ref.iterations,
                                        "true_rel_residual": rel_ref}}
+    log(f"power-law Gauss-Seidel sweep {sgs_us / 2:.1f} us ({out['gs_sweep_GBs']:.0f} GB/s)")
     log(f"power-law SpMV {ms*1e3:.1f} us ({out['spmv_GBs']:.0f} GB/s), "
         f"PBiCGStab+SGS(2): {ms32:.1f} ms, {st}, true rel. residual {rel:.2e}; "
         f"under MPIR: {msm:.1f} ms, {stm}, true rel. residual {relm:.2e}")
@@ -408,8 +419,8 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
-    ap.add_argument("--only-serve", action="store_true",
-                    help="the serving leg alone")
+    ap.add_argument("--only", choices=["serve", "powerlaw", "one-tile", "batched"],
+                    help="run one secondary leg alone and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -420,8 +431,11 @@ def main():
 
     if g.device_count() < 1:
         raise SystemExit("no GPU visible")
-    if args.only_serve:
-        print(json.dumps({"serving": bench_serving(g, args.serve_requests)}))
+    if args.only:
+        legs = {"serve": lambda: bench_serving(g, args.serve_requests),
+                "powerlaw": lambda: bench_powerlaw(g), "one-tile": lambda: bench_one_tile(g),
+                "batched": lambda: bench_batched(g)}
+        print(json.dumps({args.only: legs[args.only]()}))
         return
     n = args.n
     N = n ** 3
@@ -470,7 +484,7 @@ def main():
     rl_spmv = roofline(sp["GBs"], spmv_bytes, f"spmv-{args.format}",
                        f"sellKernel<float> SpMV ({args.format.upper()})")
     rl_ilu = roofline(ilu["GBs"], ilu_bytes, "ilu-apply",
-                      "iluRowsKernel<float,float,1> (ILU(0) fwd+bwd substitution)")
+                      "iluRowsKernel<float,float> (ILU(0) fwd+bwd substitution)")
     if args.kernels_only:
         cal = copy_calibration(g)
         print(json.dumps({"spmv": spmv, "ilu": ilu, "copy": cal,
