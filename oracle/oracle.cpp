@@ -394,7 +394,7 @@ ORC_API void orc_residual_mixed(int64_t n, const int64_t* rowPtr, const int64_t*
 
 // ---------------------------------------------------------------------------
 // LDU addressing built from a structurally symmetric CRS (no reference
-// builder exists, SURVEY Q4; follows lg/matrix/details/ldu/LDUAddressing.hpp:30-63):
+// builder exists; follows lg/matrix/details/ldu/LDUAddressing.hpp:30-63):
 // faces enumerate (l, u) with l < u in row-major order of l, then u ascending;
 // lowerAddr[f] = l (owner), upperAddr[f] = u (neighbour), upper[f] = A[l][u];
 // ownerStart[c] = first face owned by c; losort lists faces by neighbour
@@ -639,7 +639,7 @@ ORC_API void orc_dilu_apply_f64(int32_t T, const int64_t* tileStart,
 // before the sweep, one sequential pass per tile over all stored columns,
 // halo included). Arithmetic: codelet lg/codelet/ipu/matrix/solvers/gauss-seidel.cpp:41-61
 // (t = b_i; t -= off*x_col; x_i = t / diag_i). The reference's CodeDSL kernel
-// seeds t with diag_i and ignores b (SURVEY Q2); this restates the codelet.
+// seeds t with diag_i and ignores b; this restates the codelet.
 // backward != 0 runs the rows in reverse local order (symmetric GS; no
 // reference counterpart). Halo values are the values before the sweep.
 // ---------------------------------------------------------------------------

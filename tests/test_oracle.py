@@ -38,7 +38,7 @@ def test_poisson_closed_form(dims):
     nnz = nx * ny * nz + 2 * ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1))
     assert len(v) == nnz
     if dims == (32, 32, 32):
-        assert nnz == 223232
+        assert nnz == 223232  # the 32^3 Poisson benchmark matrix
 
     def lap1(m):
         return sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
