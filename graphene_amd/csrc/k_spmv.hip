@@ -233,25 +233,30 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
         const int cbase = a.sliceBase[slice];
         const uint16_t* __restrict__ cw = a.scol + a.sliceColW[slice];
         // 8 entries per step: 16 independent stream loads, then 8 gathers,
-        // then the adds in stored order
+        // then the adds in stored order; the rest of the row in steps of 4,
+        // 2 and 1 (one round of loads each)
         auto rowSum = [&](auto* __restrict__ col, int cb) {
           int k = 0;
-          for (; k + 8 <= len; k += 8) {
-            int c[8];
-            VT v[8];
+          auto batch = [&](auto nTag) {
+            constexpr int B = decltype(nTag)::value;
+            int c[B];
+            VT v[B];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+            for (int j = 0; j < B; j++) {
               c[j] = cb + (int)col[(k + j) * 64];
               v[j] = val[(k + j) * 64];
             }
-            XT xv[8];
+            XT xv[B];
 #pragma unroll
-            for (int j = 0; j < 8; j++) xv[j] = x[c[j]];
+            for (int j = 0; j < B; j++) xv[j] = x[c[j]];
 #pragma unroll
-            for (int j = 0; j < 8; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
-          }
-          for (; k < len; k++)
-            sum = sum + cvt<AT>(val[k * 64]) * (AT)x[cb + (int)col[k * 64]];
+            for (int j = 0; j < B; j++) sum = sum + cvt<AT>(v[j]) * (AT)xv[j];
+            k += B;
+          };
+          while (k + 8 <= len) batch(std::integral_constant<int, 8>{});
+          if (k + 4 <= len) batch(std::integral_constant<int, 4>{});
+          if (k + 2 <= len) batch(std::integral_constant<int, 2>{});
+          if (k < len) batch(std::integral_constant<int, 1>{});
         };
         if (cbase >= 0)
           rowSum(cw + lane, cbase);  // 16-bit offsets from the slice base
