@@ -366,6 +366,7 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
   if (4 * (int64_t)(o.maxRows + 1) > 65535) return o;
   const uint16_t zero = 0;  // slot 0 holds +0; row i of a tile is slot i + 1
   const uint8_t* luB = static_cast<const uint8_t*>(lu);
+  const int64_t gran = es == 8 ? kRowsPad : 2;  // record length granule
   o.elemOff.assign((size_t)T + 1, 0);
   o.hdrOff.assign((size_t)T + 1, 0);
   o.metaOff.assign((size_t)T + 1, 0);
@@ -427,49 +428,37 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
             cols.push_back(c);
           }
           while (!cols.empty() && !cols.back().lng && cols.back().rec[0].r < 0) cols.pop_back();
-          // record words, then the chunks in lane order: lane l (lane row
-          // l >> 4, column l & 15) takes record word 2 (l & 15) + (l >> 5)
-          // and, of a short record, chunk (l >> 4) & 1, of a long record
-          // chunk g(l >> 4) with g = 0, 1, 3, 2; lanes without a chunk
-          // store nothing
+          // records in record-word order, each contiguous and padded to an
+          // even length (float64 factors: to whole chunks, so the kernel
+          // needs no per-entry test); a long record's two words share its
+          // entries
           bool anyLong = false;
-          std::vector<const Rec*> word;
-          for (const Column& c : cols) {
-            for (int hh = 0; hh < 2; hh++) {
-              const Rec& rc = c.lng ? c.rec[0] : c.rec[hh];
-              word.push_back(&rc);
-              uint32_t w = 0;
-              if (rc.r >= 0) {
-                const int64_t chunks = (rc.l + kRowsPad - 1) / kRowsPad;
-                const int64_t slot = rows[(size_t)rc.r] - t0 + 1;
-                w = (uint32_t)chunks << 10 | (c.lng ? kRowsLong : 0u) | rc.flags |
-                    (uint32_t)slot << 16;
-              }
-              o.meta.push_back(w);
-            }
-            anyLong = anyLong || c.lng;
-          }
           int64_t elems = 0;
-          for (int l = 0; l < 64; l++) {
-            const int lr = l >> 4;
-            const size_t wi = (size_t)(2 * (l & 15) + (l >> 5));
-            if (wi >= word.size() || word[wi]->r < 0) continue;
-            const Rec& rc = *word[wi];
-            const bool lng = cols[wi / 2].lng;
-            const int64_t ch = lng ? (lr ^ (lr >> 1)) : (lr & 1);
-            const int64_t chunks = (rc.l + kRowsPad - 1) / kRowsPad;
-            if (ch >= chunks) continue;
-            for (int64_t e = ch * kRowsPad; e < (ch + 1) * kRowsPad; e++) {
-              if (e < rc.l) {
-                const int32_t a = ents[(size_t)(off[(size_t)rc.r] + rc.e0 + e)];
-                o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
-                o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
-              } else {
-                o.col.push_back(zero);
-                o.val.insert(o.val.end(), (size_t)es, 0);
+          for (const Column& c : cols) {
+            uint32_t w[2] = {0u, 0u};
+            for (int hh = 0; hh < (c.lng ? 1 : 2); hh++) {
+              const Rec& rc = c.rec[hh];
+              if (rc.r < 0) continue;
+              const int64_t stored = (rc.l + gran - 1) / gran * gran;
+              const int64_t slot = rows[(size_t)rc.r] - t0 + 1;
+              w[hh] = (uint32_t)elems | (uint32_t)stored << kRowsLenShift |
+                      (c.lng ? kRowsLong : 0u) | rc.flags | (uint32_t)slot << kRowsSlotShift;
+              for (int64_t e = 0; e < stored; e++) {
+                if (e < rc.l) {
+                  const int32_t a = ents[(size_t)(off[(size_t)rc.r] + rc.e0 + e)];
+                  o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
+                  o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
+                } else {
+                  o.col.push_back(zero);
+                  o.val.insert(o.val.end(), (size_t)es, 0);
+                }
               }
+              elems += stored;
             }
-            elems += kRowsPad;
+            if (c.lng) w[1] = w[0];
+            o.meta.push_back(w[0]);
+            o.meta.push_back(w[1]);
+            anyLong = anyLong || c.lng;
           }
           o.hdr.push_back((uint32_t)elems | (uint32_t)(2 * cols.size()) << 12 |
                           (anyLong ? 1u << 30 : 0u) | (backward ? 1u << 31 : 0u));

@@ -103,25 +103,31 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
 // 2-3) or one long record (at most kRowsMaxLen entries, all four lane rows).
 // A record is a row's entries (forward: strictly lower in-tile entries
 // ascending; backward: strictly upper in-tile entries descending, the order
-// ilu/Solver.cpp:380-442 visits them), cut into chunks of kRowsPad entries,
-// the last chunk padded with zero entries. A step stores its chunks in the
-// order of the lanes that hold them (lane l: lane row l >> 4, column l & 15,
-// record word 2 (l & 15) + (l >> 5); chunk (l >> 4) & 1 of a short record,
-// chunk 0, 1, 3, 2 for lane rows 0-3 of a long record; lanes without a chunk
-// store nothing). An entry is its column as a
-// 16-bit byte offset into the tile's x slots (4 * slot; slot 0 holds +0,
-// row i of the tile is slot i + 1) and the factor value, at the same element
-// index of their streams. A zero entry (value +0, column slot 0) adds a +0
-// product, which leaves every partial sum unchanged. A row with more than
-// kRowsMaxLen entries takes the same column of consecutive steps of its
-// level, kRowsMaxLen entries per step; the lanes carry the partial sum from
-// one step to the next (flags: continue from the carried sum; partial, the
-// row is not finished). Rows are placed longest first.
+// ilu/Solver.cpp:380-442 visits them), stored contiguously and padded to an
+// even number of entries (one zero entry for an odd row: 16-bit columns
+// then start on 4-byte boundaries; float64 factors: to a multiple of
+// kRowsPad, whose chunks then hold only the record's own entries). A step stores its records one after the
+// other in record-word order. Lane l (lane row l >> 4, column l & 15) takes
+// record word 2 (l & 15) + (l >> 5) and the record's chunk of kRowsPad
+// entries (l >> 4) & 1 of a short record, chunk 0, 1, 3, 2 for lane rows
+// 0-3 of a long record: it loads the kRowsPad entries from the chunk's
+// start and uses the ones inside the record (the rest belong to the next
+// record, step or tile, or to the zero tail; the kernel does not subtract
+// their products). An entry is its column as a 16-bit byte offset into the
+// tile's x slots (4 * slot; slot 0 holds +0, row i of the tile is slot
+// i + 1) and the factor value, at the same element index of their streams.
+// A zero entry (value +0, column slot 0) adds a +0 product, which leaves
+// every partial sum unchanged. A row with more than kRowsMaxLen entries
+// takes the same column of consecutive steps of its level, kRowsMaxLen
+// entries per step; the lanes carry the partial sum from one step to the
+// next (flags: continue from the carried sum; partial, the row is not
+// finished). Rows are placed longest first.
 //   step header (u32): elements of the step (bits 0-11), record words R =
 //     2 * columns (12-17), a long record present (30), backward (31)
 //   record words (u32, two per column, the same word twice for a long
-//     record, 0 for an empty half): chunks (bits 10-12), long (13),
-//     continue (14), partial (15), the row's slot (16-31)
+//     record, 0 for an empty half): the record's first element relative to
+//     the step's (bits 0-8), its stored length (9-14), long (15),
+//     continue (16), partial (17), the row's slot (18-31)
 // A tile's steps use elements [elemOff[t], elemOff[t+1]), headers
 // [hdrOff[t], hdrOff[t+1]) and record words [metaOff[t], metaOff[t+1]).
 // The element streams end with kRowsTail elements of zero entries.
@@ -130,9 +136,11 @@ constexpr int kRowsMaxLen = 32;
 constexpr int kRowsPad = 8;
 constexpr int kRowsColumns = 16;
 constexpr int kRowsTail = 64;
-constexpr uint32_t kRowsLong = 1u << 13;
-constexpr uint32_t kRowsCont = 1u << 14;
-constexpr uint32_t kRowsPartial = 1u << 15;
+constexpr uint32_t kRowsLong = 1u << 15;
+constexpr uint32_t kRowsCont = 1u << 16;
+constexpr uint32_t kRowsPartial = 1u << 17;
+constexpr int kRowsLenShift = 9;
+constexpr int kRowsSlotShift = 18;
 struct IluRowStream {
   std::vector<uint16_t> col;       // byte offsets (4 * local id)
   std::vector<uint8_t> val;        // factor values (es bytes each)

@@ -8,20 +8,23 @@
 // (stream format in ilu_pack.hpp). Each lane holds one chunk of 8 entries of
 // a record: a short record (<= 16 entries) uses lane rows 0 and 1 (or 2 and
 // 3) of its column, a long record (<= 32 entries) all four rows in the order
-// 0, 1, 3, 2; a step stores its chunks in lane order, so each wave-wide load
-// reads one contiguous range. Every lane reads the x values of its entries' columns from LDS
+// 0, 1, 3, 2. A step's records are stored one after the other without
+// padding to the chunk size, so a lane loads the 8 entries from its chunk's
+// start and drops the products of entries past its record's end (they
+// belong to the next record); the step's loads read one contiguous range.
+// Every lane reads the x values of its entries' columns from LDS
 // (all at once: they belong to earlier levels) and forms its products; then
 // the row's partial sum passes from lane to lane in entry order
 // (v_permlane16_swap between rows 0-1 / 2-3, v_permlane32_swap between rows
 // 1-3 and 2-0), each lane subtracting its products in the reference's order,
 // and the record's last lane finishes the row and writes it (bit-exact with
 // the sequential loop). The long-record passes run only in steps that hold
-// a long record. Chunks are padded with zero entries (value +0, the +0
-// slot); lanes without a chunk load nothing (their buffer offsets are out of
-// range, so the loads return +0 values and slot 0): their +0 products leave
-// the sum unchanged. A row longer than 32 entries continues in the same
-// column of the next steps, the lanes carrying the partial sum in a
-// register. A step's LDS write precedes the next step's reads in the wave's
+// a long record. Dropped products and a record's padding entry (value +0,
+// the +0 slot) are +0; lanes without a chunk load nothing (their buffer
+// offsets are out of range, so the loads return +0 values and slot 0):
+// subtracting +0 leaves the sum unchanged. A row longer than 32 entries
+// continues in the same column of the next steps, the lanes carrying the
+// partial sum in a register. A step's LDS write precedes the next step's reads in the wave's
 // program order, and LDS executes a wave's operations in order, so no
 // barrier separates levels.
 //
@@ -261,24 +264,25 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   const int dilu = p.dilu;
   Words4 vr[D][NV], cr[D];
   uint32_t mr[D], hr[D];
+  int nr[D];  // entries of the lane's chunk inside its record (0-8)
   X carry = (X)0;   // partial sum of a row continued in the next step
   int ldRow = 0;    // first record word of the next step to load
   uint32_t ldElem = 0;  // its first element
   auto header = [&](int k) { return __builtin_amdgcn_readfirstlane(hdr[k]); };
-  // loads of a step's chunks into ring slot u: the step stores the chunks
-  // in lane order, so a lane's chunk follows those of the lanes below it
-  // (a wave-wide load reads one contiguous range); lanes without a chunk
-  // take an offset outside the buffer: their loads return zeros without a
-  // memory access
+  // loads of a step's chunks into ring slot u: a lane's chunk starts
+  // kRowsChunk * ch entries into its record (the record's first element is
+  // in its record word); the step's records are contiguous, so a wave-wide
+  // load reads one range; lanes without a chunk take an offset outside the
+  // buffer: their loads return zeros without a memory access
   auto issue = [&](int u, uint32_t h, uint32_t mw) {
     const int R = (int)((h >> 12) & 63u);
     const uint32_t m = midx < R ? mw : 0u;
     const int ch = (m & kRowsLongBit) ? chLong : chShort;
-    const bool has = ch < (int)((m >> 10) & 7u);
-    const uint64_t bal = __builtin_amdgcn_ballot_w64(has);
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-    const uint32_t e = has ? ldElem + (uint32_t)kRowsChunk * rank : 0x8000000u;
+    const int rem = (int)((m >> kRowsLenShift) & 63u) - kRowsChunk * ch;
+    const bool has = rem > 0;
+    nr[u] = rem < 0 ? 0 : (rem > kRowsChunk ? kRowsChunk : rem);
+    const uint32_t e =
+        has ? ldElem + (m & 511u) + (uint32_t)(kRowsChunk * ch) : 0x8000000u;
     const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
 #pragma unroll
     for (int i = 0; i < NV; i++) vr[u][i] = bufLoad16(vrs, vo, 16 * i);
@@ -303,7 +307,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       const uint32_t mn = meta[ldRow + midx];
       const uint32_t h = hr[u], m = mr[u];
       const bool bwd = (h >> 31) != 0, anyLong = ((h >> 30) & 1u) != 0;
-      const int loc = (int)(m >> 16);  // the row's slot (0: no record)
+      const int loc = (int)(m >> kRowsSlotShift);  // the row's slot (0: no record)
       const bool lng = (m & kRowsLongBit) != 0, cont = (m & kRowsContBit) != 0;
       const bool partial = (m & kRowsPartialBit) != 0;
       X xv[kRowsChunk];
@@ -318,7 +322,15 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       __builtin_amdgcn_sched_barrier(0);
       X prod[kRowsChunk];
 #pragma unroll
-      for (int j = 0; j < kRowsChunk; j++) prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
+      for (int j = 0; j < kRowsChunk; j++) {
+        // entries past the record's end belong to other records: +0 instead
+        // (float64 records are padded to whole chunks)
+        const X q = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
+        if constexpr (sizeof(FT) == 8)
+          prod[j] = q;
+        else
+          prod[j] = j < nr[u] ? q : (X)0;
+      }
       // chunk 0 (rows 0, 2), chunk 1 (rows 1, 3), in entry order
       X acc = cont ? carry : bwd ? (X)0 : pX;
 #pragma unroll

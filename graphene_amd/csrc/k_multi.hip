@@ -52,8 +52,8 @@ __global__ __launch_bounds__(kSpmvThreads) void sellMultiKernel(SpmvArgs a, int6
         float v[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-          c[e] = cb + (int)col[(k + e) * 64];
-          v[e] = val[(k + e) * 64];
+          c[e] = cb + (int)streamLoad(&col[(k + e) * 64]);
+          v[e] = streamLoad(&val[(k + e) * 64]);
         }
 #pragma unroll
         for (int j = 0; j < K; j++) {
@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kSpmvThreads) void sellMultiKernel(SpmvArgs a, int6
         }
       }
       for (; k < len; k++) {
-        const int c = cb + (int)col[k * 64];
-        const float v = val[k * 64];
+        const int c = cb + (int)streamLoad(&col[k * 64]);
+        const float v = streamLoad(&val[k * 64]);
 #pragma unroll
         for (int j = 0; j < K; j++)
           if (act[j]) sum[j] = sum[j] + v * x[j * stride + c];

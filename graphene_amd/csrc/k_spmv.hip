@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
         const uint16_t* __restrict__ cw = a.scol + a.sliceColW[slice];
         // 8 entries per step: 16 independent stream loads, then 8 gathers,
         // then the adds in stored order; the rest of the row in steps of 4,
-        // 2 and 1 (one round of loads each)
+        // 2 and 1 (one round of loads each); stream loads are non-temporal
         auto rowSum = [&](auto* __restrict__ col, int cb) {
           int k = 0;
           auto batch = [&](auto nTag) {
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
             VT v[B];
 #pragma unroll
             for (int j = 0; j < B; j++) {
-              c[j] = cb + (int)col[(k + j) * 64];
-              v[j] = val[(k + j) * 64];
+              c[j] = cb + (int)streamLoad(&col[(k + j) * 64]);
+              v[j] = streamLoad(&val[(k + j) * 64]);
             }
             XT xv[B];
 #pragma unroll

@@ -245,9 +245,11 @@ constexpr int kRowsMaxLen = 32;  // entries of a record in one step (long record
 constexpr int kRowsChunk = 8;    // entries per lane
 constexpr int kRowsTail = 64;    // padding elements after the streams
 // record word flags
-constexpr uint32_t kRowsLongBit = 1u << 13;
-constexpr uint32_t kRowsContBit = 1u << 14;
-constexpr uint32_t kRowsPartialBit = 1u << 15;
+constexpr uint32_t kRowsLongBit = 1u << 15;
+constexpr uint32_t kRowsContBit = 1u << 16;
+constexpr uint32_t kRowsPartialBit = 1u << 17;
+constexpr int kRowsLenShift = 9;     // record word: stored length (6 bits)
+constexpr int kRowsSlotShift = 18;   // record word: the row's slot (14 bits)
 size_t rowsSharedBytes(const RowsArgs& p, DType ft, DType xt);
 void launchIluRows(const RowsArgs& p, const void* invD, DType ft, const void* rhs,
                    void* out, DType xt, const PreOp& pre, DevState* st,

@@ -290,7 +290,9 @@ class IluSolver : public Solver {
   void buildRows(hipStream_t s) {
     static_assert(kRowsMaxLen == gph::kRowsMaxLen && kRowsTail == gph::kRowsTail &&
                       kRowsPad == gph::kRowsChunk && kRowsLong == gph::kRowsLongBit &&
-                      kRowsCont == gph::kRowsContBit && kRowsPartial == gph::kRowsPartialBit,
+                      kRowsCont == gph::kRowsContBit && kRowsPartial == gph::kRowsPartialBit &&
+                      kRowsLenShift == gph::kRowsLenShift &&
+                      kRowsSlotShift == gph::kRowsSlotShift,
                   "record format of stream and kernel");
     const int64_t nnz = A_.nnzOff(), n = A_.n();
     const size_t es = gph::dtypeSize(ft_);
