@@ -380,10 +380,11 @@ def bench_serving(g, n_req, rates=(50, 200, 800), workers=8, budget=64 << 30):
     return out
 
 
-def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=1024):
+def mpir_cfg(inner_rel, inner_max, storage="float32", factors=None, rows_per_wave=1024,
+             ext="float64"):
     return {
         "type": "IterativeRefinement", "relResidual": 1e-10, "maxIterations": 50,
-        "extendedPrecisionType": "float64", "workingType": "float32",
+        "extendedPrecisionType": ext, "workingType": "float32",
         "innerSolver": {
             "type": "PBiCGStab", "relResidual": inner_rel, "maxIterations": inner_max,
             "workingType": "float32", "matrixValueType": storage,
@@ -502,8 +503,13 @@ def main():
             t = time.perf_counter()
             st = S.solve(x, bb)
             times.append(time.perf_counter() - t)
-        x64 = x if xdtype == np.float64 else g.DeviceVector.from_numpy(
-            x.numpy().astype(np.float64))
+        if xdtype == np.float64:
+            x64 = x
+        elif xdtype == g.TWOFLOAT:  # double words: hi + lo in float64
+            xn = x.numpy()
+            x64 = g.DeviceVector.from_numpy(xn["hi"].astype(np.float64) + xn["lo"])
+        else:
+            x64 = g.DeviceVector.from_numpy(x.numpy().astype(np.float64))
         r = A.residual(x64, b, destType=g.FLOAT64, intermediateType=g.FLOAT64)
         rel = float(np.linalg.norm(r.numpy())) / bnorm
         log(f"{label}: {np.median(times)*1e3:.2f} ms, {st}, true rel. residual {rel:.3e}")
@@ -560,6 +566,13 @@ def main():
         extra["mpir_fp16_factors"] = {"ms": 1e3 * float(np.median(tf)),
                                       "outer": stf.iterations, "inner": stf.innerIterations,
                                       "true_rel_residual": relf, "forward_error": fwd(xf)}
+        # twofloat32 extended precision (the reference's choice on hardware
+        # without FP64): solution and outer residual in double words
+        tt, stt, relt, xt2 = run(mpir_cfg(args.inner_rel, args.inner_max, ext="twofloat32"),
+                                 g.TWOFLOAT, "MPIR FP32/twofloat32", reps=2)
+        extra["mpir_twofloat"] = {"ms": 1e3 * float(np.median(tt)),
+                                  "outer": stt.iterations, "inner": stt.innerIterations,
+                                  "true_rel_residual": relt, "forward_error": fwd(xt2)}
     if not args.skip_fp64:
         t64, st64, rel64, x64 = run(fp64_cfg("float64"), np.float64, "plain FP64 PBiCGStab",
                                     reps=3)
@@ -567,7 +580,7 @@ def main():
         extra["fp64"] = {"ms": m64, "iterations": st64.iterations,
                          "true_rel_residual": rel64, "forward_error": fwd(x64),
                          "mpir_over_fp64": ms_step / m64}
-        for k in ("mpir_fp16_storage", "mpir_fp16_factors"):
+        for k in ("mpir_fp16_storage", "mpir_fp16_factors", "mpir_twofloat"):
             if k in extra:
                 extra["fp64"][k + "_over_fp64"] = extra[k]["ms"] / m64
         t64b, st64b, rel64b, x64b = run(fp64_cfg("float32"), np.float64,

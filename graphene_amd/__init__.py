@@ -20,12 +20,15 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgraphene_amd.so")
 
-FLOAT16, FLOAT32, FLOAT64 = 1, 2, 3
+FLOAT16, FLOAT32, FLOAT64, TWOFLOAT32 = 1, 2, 3, 4
 CRS, LDU = 0, 1
+# twofloat32 (two<float>): per element a float pair whose value is hi + lo
+TWOFLOAT = np.dtype([("hi", "<f4"), ("lo", "<f4")])
 _NP2T = {np.dtype(np.float16): FLOAT16, np.dtype(np.float32): FLOAT32,
-         np.dtype(np.float64): FLOAT64}
+         np.dtype(np.float64): FLOAT64, TWOFLOAT: TWOFLOAT32}
 _T2NP = {v: k for k, v in _NP2T.items()}
-_NAME2T = {"float16": FLOAT16, "float32": FLOAT32, "float64": FLOAT64}
+_NAME2T = {"float16": FLOAT16, "float32": FLOAT32, "float64": FLOAT64,
+           "twofloat32": TWOFLOAT32}
 
 
 class GrapheneError(RuntimeError):
@@ -383,7 +386,12 @@ class DeviceVector:
         return out
 
     def fill(self, value):
-        self.upload(np.full(self.n, value, self.dtype))
+        if self.dtype == TWOFLOAT:
+            a = np.zeros(self.n, TWOFLOAT)
+            a["hi"] = value
+            self.upload(a)
+        else:
+            self.upload(np.full(self.n, value, self.dtype))
 
     def cast(self, dtype):
         """Expression::cast (libgraphene/dsl/tensor/Expression.hpp:152) on the device."""

@@ -38,6 +38,7 @@ SOURCES = [
     "k_multi.hip",
     "k_ilu_block.hip",
     "k_gs.hip",
+    "k_dword.hip",
 ]
 
 COMMON = [

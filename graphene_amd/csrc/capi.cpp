@@ -65,6 +65,8 @@ gph::DType dt(gph_type t) {
       return gph::DType::F32;
     case GPH_FLOAT64:
       return gph::DType::F64;
+    case GPH_TWOFLOAT32:
+      return gph::DType::TF32;
   }
   gph::fail(gph::kInvalid, "invalid element type " + std::to_string((int)t));
 }

@@ -226,9 +226,16 @@ void readNorm(const Json& c, const char* k, VectorNorm& v) {
   const Json* j = c.get(k);
   if (j) v = parseVectorNorm(asText(*j));
 }
-void readType(const Json& c, const char* k, std::optional<gph::DType>& v) {
+// twofloat32 is accepted as an extended precision only (the reference's
+// inferType admits it with float32 operands only, dsl/common/details/
+// Expressions.cpp:41-52; the working precision is that of x)
+void readType(const Json& c, const char* k, std::optional<gph::DType>& v,
+              bool extended = false) {
   const Json* j = c.get(k);
-  if (j) v = parseType(asText(*j));
+  if (!j) return;
+  v = parseType(asText(*j));
+  if (*v == gph::DType::TF32 && !extended)
+    fail(gph::kUnsupported, std::string(k) + ": twofloat32 is supported as extendedPrecisionType only");
 }
 // Configuration.cpp:51-77
 void readMulticolor(const Json& c, const char* k,
@@ -270,12 +277,12 @@ std::string normToString(VectorNorm n) {
   return "None";
 }
 
-// Floating-point names of lg/common/Type.cpp:69-88. twofloat32 is carried as
-// float64 on this hardware (native FP64 replaces the double-word emulation).
+// Floating-point names of lg/common/Type.cpp:69-88.
 gph::DType parseType(const std::string& s) {
   if (s == "float16") return gph::DType::F16;
   if (s == "float32") return gph::DType::F32;
-  if (s == "float64" || s == "twofloat32") return gph::DType::F64;
+  if (s == "float64") return gph::DType::F64;
+  if (s == "twofloat32") return gph::DType::TF32;
   fail(kInvalid, "Unknown type in config: " + s);
 }
 
@@ -308,7 +315,7 @@ iterativerefinement::Configuration::Configuration(const Json& c) {
   readBool(c, "printPerformanceAfterSolve", printPerformanceAfterSolve);
   readBool(c, "printPerformanceEachIteration", printPerformanceEachIteration);
   readNorm(c, "norm", norm);
-  readType(c, "extendedPrecisionType", extendedPrecisionType);
+  readType(c, "extendedPrecisionType", extendedPrecisionType, true);
   readType(c, "workingType", workingType);
   readType(c, "matrixValueType", matrixValueType);
   const Json* in = c.get("innerSolver");

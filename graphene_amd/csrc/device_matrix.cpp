@@ -410,6 +410,15 @@ void DeviceMatrix::residual(const Vec& x, const Vec& b, const Vec& r,
   a.b = b.ptr;
   a.out = r.ptr;
   a.out64 = r64;
+  if (acc == DType::TF32 || x.type == DType::TF32) {
+    // double-word residual: x in twofloat32, b, r and the values float32
+    if (acc != DType::TF32 || x.type != DType::TF32 || b.type != DType::F32 ||
+        r.type != DType::F32 || vt != DType::F32)
+      gph::fail(gph::kUnsupported,
+                "twofloat32 residual: x and accumulation twofloat32, b, r and values float32");
+    gph::launchResidualDW(a, s);
+    return;
+  }
   gph::launchSpmv(a, vt, x.type, b.type, r.type, acc, gph::kModeResidual,
                   gph::kRedNone, s);
 }

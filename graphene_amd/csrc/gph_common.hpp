@@ -13,8 +13,10 @@
 namespace gph {
 
 // Element types. Names follow lg/common/Type.cpp:69-88 ("float16",
-// "float32", "float64"); "twofloat32" is mapped to float64 on this hardware.
-enum class DType : int32_t { F16 = 1, F32 = 2, F64 = 3 };
+// "float32", "float64", "twofloat32"). TF32 is the double-word type
+// two<float> (a float pair {hi, lo}, 8 bytes; dword.hpp), used as the
+// extended precision of mixed-precision refinement.
+enum class DType : int32_t { F16 = 1, F32 = 2, F64 = 3, TF32 = 4 };
 
 inline size_t dtypeSize(DType t) {
   switch (t) {
@@ -23,6 +25,7 @@ inline size_t dtypeSize(DType t) {
     case DType::F32:
       return 4;
     case DType::F64:
+    case DType::TF32:
       return 8;
   }
   throw std::invalid_argument("unknown element type");
@@ -36,6 +39,8 @@ inline const char* dtypeName(DType t) {
       return "float32";
     case DType::F64:
       return "float64";
+    case DType::TF32:
+      return "twofloat32";
   }
   return "?";
 }

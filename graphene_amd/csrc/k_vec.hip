@@ -308,6 +308,7 @@ void launchUniform(double* x, int64_t n, uint64_t seed, hipStream_t s) {
 
 void launchCast(const void* in, DType it, void* out, DType ot, int64_t n,
                 hipStream_t s) {
+  if (launchCastDW(in, it, out, ot, n, s)) return;
   if (n <= 0) return;
   const int g = elemGrid(n);
   const DType H = DType::F16, F = DType::F32, D = DType::F64;

@@ -99,6 +99,15 @@ void launchNorm2(const VecArgs& v, DType t, hipStream_t s);
 // ...V ops, which take the finished norm)
 void launchNormKind(const VecArgs& v, DType t, int kind, hipStream_t s);
 void launchFill(void* x, DType t, int64_t n, double value, hipStream_t s);
+// --- double-word ("twofloat32") kernels (k_dword.hip) ----------------------
+// r = fl32(b - A x) in double-word arithmetic (x: TF32, b, r and the matrix
+// values float32); out64 (optional) = the residual rounded to float64;
+// with state and fin: sum of its squares finalised with fin
+void launchResidualDW(const SpmvArgs& a, hipStream_t s);
+// phi (TF32) += d (float32), in double words
+void launchAccumulateDW(void* phi, const void* d, DType dt, int64_t n, hipStream_t s);
+// casts float32 / float64 <-> TF32; false when neither type is TF32
+bool launchCastDW(const void* in, DType it, void* out, DType ot, int64_t n, hipStream_t s);
 // ||x||_2 with the reduction scratch in a static device array instead of an
 // allocation (k_noalloc.hip); synchronous on stream s.
 double norm2NoAlloc(const void* x, DType t, int64_t n, hipStream_t s);

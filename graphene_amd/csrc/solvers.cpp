@@ -891,9 +891,11 @@ class PBiCGStabSolver : public Solver {
 
 // ---------------------------------------------------------------------------
 // Mixed-precision iterative refinement (iterative-refinement/Solver.cpp:48-123).
-// The residual norm is the FP64 norm of the unrounded residual (the
-// reference takes the norm of the residual after rounding to the working
-// type, :80-81 and :110).
+// Extended precision float64, or twofloat32: phi in double words, the
+// residual and the update in double-word arithmetic (k_dword.hip). The
+// residual norm is the FP64 norm of the unrounded residual (the reference
+// takes the norm of the residual after rounding to the working type,
+// :80-81 and :110).
 // ---------------------------------------------------------------------------
 class IterativeRefinementSolver : public Solver {
  public:
@@ -915,8 +917,14 @@ class IterativeRefinementSolver : public Solver {
     hipStream_t s = A_.stream();
     const int64_t n = A_.n();
     const DType E = cfg_->extendedPrecisionType.value_or(x.type);
-    const DType W = cfg_->workingType.value_or(x.type);
+    // x in twofloat32 is the solution itself (phi); the working type is then
+    // float32 unless configured
+    const DType W = cfg_->workingType.value_or(x.type == DType::TF32 ? DType::F32 : x.type);
     const DType V = cfg_->matrixValueType.value_or(DType::F32);
+    const bool dw = E == DType::TF32;  // double-word extended precision
+    if (dw && (W != DType::F32 || V != DType::F32 || b.type != DType::F32))
+      fail(kUnsupported,
+           "twofloat32 refinement: working type, matrix values and b must be float32");
     Vec phi = x;
     if (x.type != E) {
       phi = ws_.get(0, E, n);
@@ -933,8 +941,8 @@ class IterativeRefinementSolver : public Solver {
     const int kind = normKind(cfg_->norm);
     double bNorm = 0;
     if (cfg_->relTolerance > 0) bNorm = A_.norm(b, kind, s);
-    // the unrounded residual, for norms other than L2
-    Vec r64 = kind != gph::kNormL2 ? ws_.get(3, E, n) : Vec{};
+    // the unrounded residual (float64), for norms other than L2
+    Vec r64 = kind != gph::kNormL2 ? ws_.get(3, DType::F64, n) : Vec{};
 
     // r = fl_W(b - A phi) accumulated in E, norm of the unrounded residual
     auto residual = [&]() {
@@ -946,7 +954,10 @@ class IterativeRefinementSolver : public Solver {
       a.partials = partials_.get();
       a.fin = gph::kFinNormAux;
       if (r64.ptr) a.out64 = static_cast<double*>(r64.ptr);
-      gph::launchSpmv(a, V, E, b.type, W, E, gph::kModeResidual, gph::kRedSelf, s);
+      if (dw)
+        gph::launchResidualDW(a, s);
+      else
+        gph::launchSpmv(a, V, E, b.type, W, E, gph::kModeResidual, gph::kRedSelf, s);
       if (r64.ptr) {
         gph::VecArgs v;
         v.n = (int32_t)n;
@@ -954,7 +965,7 @@ class IterativeRefinementSolver : public Solver {
         v.state = st;
         v.partials = partials_.get();
         v.fin = gph::kFinNormAuxV;
-        gph::launchNormKind(v, E, kind, s);
+        gph::launchNormKind(v, DType::F64, kind, s);
       }
       double out = 0;
       GPH_HIP(hipMemcpyAsync(&out, &st->aux, sizeof(double),
@@ -972,8 +983,10 @@ class IterativeRefinementSolver : public Solver {
       stats.innerIterations += in.iterations;
       if (E == DType::F64) {
         gph::launchAccumulate(static_cast<double*>(phi.ptr), d.ptr, W, n, s);
+      } else if (dw) {
+        gph::launchAccumulateDW(phi.ptr, d.ptr, W, n, s);
       } else {
-        fail(kUnsupported, "IterativeRefinement needs float64 extended precision");
+        fail(kUnsupported, "IterativeRefinement needs float64 or twofloat32 extended precision");
       }
       stats.finalResidual = residual();
       stats.converged =

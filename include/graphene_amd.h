@@ -34,8 +34,17 @@ typedef enum {
   GPH_ERR_UNSUPPORTED = 4
 } gph_status;
 
-/* Element types, names of lg/common/Type.cpp:69-88. */
-typedef enum { GPH_FLOAT16 = 1, GPH_FLOAT32 = 2, GPH_FLOAT64 = 3 } gph_type;
+/* Element types, names of lg/common/Type.cpp:69-88. GPH_TWOFLOAT32 is the
+ * double-word type two<float>: per element a float pair {hi, lo} (8 bytes)
+ * whose value is hi + lo. It is the extended precision of refinement
+ * ("extendedPrecisionType": "twofloat32") and the x and accumulation type of
+ * gph_residual; casts convert float32/float64 to and from it. */
+typedef enum {
+  GPH_FLOAT16 = 1,
+  GPH_FLOAT32 = 2,
+  GPH_FLOAT64 = 3,
+  GPH_TWOFLOAT32 = 4
+} gph_type;
 
 /* lg/matrix/MatrixFormat.hpp */
 typedef enum { GPH_CRS = 0, GPH_LDU = 1 } gph_format;
@@ -202,7 +211,9 @@ int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes);
 int gph_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
              const void* x, gph_type y_type, void* y, gph_type acc_type);
 /* r = b - A*x (Matrix::residual, lg/matrix/Matrix.hpp:85-93). r64 (optional)
- * receives the unrounded float64 residual when acc_type is float64. */
+ * receives the unrounded float64 residual when acc_type is float64, the
+ * double-word residual rounded to float64 when acc_type is twofloat32 (then
+ * x is twofloat32, b, r and value_type float32). */
 int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
                  const void* x, gph_type b_type, const void* b,
                  gph_type r_type, void* r, gph_type acc_type, double* r64);

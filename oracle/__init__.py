@@ -85,6 +85,7 @@ def lib():
         L.orc_pbicgstab_f32.restype = C.c_int32
         L.orc_pbicgstab_f64.restype = C.c_int32
         L.orc_mpir.restype = C.c_int32
+        L.orc_mpir_dw.restype = C.c_int32
         L.orc_max_threads.restype = C.c_int32
         L.orc_vec_norm_f32.restype = C.c_float
         L.orc_vec_norm_f64.restype = C.c_double
@@ -226,6 +227,71 @@ def residual_mixed(L, x64, b32):
     lib().orc_residual_mixed(_i64(n), _p(rp), _p(col), _p(off), _p(diag),
                              _p(np.ascontiguousarray(x64, np.float64)),
                              _p(np.ascontiguousarray(b32, np.float32)), _p(r32), _p(r64))
+    return r32, r64
+
+
+TWOFLOAT = np.dtype([("hi", "<f4"), ("lo", "<f4")])
+
+
+def _pairs(a):
+    """(n, 2) float32 view of double words given as (n, 2) floats or the
+    structured TWOFLOAT dtype."""
+    a = np.ascontiguousarray(a)
+    if a.dtype == TWOFLOAT:
+        a = a.view(np.float32).reshape(-1, 2)
+    return np.ascontiguousarray(a, np.float32).reshape(-1, 2)
+
+
+def dw_two_sum(a, b):
+    out = np.empty(2, np.float32)
+    lib().orc_dw_two_sum(C.c_float(a), C.c_float(b), _p(out))
+    return out
+
+
+def dw_two_prod(a, b):
+    out = np.empty(2, np.float32)
+    lib().orc_dw_two_prod(C.c_float(a), C.c_float(b), _p(out))
+    return out
+
+
+def dw_add(x, y):
+    """Double-word sums of (n, 2) float32 arrays (hi, lo)."""
+    x, y = _pairs(x), _pairs(y)
+    z = np.empty_like(x)
+    lib().orc_dw_add(_i64(len(x)), _p(x), _p(y), _p(z))
+    return z
+
+
+def dw_mul(x, y):
+    x, y = _pairs(x), _pairs(y)
+    z = np.empty_like(x)
+    lib().orc_dw_mul(_i64(len(x)), _p(x), _p(y), _p(z))
+    return z
+
+
+def residual_dw(L, xhl, b32):
+    """Double-word residual (CRS order): r32 = fl32(b - A x), r64 = the same
+    rounded to float64; x as (n, 2) float pairs."""
+    rp, col, off, diag = _sys_arrays(L)
+    n = len(diag)
+    x = _pairs(xhl)
+    r32 = np.empty(n, np.float32)
+    r64 = np.empty(n, np.float64)
+    lib().orc_residual_dw(_i64(n), _p(rp), _p(col), _p(off), _p(diag), _p(x),
+                          _p(np.ascontiguousarray(b32, np.float32)), _p(r32), _p(r64))
+    return r32, r64
+
+
+def ldu_residual_dw(U, xhl, b32):
+    n = len(U["diag"])
+    x = _pairs(xhl)
+    r32 = np.empty(n, np.float32)
+    r64 = np.empty(n, np.float64)
+    lib().orc_ldu_residual_dw(_i64(n), _p(U["lower_addr"]), _p(U["upper_addr"]),
+                              _p(U["owner_start"]), _p(U["neighbour_start"]),
+                              _p(U["neighbour_col_ind"]), _p(U["upper"]), _p(U["lower"]),
+                              _p(U["diag"]), _p(x), _p(np.ascontiguousarray(b32, np.float32)),
+                              _p(r32), _p(r64))
     return r32, r64
 
 
@@ -384,4 +450,22 @@ class System:
         lib().orc_mpir(C.c_void_p(self._h), C.byref(c), _p(phi),
                        _p(np.ascontiguousarray(b32, np.float32)), C.byref(st),
                        C.byref(inner_it))
+        return phi, st, inner_it.value
+
+    def mpir_dw(self, b32, relResidual=1e-10, maxIterations=100, inner=None,
+                normFromRounded=False, norm="L2"):
+        """MPIR with twofloat32 extended precision: phi returned as (n, 2)
+        float32 (hi, lo) pairs."""
+        c = MPIRCfg()
+        c.norm = NORMS[norm]
+        c.relResidual = relResidual
+        c.maxIterations = maxIterations
+        c.normFromRounded = int(normFromRounded)
+        c.inner = inner if inner is not None else self.pbicgstab_cfg(1e-3, 100)
+        st = Stats()
+        phi = np.zeros((self.n, 2), np.float32)
+        inner_it = C.c_int32(0)
+        lib().orc_mpir_dw(C.c_void_p(self._h), C.byref(c), _p(phi),
+                          _p(np.ascontiguousarray(b32, np.float32)), C.byref(st),
+                          C.byref(inner_it))
         return phi, st, inner_it.value

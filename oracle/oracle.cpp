@@ -393,6 +393,101 @@ ORC_API void orc_residual_mixed(int64_t n, const int64_t* rowPtr, const int64_t*
 }
 
 // ---------------------------------------------------------------------------
+// Double-word ("twofloat32", two<float>) arithmetic: a value is hi + lo, two
+// floats. The reference's extended precision on hardware without FP64
+// (iterative-refinement/Solver.cpp:64-110 with the residuals of
+// CRSMatrix.cpp:107-125 and LDUMatrix.cpp:130-159 instantiated for
+// two<float>); its arithmetic comes from libtwofloat 0.2.0 (conanfile.py:64),
+// which the reference checkout does not contain, so this restates the
+// published algorithms instead (Joldes, Muller, Popescu, ACM TOMS 44(2),
+// 2017; u = 2^-24):
+//   sum:     2Sum(xh, yh) -> (sh, sl); 2Sum(xl, yl) -> (th, tl); c = sl + th;
+//            Fast2Sum(sh, c) -> (vh, vl); w = tl + vl; Fast2Sum(vh, w)
+//            (relative error <= 3u^2 + 13u^3)
+//   product: ch = xh*yh, cl1 = fma(xh, yh, -ch); t0 = xl*yl;
+//            t1 = fma(xh, yl, t0); cl2 = fma(xl, yh, t1); cl3 = cl1 + cl2;
+//            Fast2Sum(ch, cl3)  (relative error <= 5u^2)
+//   difference: sum with the negated operand; float -> two: (f, 0);
+//   two -> float: hi + lo.
+// Which variants libtwofloat's default operators use is not known here
+// (parity unpinned against libtwofloat; the tests pin these against exact
+// rational arithmetic).
+// ---------------------------------------------------------------------------
+struct Two {
+  float h = 0.0f, l = 0.0f;
+  Two() = default;
+  Two(float v) : h(v), l(0.0f) {}
+  Two(float a, float b) : h(a), l(b) {}
+  float eval() const { return h + l; }
+  double dbl() const { return (double)h + (double)l; }
+};
+static_assert(sizeof(Two) == 8, "two<float> is a float pair");
+static void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float a1 = s - b;
+  const float b1 = s - a1;
+  e = (a - a1) + (b - b1);
+}
+static void fast_two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  e = b - (s - a);
+}
+static Two operator+(const Two& x, const Two& y) {
+  float sh, sl, th, tl, vh, vl, zh, zl;
+  two_sum(x.h, y.h, sh, sl);
+  two_sum(x.l, y.l, th, tl);
+  const float c = sl + th;
+  fast_two_sum(sh, c, vh, vl);
+  const float w = tl + vl;
+  fast_two_sum(vh, w, zh, zl);
+  return Two(zh, zl);
+}
+static Two operator-(const Two& x, const Two& y) { return x + Two(-y.h, -y.l); }
+static Two operator*(const Two& x, const Two& y) {
+  const float ch = x.h * y.h;
+  const float cl1 = std::fma(x.h, y.h, -ch);
+  const float t0 = x.l * y.l;
+  const float t1 = std::fma(x.h, y.l, t0);
+  const float cl2 = std::fma(x.l, y.h, t1);
+  const float cl3 = cl1 + cl2;
+  float zh, zl;
+  fast_two_sum(ch, cl3, zh, zl);
+  return Two(zh, zl);
+}
+// primitives exposed for the exact-arithmetic pins
+ORC_API void orc_dw_two_sum(float a, float b, float* out) { two_sum(a, b, out[0], out[1]); }
+ORC_API void orc_dw_two_prod(float a, float b, float* out) {
+  out[0] = a * b;
+  out[1] = std::fma(a, b, -out[0]);
+}
+ORC_API void orc_dw_add(int64_t n, const float* x, const float* y, float* z) {
+  for (int64_t i = 0; i < n; i++) {
+    const Two r = Two(x[2 * i], x[2 * i + 1]) + Two(y[2 * i], y[2 * i + 1]);
+    z[2 * i] = r.h;
+    z[2 * i + 1] = r.l;
+  }
+}
+ORC_API void orc_dw_mul(int64_t n, const float* x, const float* y, float* z) {
+  for (int64_t i = 0; i < n; i++) {
+    const Two r = Two(x[2 * i], x[2 * i + 1]) * Two(y[2 * i], y[2 * i + 1]);
+    z[2 * i] = r.h;
+    z[2 * i + 1] = r.l;
+  }
+}
+// r = fl32(two(b) - (two(diag) * x_i + sum two(a_ij) * x_j)), x as (hi, lo)
+// pairs; r64 (optional) = the double-word residual rounded to float64.
+ORC_API void orc_residual_dw(int64_t n, const int64_t* rowPtr, const int64_t* col,
+                             const float* off, const float* diag, const float* xhl,
+                             const float* b, float* r32, double* r64) {
+  const Two* x = reinterpret_cast<const Two*>(xhl);
+  for (int64_t p = 0; p < n; p++) {
+    const Two r = Two(b[p]) - row_sum<float, Two, Two>(p, rowPtr, col, off, diag, x);
+    if (r32) r32[p] = r.eval();
+    if (r64) r64[p] = r.dbl();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // LDU addressing built from a structurally symmetric CRS (no reference
 // builder exists; follows lg/matrix/details/ldu/LDUAddressing.hpp:30-63):
 // faces enumerate (l, u) with l < u in row-major order of l, then u ascending;
@@ -487,6 +582,23 @@ ORC_API void orc_ldu_residual_mixed(int64_t n, const int64_t* lowerAddr,
                                        lower, diag, x);
     if (r32) r32[c] = (float)r;
     if (r64) r64[c] = r;
+  }
+}
+
+ORC_API void orc_ldu_residual_dw(int64_t n, const int64_t* lowerAddr,
+                                 const int64_t* upperAddr, const int64_t* ownerStart,
+                                 const int64_t* neighbourStart,
+                                 const int64_t* neighbourColInd, const float* upper,
+                                 const float* lower, const float* diag,
+                                 const float* xhl, const float* b, float* r32,
+                                 double* r64) {
+  const Two* x = reinterpret_cast<const Two*>(xhl);
+  for (int64_t c = 0; c < n; c++) {
+    const Two r = Two(b[c]) - ldu_row<Two, Two>(c, lowerAddr, upperAddr, ownerStart,
+                                                neighbourStart, neighbourColInd, upper,
+                                                lower, diag, x);
+    if (r32) r32[c] = r.eval();
+    if (r64) r64[c] = r.dbl();
   }
 }
 
@@ -1066,6 +1178,54 @@ ORC_API int32_t orc_mpir(void* h, const OrcMPIRCfg* c, double* phi, const float*
       pbicgstab<float>(s, &c->inner, d.data(), r32.data(), &ist);
       for (int64_t i = 0; i < n; i++) phi[i] = phi[i] + (double)d[i];
     }
+    inner += ist.iterations;
+    st->finalResidual = resid();
+    st->converged = converged_test(st->finalResidual, st->initialResidual,
+                                   bNorm, c->absTolerance, c->relTolerance,
+                                   c->relResidual);
+  }
+  if (innerIterations) *innerIterations = inner;
+  return 0;
+}
+
+// MPIR with extendedPrecisionType "twofloat32" (iterative-refinement/
+// Solver.cpp:48-123): phi as double words ((hi, lo) float pairs), the
+// residual in double-word arithmetic (orc_residual_dw), the update
+// phi = phi + two(d). Norm of the double-word residual rounded to float64,
+// as the FP64 variant above uses the unrounded residual.
+ORC_API int32_t orc_mpir_dw(void* h, const OrcMPIRCfg* c, float* phihl, const float* b,
+                            OrcStats* st, int32_t* innerIterations) {
+  OrcSystem* s = (OrcSystem*)h;
+  int64_t n = s->n;
+  const int32_t T = s->T;
+  const int64_t* tsp = s->tileStart.data();
+  Two* phi = reinterpret_cast<Two*>(phihl);
+  std::vector<float> r32(n);
+  std::vector<double> r64(n);
+  auto resid = [&]() {
+    orc_residual_dw(n, s->rowPtr.data(), s->col.data(), s->off.data(), s->diag.data(),
+                    phihl, b, r32.data(), r64.data());
+    if (c->normFromRounded) return (double)vec_norm<float>(r32.data(), n, T, tsp, c->norm);
+    return vec_norm<double>(r64.data(), n, T, tsp, c->norm);
+  };
+  double bNorm = 0;
+  if (c->relTolerance > 0) {
+    std::vector<double> bd(b, b + n);
+    bNorm = vec_norm<double>(bd.data(), n, T, tsp, c->norm);
+  }
+  st->initialResidual = resid();
+  st->finalResidual = st->initialResidual;
+  st->iterations = 0;
+  st->converged = 0;
+  st->singular = 0;
+  int32_t inner = 0;
+  while (!((st->converged && st->iterations >= c->minIterations) ||
+           st->iterations >= c->maxIterations)) {
+    st->iterations++;
+    OrcStats ist;
+    std::vector<float> d(n, 0.0f);
+    pbicgstab<float>(s, &c->inner, d.data(), r32.data(), &ist);
+    for (int64_t i = 0; i < n; i++) phi[i] = phi[i] + Two(d[i]);
     inner += ist.iterations;
     st->finalResidual = resid();
     st->converged = converged_test(st->finalResidual, st->initialResidual,

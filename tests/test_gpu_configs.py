@@ -7,7 +7,8 @@ cfg1  128^3 27-point convection-diffusion (Peclet 10), 16x8x8-cell tiles:
       CRS and LDU SpMV bit-exact against the oracle at full size; MPIR
       (FP64 outer, FP32 PBiCGStab + block ILU(0) inner) to 1e-10 with the
       true FP64 residual checked and the outer/inner counts compared with
-      one oracle MPIR run; forward error against an FP64 solve to 1e-14.
+      one oracle MPIR run; forward error against an FP64 solve to 1e-14; the
+      same refinement in twofloat32 to a true 1e-10.
 cfg2  4,194,304-row power-law matrix, 4096 contiguous tiles: CRS SpMV (both
       kernel forms) bit-exact against the oracle; PBiCGStab + symmetric
       Gauss-Seidel (2 sweeps) iteration count against the oracle, and the
@@ -132,6 +133,27 @@ def test_cfg1_mpir_against_oracle(cfg1):
     fwd = float(np.linalg.norm(xm - xr) / np.linalg.norm(xr))
     print(f"cfg1 forward error of MPIR vs FP64 (1e-14): {fwd:.3e}", flush=True)
     assert fwd <= 1e-8
+
+
+def test_cfg1_mpir_twofloat(cfg1):
+    """The same refinement with twofloat32 extended precision (the solution
+    in double words): converged, true FP64 residual of hi + lo <= 1e-10, the
+    outer count of the FP64-extended refinement +-1."""
+    h, b = cfg1["h"], cfg1["b"]
+    A = h.copyToTile(g.LDU)
+    x = g.DeviceVector(h.nrows, g.TWOFLOAT)
+    x.fill(0.0)
+    cfg = dict(mpir_cfg(ILU_INNER), extendedPrecisionType="twofloat32")
+    st = g.Solver(A, cfg).solve(x, dev(b))
+    xn = x.numpy()
+    x64 = xn["hi"].astype(np.float64) + xn["lo"]
+    rel = true_rel(A, dev(x64), dev(b))
+    phi = dev(np.zeros(h.nrows, np.float64))
+    st64 = g.Solver(A, mpir_cfg(ILU_INNER)).solve(phi, dev(b))
+    print(f"cfg1 MPIR twofloat32: {st} (true rel. residual {rel:.3e}); float64: {st64}",
+          flush=True)
+    assert st.converged and rel <= 1e-10
+    assert abs(st.iterations - st64.iterations) <= 1
 
 
 # ---- cfg2 -------------------------------------------------------------------

@@ -572,3 +572,132 @@ def test_vector_norms_vs_numpy():
     assert o.vec_norm(v32, ts, "LINF") == np.max(np.abs(v32))
     assert np.isclose(o.vec_norm(v32, ts, "L1"), np.sum(np.abs(v.astype(np.float32)),
                                                           dtype=np.float64), rtol=1e-5)
+
+
+# --- double-word ("twofloat32") arithmetic: exact rational pins ---------------
+from fractions import Fraction as Fr  # noqa: E402
+
+U = Fr(1, 2 ** 24)  # unit roundoff of float32
+
+
+def _fr(v):
+    return Fr(float(v))
+
+
+def _dw_of(d):
+    """Normalized double words (hi, lo) of float64 values."""
+    hi = d.astype(np.float32)
+    return np.stack([hi, (d - hi.astype(np.float64)).astype(np.float32)], 1)
+
+
+def _dw_samples(rng, m, cancel=False):
+    """Pairs of double-word operands; with cancel, x + y nearly cancels."""
+    d = rng.standard_normal(m) * 10.0 ** rng.integers(-8, 8, m)
+    if cancel:
+        e = -d * (1 + rng.standard_normal(m) * 10.0 ** rng.integers(-12, -2, m))
+    else:
+        e = rng.standard_normal(m) * 10.0 ** rng.integers(-8, 8, m)
+    return _dw_of(d), _dw_of(e)
+
+
+def test_dw_error_free_transforms():
+    """2Sum and the FMA product are exact: hi + lo == a + b (a * b) in
+    rational arithmetic, hi == fl32 of the operation."""
+    rng = np.random.default_rng(5)
+    a = (rng.standard_normal(400) * 10.0 ** rng.integers(-20, 20, 400)).astype(np.float32)
+    b = (rng.standard_normal(400) * 10.0 ** rng.integers(-20, 20, 400)).astype(np.float32)
+    for x, y in list(zip(a, b)) + [(1.0, 2.0 ** -30), (3.0, -3.0), (0.0, 0.0), (-1e30, 1e-30)]:
+        x, y = np.float32(x), np.float32(y)
+        s, e = o.dw_two_sum(x, y)
+        assert _fr(s) + _fr(e) == _fr(x) + _fr(y)
+        assert s == np.float32(np.float32(x) + np.float32(y))
+        p, q = o.dw_two_prod(x, y)
+        if np.isfinite(p) and abs(float(p)) > 1e-30:
+            assert _fr(p) + _fr(q) == _fr(x) * _fr(y)
+
+
+@pytest.mark.parametrize("cancel", [False, True])
+def test_dw_add_within_bound(cancel):
+    """Accurate double-word sum: |z - (x + y)| <= (3u^2 + 13u^3) |x + y| (also
+    under cancellation, where the sloppy variant fails), result normalized."""
+    rng = np.random.default_rng(7 + cancel)
+    x, y = _dw_samples(rng, 500, cancel)
+    z = o.dw_add(x, y)
+    bound = 3 * U ** 2 + 13 * U ** 3
+    for xi, yi, zi in zip(x, y, z):
+        exact = _fr(xi[0]) + _fr(xi[1]) + _fr(yi[0]) + _fr(yi[1])
+        got = _fr(zi[0]) + _fr(zi[1])
+        assert abs(got - exact) <= bound * abs(exact)
+        assert np.float32(zi[0] + zi[1]) == zi[0]  # |lo| <= ulp(hi) / 2
+
+
+def test_dw_mul_within_bound():
+    """Double-word product: |z - x y| <= 5u^2 |x y| (dropping a cross term
+    gives errors of order u)."""
+    rng = np.random.default_rng(9)
+    x, y = _dw_samples(rng, 500)
+    z = o.dw_mul(x, y)
+    for xi, yi, zi in zip(x, y, z):
+        exact = (_fr(xi[0]) + _fr(xi[1])) * (_fr(yi[0]) + _fr(yi[1]))
+        got = _fr(zi[0]) + _fr(zi[1])
+        assert abs(got - exact) <= 5 * U ** 2 * abs(exact)
+
+
+def _dw_vec(rng, n):
+    d = rng.uniform(-1, 1, n) / 3.0
+    hi = d.astype(np.float32)
+    return np.stack([hi, (d - hi.astype(np.float64)).astype(np.float32)], 1)
+
+
+def test_dw_residual_vs_exact():
+    """Double-word residual of a small convection-diffusion-like system in
+    several tiles: within the double-word error bound of the exact rational
+    residual (a float64-accumulated residual is u-accurate only when there is
+    no cancellation; here b ~ A x so the residual cancels to ~1e-7 |terms|)."""
+    rng = np.random.default_rng(3)
+    L = poisson_layout(4, 3, 3, 2, np.repeat([0, 1], 18).astype(np.int32))
+    n = len(L["diag"])
+    x = _dw_vec(rng, n)
+    xf = np.array([_fr(a) + _fr(b) for a, b in x], dtype=object)
+    # b = fl32(A x) so that the residual is small relative to its terms
+    b = o.spmv(L, x[:, 0].astype(np.float64) + x[:, 1]).astype(np.float32)
+    r32, r64 = o.residual_dw(L, x, b)
+    rp, col, off, dg = L["row_ptr"], L["col"], L["off"], L["diag"]
+    for p in range(n):
+        terms = [_fr(dg[p]) * xf[p]] + [_fr(off[a]) * xf[col[a]] for a in range(rp[p], rp[p + 1])]
+        exact = _fr(b[p]) - sum(terms)
+        scale = abs(_fr(b[p])) + sum(abs(t) for t in terms)
+        k = len(terms) + 1
+        assert abs(Fr(float(r64[p])) - exact) <= 2 * k * 5 * U ** 2 * scale + abs(exact) * 2 ** -53
+        assert r32[p] == np.float32(r64[p]) or abs(_fr(r32[p]) - exact) <= abs(exact) * U * 2
+
+
+def test_dw_ldu_equals_crs_on_exact_inputs():
+    """LDU and CRS double-word residuals agree exactly when every operation is
+    exact (small integers), so they differ only in summation order."""
+    L = poisson_layout(4, 4, 3)
+    n = len(L["diag"])
+    rng = np.random.default_rng(1)
+    x = np.stack([rng.integers(-8, 8, n).astype(np.float32), np.zeros(n, np.float32)], 1)
+    b = rng.integers(-64, 64, n).astype(np.float32)
+    U_ = o.ldu_build(L)
+    assert np.array_equal(o.residual_dw(L, x, b)[1], o.ldu_residual_dw(U_, x, b)[1])
+    ref = b.astype(np.float64) - o.spmv(L, x[:, 0].astype(np.float64))
+    assert np.array_equal(o.residual_dw(L, x, b)[1], ref)
+
+
+def test_mpir_twofloat_converges():
+    """MPIR with twofloat32 extended precision reaches the tolerance in a
+    float64 residual of its solution, with the outer count of the float64
+    variant (+-1)."""
+    L = poisson_layout(10, 10, 10)
+    n = len(L["diag"])
+    b = o.spmv(L, o.uniform_pm1(4, n).astype(np.float64)).astype(np.float32)
+    S = o.System(L)
+    phi, st, inner = S.mpir_dw(b, 1e-10, 30)
+    assert st.converged
+    x = phi[:, 0].astype(np.float64) + phi[:, 1]
+    r = b.astype(np.float64) - o.spmv(L, x)
+    assert np.linalg.norm(r) <= 1.01e-10 * st.initialResidual
+    _, st64, _ = S.mpir(b, 1e-10, 30)
+    assert abs(st.iterations - st64.iterations) <= 1
