@@ -12,11 +12,13 @@ namespace graphene::batch {
 
 using gph::DType;
 using gph::fail;
+using graphene::Json;
 using matrix::Vec;
 namespace solver = matrix::solver;
 
 namespace {
 constexpr int kGraphIterations = 4;  // inner iterations per graph launch
+constexpr int kOuterGraphSteps = 8;  // steps per graph launch, outer step on the device
 
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -39,6 +41,13 @@ gph::DevState stateFor(float absTol, float relRes, int minIt, int maxIt) {
 std::shared_ptr<solver::ilu::Configuration> BatchSolver::parseConfig(
     const std::string& configJson) {
   auto cfg = solver::Configuration::fromJSON(configJson);
+  // "outerStep": "device" (default) or "host": where the refinement's outer
+  // step runs (a key of the batch only)
+  if (const Json* o = Json::parse(configJson).get("outerStep")) {
+    if (o->kind != Json::String || (o->str != "device" && o->str != "host"))
+      fail(gph::kInvalid, "outerStep: \"device\" or \"host\"");
+    outerOnDevice_ = o->str == "device";
+  }
   std::shared_ptr<solver::pbicgstab::Configuration> pb;
   if (auto ir = std::dynamic_pointer_cast<solver::iterativerefinement::Configuration>(cfg)) {
     mpir_ = true;
@@ -89,6 +98,11 @@ void BatchSolver::setupChunks(hipStream_t st) {
   const size_t n = (size_t)sysRow_[(size_t)S_];
   for (gph::DevBuf* b : {&r_, &r0_, &p_, &v_, &y_, &s_, &z_, &t_, &d_, &rw_}) b->alloc(n * 4);
   if (mpir_) r64_.alloc(n * 8);
+  if (mpir_ && !multi_) {
+    phi_.alloc(n * 8);
+    b_.alloc(n * 4);
+  }
+  phase_.alloc((size_t)S_ * sizeof(int32_t));
   args_.S = S_;
   args_.C = C_;
   args_.chunkSys = chunkSys_.as<int32_t>();
@@ -97,6 +111,7 @@ void BatchSolver::setupChunks(hipStream_t st) {
   args_.st = inner_.as<gph::DevState>();
   args_.glob = glob_.as<gph::DevState>();
   args_.partials = partials_.get();
+  args_.phase = phase_.as<int32_t>();
   GPH_HIP(hipHostMalloc((void**)&pinned_, 2 * sizeof(gph::DevState), 0));
   gph::DevState zero{};
   GPH_HIP(hipMemcpyAsync(glob_.get(), &zero, sizeof(zero), hipMemcpyHostToDevice, st));
@@ -171,8 +186,8 @@ BatchSolver::BatchSolver(const matrix::DeviceMatrix& A, const std::vector<int32_
     }
   }
   // work of finished systems is skipped: ILU wave groups and SpMV slices
-  // whose systems have all finished return at once
-  ilu_->setSystemSkip(args_.st, sysRow_);
+  // with no inner solve running return at once
+  ilu_->setSystemSkip(args_.phase, sysRow_);
   {
     const gph::SpmvArgs a = A.spmvArgs(DType::F32);
     if (a.numSlices > 0) {
@@ -237,7 +252,7 @@ size_t BatchSolver::deviceBytes() const {
   size_t b = 0;
   for (const gph::DevBuf* q : {&chunkSys_, &chunkRow_, &sysChunk_, &inner_, &outer_, &glob_,
                                &partials_, &sliceSys_, &r_, &r0_, &p_, &v_, &y_, &s_, &z_, &t_,
-                               &d_, &rw_, &r64_})
+                               &d_, &rw_, &r64_, &phase_, &phi_, &b_})
     b += q->bytes();
   return b;
 }
@@ -246,6 +261,8 @@ BatchSolver::~BatchSolver() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
+  if (outerExec_) (void)hipGraphExecDestroy(outerExec_);
+  if (outerGraph_) (void)hipGraphDestroy(outerGraph_);
   if (pinned_) (void)hipHostFree(pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -270,8 +287,9 @@ void BatchSolver::enqueueIteration(hipStream_t s) {
     a.state = glob;
     a.checkDone = 1;
     if (sliceSys_) {
-      a.sysState = args_.st;
+      a.sysPhase = args_.phase;
       a.sliceSys = sliceSys_.as<int32_t>();
+      a.runPhase = 0;
     }
     gph::launchSpmv(a, DType::F32, DType::F32, DType::F32, DType::F32, DType::F32,
                     gph::kModeSpmv, gph::kRedNone, s);
@@ -305,11 +323,96 @@ void BatchSolver::captureGraph(hipStream_t s) {
   GPH_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
 }
 
+// Refinement step of the systems whose inner solve has finished (phase 1),
+// on the device: phi += d, the FP64 residual of their SpMV slices, norm and
+// convergence, inner restart (first: the initial residual of every system,
+// nothing accumulated).
+void BatchSolver::enqueueOuterStep(hipStream_t s, int first) {
+  gph::DevState* glob = glob_.as<gph::DevState>();
+  if (!first) gph::launchBatchPhaseAccumulate(args_, phi_.as<double>(), d_.as<float>(), s);
+  gph::SpmvArgs a = A_.spmvArgs(DType::F32);
+  a.x = phi_.get();
+  a.b = b_.get();
+  a.out = rw_.get();
+  a.out64 = r64_.as<double>();
+  a.state = glob;
+  a.checkDone = 1;
+  a.sysPhase = args_.phase;
+  a.sliceSys = sliceSys_.as<int32_t>();
+  a.runPhase = 1;
+  gph::launchSpmv(a, DType::F32, DType::F64, DType::F32, DType::F32, DType::F64,
+                  gph::kModeResidual, gph::kRedNone, s);
+  gph::launchBatchPhaseOuter(args_, outer_.as<gph::DevState>(), r64_.as<double>(), first, s);
+  gph::launchBatchPhaseRestart(args_, rw_.as<float>(), d_.as<float>(), r_.as<float>(),
+                               r0_.as<float>(), p_.as<float>(), s);
+}
+
+// Batched refinement with the outer step on the device: every system moves
+// between its inner solve (phase 0), its outer step (phase 1) and finished
+// (phase 2) on its own; a captured graph of kOuterGraphSteps steps (one
+// PBiCGStab iteration of the running systems, then the outer step of the
+// systems that need one) is replayed until every system has finished.
+void BatchSolver::solveDeviceOuter(const Vec& x, const Vec& b) {
+  hipStream_t s = stream();
+  const int64_t n = sysRow_[(size_t)S_];
+  args_.deviceOuter = 1;
+  std::vector<gph::DevState> init((size_t)S_, innerInit_);
+  std::vector<gph::DevState> oinit((size_t)S_, outerInit_);
+  std::vector<int32_t> ph((size_t)S_, 1);
+  gph::DevState g0{};
+  g0.iter = S_;  // every system's first outer step (initial residual) is due
+  GPH_HIP(hipMemcpyAsync(inner_.get(), init.data(), init.size() * sizeof(gph::DevState),
+                         hipMemcpyHostToDevice, s));
+  GPH_HIP(hipMemcpyAsync(outer_.get(), oinit.data(), oinit.size() * sizeof(gph::DevState),
+                         hipMemcpyHostToDevice, s));
+  GPH_HIP(hipMemcpyAsync(phase_.get(), ph.data(), ph.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice, s));
+  GPH_HIP(hipMemcpyAsync(glob_.get(), &g0, sizeof(g0), hipMemcpyHostToDevice, s));
+  GPH_HIP(hipMemcpyAsync(b_.get(), b.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  gph::launchFill(phi_.get(), DType::F64, n, 0.0, s);
+  xInner_ = d_.as<float>();
+  enqueueOuterStep(s, 1);  // initial residuals, first inner starts
+  if (!outerExec_) {
+    GPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kOuterGraphSteps; k++) {
+      enqueueIteration(s);
+      enqueueOuterStep(s, 0);
+    }
+    GPH_HIP(hipStreamEndCapture(s, &outerGraph_));
+    GPH_HIP(hipGraphInstantiate(&outerExec_, outerGraph_, nullptr, nullptr, 0));
+  }
+  // two graph launches in flight; the flag copy of the older one is checked
+  hipEvent_t ev[2];
+  GPH_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  GPH_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  auto launch = [&](int slot) {
+    GPH_HIP(hipGraphLaunch(outerExec_, s));
+    launches_++;
+    GPH_HIP(hipMemcpyAsync(pinned_ + slot, glob_.get(), sizeof(gph::DevState),
+                           hipMemcpyDeviceToHost, s));
+    GPH_HIP(hipEventRecord(ev[slot], s));
+  };
+  int cur = 0;
+  launch(cur);
+  for (;;) {
+    launch(cur ^ 1);
+    GPH_HIP(hipEventSynchronize(ev[cur]));
+    if (pinned_[cur].done) break;
+    cur ^= 1;
+  }
+  GPH_HIP(hipMemcpyAsync(x.ptr, phi_.get(), (size_t)n * 8, hipMemcpyDeviceToDevice, s));
+  GPH_HIP(hipStreamSynchronize(s));
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  args_.deviceOuter = 0;
+}
+
 // Batched PBiCGStab from x = 0 on the systems whose inner state is not
 // finished; adds each system's iteration count to innerIts.
 void BatchSolver::innerSolve(const void* rhs, void* x, std::vector<int64_t>& innerIts) {
   hipStream_t s = stream();
   xInner_ = static_cast<float*>(x);  // always d_ (captured in the graph)
+  gph::launchBatchPhaseSync(args_, s);
   gph::launchBatchStart(args_, static_cast<const float*>(rhs), static_cast<float*>(x),
                         r_.as<float>(), r0_.as<float>(), p_.as<float>(), s);
   gph::launchBatchFin(args_, gph::kFinInit, 1, s);
@@ -370,7 +473,22 @@ std::vector<SystemStats> BatchSolver::solve(const Vec& x, const Vec& b) {
     seconds_ = now() - t0;
     return out;
   }
-  // refinement: phi = 0 (x is overwritten), r = fl32(b - A phi) in FP64
+  if (!multi_ && sliceSys_ && outerOnDevice_) {
+    // refinement, outer step on the device
+    solveDeviceOuter(x, b);
+    std::vector<gph::DevState> os((size_t)S_);
+    GPH_HIP(hipMemcpy(os.data(), outer_.get(), os.size() * sizeof(gph::DevState),
+                      hipMemcpyDeviceToHost));
+    for (int32_t k = 0; k < S_; k++) {
+      const auto& q = os[(size_t)k];
+      out[(size_t)k] = SystemStats{q.iter, q.innerTotal, q.r0n, q.rF, q.converged != 0, false};
+    }
+    seconds_ = now() - t0;
+    return out;
+  }
+  // refinement, outer step on the host (several right-hand sides of one
+  // system, or a matrix without the sliced layout): phi = 0 (x is
+  // overwritten), r = fl32(b - A phi) in FP64
   std::vector<gph::DevState> oinit((size_t)S_, outerInit_);
   GPH_HIP(hipMemcpyAsync(outer_.get(), oinit.data(), oinit.size() * sizeof(gph::DevState),
                          hipMemcpyHostToDevice, s));

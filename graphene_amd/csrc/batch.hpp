@@ -11,10 +11,16 @@
 // entry coupling two systems (checked). SpMV and the block-Jacobi ILU(0)
 // run once per step on the union; per system are the Krylov scalars, the
 // convergence state, the (segmented, deterministic) dot products and the
-// masked vector updates. The inner iteration is captured once in a HIP
-// graph of several iterations and replayed until every system has
-// finished; finished systems are skipped and a finished batch makes every
-// kernel return at once. Setup picks the ILU substitution kernel (and the
+// masked vector updates. Refinement runs its outer step on the device: each
+// system moves on its own between its inner solve (phase 0), its outer step
+// (phase 1: phi += d, the FP64 residual of its rows, norm, convergence and
+// the inner restart) and finished (phase 2); one PBiCGStab iteration of the
+// running systems followed by the outer step of the systems that need one
+// is captured in a HIP graph of several such steps and replayed until every
+// system has finished ("outerStep": "host": the inner solves of all systems
+// end before the host enqueues their outer steps). Work of systems in
+// another phase is skipped and a finished batch makes every kernel return
+// at once. Setup picks the ILU substitution kernel (and the
 // packed kernel's rows per wave) on the union by a few timed applications.
 #pragma once
 
@@ -73,6 +79,8 @@ class BatchSolver {
   void innerSolve(const void* rhs, void* x, std::vector<int64_t>& innerIts);
   void enqueueIteration(hipStream_t s);
   void captureGraph(hipStream_t s);
+  void enqueueOuterStep(hipStream_t s, int first);
+  void solveDeviceOuter(const matrix::Vec& x, const matrix::Vec& b);
 
   const matrix::DeviceMatrix& A_;
   int32_t S_ = 0, C_ = 0;
@@ -83,9 +91,14 @@ class BatchSolver {
   gph::DevState innerInit_{}, outerInit_{};
   std::shared_ptr<matrix::solver::Solver> ilu_;
   bool multi_ = false;  // nrhs right-hand sides of one system
+  bool outerOnDevice_ = true;  // refinement: outer step on the device
   hipStream_t stream_ = nullptr;
   gph::DevBuf chunkSys_, chunkRow_, sysChunk_, inner_, outer_, glob_, partials_, sliceSys_;
   gph::DevBuf r_, r0_, p_, v_, y_, s_, z_, t_, d_, rw_, r64_;
+  gph::DevBuf phase_;    // per system: 0 inner solve, 1 outer step due, 2 finished
+  gph::DevBuf phi_, b_;  // refinement on the device: solution and right-hand side
+  hipGraph_t outerGraph_ = nullptr;
+  hipGraphExec_t outerExec_ = nullptr;
   gph::BatchArgs args_;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;

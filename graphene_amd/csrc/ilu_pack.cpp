@@ -359,14 +359,15 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
   return o;
 }
 
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, bool dense) {
   IluRowStream o;
+  o.dense = dense && es != 8;
   const int32_t T = m.numTiles;
   for (int32_t t = 0; t < T; t++) o.maxRows = std::max<int32_t>(o.maxRows, (int32_t)m.tileRows(t));
   if (4 * (int64_t)(o.maxRows + 1) > 65535) return o;
   const uint16_t zero = 0;  // slot 0 holds +0; row i of a tile is slot i + 1
   const uint8_t* luB = static_cast<const uint8_t*>(lu);
-  const int64_t gran = es == 8 ? kRowsPad : 2;  // record length granule
+  const int64_t gran = o.dense ? 2 : kRowsPad;  // record length granule
   o.elemOff.assign((size_t)T + 1, 0);
   o.hdrOff.assign((size_t)T + 1, 0);
   o.metaOff.assign((size_t)T + 1, 0);
@@ -428,10 +429,9 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es) {
             cols.push_back(c);
           }
           while (!cols.empty() && !cols.back().lng && cols.back().rec[0].r < 0) cols.pop_back();
-          // records in record-word order, each contiguous and padded to an
-          // even length (float64 factors: to whole chunks, so the kernel
-          // needs no per-entry test); a long record's two words share its
-          // entries
+          // records in record-word order, each contiguous, padded to whole
+          // chunks or (dense) to an even length; a long record's two words
+          // share its entries
           bool anyLong = false;
           int64_t elems = 0;
           for (const Column& c : cols) {

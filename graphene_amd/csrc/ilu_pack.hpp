@@ -103,10 +103,10 @@ IluLevelStream packIluLevels(const host::HostMatrix& m, int64_t targetRows,
 // 2-3) or one long record (at most kRowsMaxLen entries, all four lane rows).
 // A record is a row's entries (forward: strictly lower in-tile entries
 // ascending; backward: strictly upper in-tile entries descending, the order
-// ilu/Solver.cpp:380-442 visits them), stored contiguously and padded to an
-// even number of entries (one zero entry for an odd row: 16-bit columns
-// then start on 4-byte boundaries; float64 factors: to a multiple of
-// kRowsPad, whose chunks then hold only the record's own entries). A step stores its records one after the
+// ilu/Solver.cpp:380-442 visits them), stored contiguously and padded with
+// zero entries to a multiple of kRowsPad, or (a dense stream; not for
+// float64 factors) to an even number of entries (one zero entry for an odd
+// row: 16-bit columns then start on 4-byte boundaries). A step stores its records one after the
 // other in record-word order. Lane l (lane row l >> 4, column l & 15) takes
 // record word 2 (l & 15) + (l >> 5) and the record's chunk of kRowsPad
 // entries (l >> 4) & 1 of a short record, chunk 0, 1, 3, 2 for lane rows
@@ -152,10 +152,17 @@ struct IluRowStream {
   int32_t maxRows = 0;             // largest tile (the zero slot's id)
   int32_t maxSteps = 0;            // most steps of one tile (both directions)
   int64_t maxRecs = 0;             // most records of one tile
+  bool dense = false;              // records padded to even lengths only
   bool ok = false;  // false: ids or the stream too large for the formats
 };
 
-IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es);
+// dense: records padded to an even length (element size 2 or 4) instead of
+// whole chunks. Fewer bytes for rows longer than a chunk, at the price of a
+// per-entry test in the kernel and more registers: worth it when the rows
+// average more than kRowsPad in-tile entries per direction (3D 27-point
+// tiles: 22.5k instead of 28.2k elements per 16x8x8 tile, apply 95 -> 89 us;
+// 2D 9-point tiles of 4-entry rows: 10 % slower).
+IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, bool dense);
 
 // Level-major stream of the ILU(0) factors for the workgroup-per-tile
 // kernel (tiles of any size; x in LDS when it fits).

@@ -76,20 +76,42 @@ __device__ __forceinline__ X chunkSumWave(const X* part, int c0, int c1, int lan
 // the counts are integers, so the result is order-free).
 template <typename X>
 __global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int K) {
-  __shared__ int nActive;
+  __shared__ int nActive, nDue;
   if (op != kFinInit && b.glob->done) return;
-  if (threadIdx.x == 0) nActive = 0;
+  // device-side refinement: the restart has nothing to do when no outer step
+  // is due
+  if (b.deviceOuter && op == kFinInit && b.glob->iter == 0) return;
+  if (threadIdx.x == 0) {
+    nActive = 0;
+    nDue = 0;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nw = (int)(blockDim.x >> 6);
   const X* part = static_cast<const X*>(b.partials);
   int mine = 0;
-  auto finish = [&](DevState* st, const double (&sum)[2]) {
-    if (op == kFinInit && st->done) {
+  // device-side refinement: the initialisation restarts the inner solve of
+  // the systems whose outer step is due (phase 1, inner state finished)
+  const bool restart = b.deviceOuter && op == kFinInit;
+  auto skip = [&](int s, const DevState* st) {
+    if (restart) return b.phase[s] != 1;
+    return st->done && op != kFinInit;
+  };
+  auto finish = [&](int s, DevState* st, const double (&sum)[2]) {
+    if (op == kFinInit && st->done && !restart) {
       // a system excluded from this solve (its outer iteration has
       // converged) stays finished
     } else {
       finalize<X>(st, op, sum[0], sum[1]);
+    }
+    if (b.phase) {
+      if (restart && !st->done) {
+        b.phase[s] = 0;
+        atomicAdd(&nDue, -1);
+      } else if (!restart && st->done && b.phase[s] == 0) {
+        b.phase[s] = b.deviceOuter ? 1 : 2;
+        if (b.deviceOuter) atomicAdd(&nDue, 1);
+      }
     }
     mine += st->done ? 0 : 1;
   };
@@ -98,25 +120,31 @@ __global__ __launch_bounds__(1024) void batchFinKernel(BatchArgs b, int op, int 
     const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
     if (c1 - c0 > kSerialChunks) continue;
     DevState* st = b.st + s;
-    if (st->done && op != kFinInit) continue;
+    if (skip(s, st)) continue;
     double sum[2] = {0.0, 0.0};
     for (int k = 0; k < K; k++) sum[k] = (double)chunkSumSerial(part + k * b.C, c0, c1);
-    finish(st, sum);
+    finish(s, st, sum);
   }
   // the others: one wave each
   for (int s = wave; s < b.S; s += nw) {
     const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
     if (c1 - c0 <= kSerialChunks) continue;
     DevState* st = b.st + s;
-    if (st->done && op != kFinInit) continue;  // the same for every lane of the wave
+    if (skip(s, st)) continue;  // the same for every lane of the wave
     double sum[2] = {0.0, 0.0};
     for (int k = 0; k < K; k++) sum[k] = (double)chunkSumWave(part + k * b.C, c0, c1, lane);
-    if (lane == 0) finish(st, sum);
+    if (lane == 0) finish(s, st, sum);
   }
-  if (op == kFinInit || op == kFinUpdate || op == kFinOmega) {
+  // glob->done: no system running (device-side refinement: set by the outer
+  // step instead, and glob->iter counts the systems whose outer step is due)
+  if (!b.deviceOuter && (op == kFinInit || op == kFinUpdate || op == kFinOmega)) {
     atomicAdd(&nActive, mine);
     __syncthreads();
     if (threadIdx.x == 0) b.glob->done = nActive == 0 ? 1 : 0;
+  }
+  if (b.deviceOuter) {
+    __syncthreads();
+    if (threadIdx.x == 0 && nDue != 0) b.glob->iter = b.glob->iter + nDue;
   }
 }
 
@@ -268,6 +296,127 @@ __global__ __launch_bounds__(kBatchThreads) void batchNorm64Kernel(BatchArgs b,
   if (threadIdx.x == 0) static_cast<double*>(b.partials)[blockIdx.x] = acc[0];
 }
 
+__global__ __launch_bounds__(1024) void batchPhaseSyncKernel(BatchArgs b) {
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x) b.phase[s] = b.st[s].done ? 2 : 0;
+}
+
+// ---- refinement step on the device, per system in phase 1 -------------------
+// phi += d (iterative-refinement/Solver.cpp:103)
+// (grid-stride over chunks: the grid is small, so that a step with no outer
+// step due costs little)
+__global__ __launch_bounds__(kBatchThreads) void batchPhaseAccumulateKernel(BatchArgs b,
+                                                                            double* phi,
+                                                                            const float* d) {
+  if (b.glob->done || b.glob->iter == 0) return;
+  for (int c = blockIdx.x; c < b.C; c += gridDim.x) {
+    const ChunkRange ch = chunkOf(b, c);
+    if (b.phase[ch.sys] != 1) continue;
+    for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads)
+      phi[i] = phi[i] + (double)d[i];
+  }
+}
+
+// FP64 chunk partials of the squared unrounded residual (0 for chunks of
+// systems in another phase)
+__global__ __launch_bounds__(kBatchThreads) void batchPhaseNorm64Kernel(BatchArgs b,
+                                                                        const double* r64) {
+  __shared__ double lds[kBatchThreads / 64];
+  if (b.glob->done || b.glob->iter == 0) return;
+  for (int c = blockIdx.x; c < b.C; c += gridDim.x) {
+    const ChunkRange ch = chunkOf(b, c);
+    if (b.phase[ch.sys] != 1) continue;  // uniform in the block
+    double acc[1] = {0.0};
+    for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads)
+      acc[0] = acc[0] + r64[i] * r64[i];
+    blockSum<double, 1, kBatchThreads>(acc, lds);
+    if (threadIdx.x == 0) static_cast<double*>(b.partials)[c] = acc[0];
+  }
+}
+
+// Outer step of the systems in phase 1 (iterative-refinement/Solver.cpp:
+// 89-112): the norm of the residual, the inner iterations of the finished
+// inner solve counted, one refinement iteration counted and convergence
+// tested (first: the initial residual); a finished system goes to phase 2.
+// Then glob->done = every system finished.
+__global__ __launch_bounds__(1024) void batchPhaseOuterKernel(BatchArgs b, DevState* outer,
+                                                              int first) {
+  __shared__ int nLeft, nGone;
+  if (b.glob->done || b.glob->iter == 0) return;
+  if (threadIdx.x == 0) {
+    nLeft = 0;
+    nGone = 0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  const double* part = static_cast<const double*>(b.partials);
+  for (int q = 0; q < 2; q++) {  // pass 0: thread per system; pass 1: wave per system
+    for (int s = q == 0 ? (int)threadIdx.x : wave; s < b.S;
+         s += q == 0 ? (int)blockDim.x : nw) {
+      const int c0 = b.sysChunk[s], c1 = b.sysChunk[s + 1];
+      if ((c1 - c0 > kSerialChunks) != (q == 1)) continue;
+      if (b.phase[s] != 1) continue;  // the same for every lane of the wave
+      const double acc = q == 0 ? chunkSumSerial(part, c0, c1) : chunkSumWave(part, c0, c1, lane);
+      if (q == 1 && lane != 0) continue;
+      const double rF = sqrt(acc);
+      DevState* o = outer + s;
+      if (first) {
+        o->r0n = rF;
+        o->rF = rF;
+        o->iter = 0;
+        o->converged = 0;
+        o->innerTotal = 0;
+        o->done = o->maxIt <= 0 ? 1 : 0;
+      } else {
+        o->innerTotal = o->innerTotal + b.st[s].iter;
+        o->iter = o->iter + 1;
+        o->rF = rF;
+        o->converged = convergedTest(o, rF) ? 1 : 0;
+        o->done = (o->converged && o->iter >= o->minIt) || o->iter >= o->maxIt ? 1 : 0;
+      }
+      if (o->done) {
+        b.phase[s] = 2;
+        b.st[s].done = 1;
+        atomicAdd(&nGone, 1);
+      }
+    }
+  }
+  __syncthreads();
+  int mine = 0;
+  for (int s = threadIdx.x; s < b.S; s += blockDim.x) mine += b.phase[s] != 2 ? 1 : 0;
+  atomicAdd(&nLeft, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    b.glob->done = nLeft == 0 ? 1 : 0;
+    b.glob->iter = b.glob->iter - nGone;
+  }
+}
+
+// Inner restart of the systems in phase 1: x = 0, r = r0 = p = rhs (the
+// rounded residual) and the chunk partials of r.r for the initialisation.
+__global__ __launch_bounds__(kBatchThreads) void batchPhaseStartKernel(BatchArgs b,
+                                                                       const float* rhs, float* x,
+                                                                       float* r, float* r0,
+                                                                       float* p) {
+  __shared__ float lds[kBatchThreads / 64];
+  if (b.glob->done || b.glob->iter == 0) return;
+  for (int c = blockIdx.x; c < b.C; c += gridDim.x) {
+    const ChunkRange ch = chunkOf(b, c);
+    if (b.phase[ch.sys] != 1) continue;  // uniform in the block
+    float acc[1] = {0.0f};
+    for (int i = ch.r0 + threadIdx.x; i < ch.r1; i += kBatchThreads) {
+      const float v = rhs[i];
+      x[i] = 0.0f;
+      r[i] = v;
+      r0[i] = v;
+      p[i] = v;
+      acc[0] = acc[0] + v * v;
+    }
+    blockSum<float, 1, kBatchThreads>(acc, lds);
+    if (threadIdx.x == 0) static_cast<float*>(b.partials)[c] = acc[0];
+  }
+}
+
 // ---------------------------------------------------------------------------
 void launchBatchDot1(const BatchArgs& b, const void* a0, const void* b0, hipStream_t s) {
   BatchDot<1> d;
@@ -322,6 +471,34 @@ void launchBatchOuter(const BatchArgs& b, DevState* outer, const double* r64, in
                       hipStream_t s) {
   hipLaunchKernelGGL(batchNorm64Kernel, dim3(b.C), dim3(kBatchThreads), 0, s, b, outer, r64);
   hipLaunchKernelGGL(batchOuterKernel, dim3(1), dim3(1024), 0, s, b, outer, first);
+  GPH_HIP(hipGetLastError());
+}
+
+// grid of the outer-step chunk kernels (grid-stride over the chunks)
+static int phaseGrid(const BatchArgs& b) { return b.C < 1024 ? b.C : 1024; }
+
+void launchBatchPhaseSync(const BatchArgs& b, hipStream_t s) {
+  if (!b.phase) return;
+  hipLaunchKernelGGL(batchPhaseSyncKernel, dim3(1), dim3(1024), 0, s, b);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchPhaseAccumulate(const BatchArgs& b, double* phi, const float* d,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(batchPhaseAccumulateKernel, dim3(phaseGrid(b)), dim3(kBatchThreads), 0, s, b,
+                     phi, d);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchPhaseOuter(const BatchArgs& b, DevState* outer, const double* r64, int first,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(batchPhaseNorm64Kernel, dim3(phaseGrid(b)), dim3(kBatchThreads), 0, s, b, r64);
+  hipLaunchKernelGGL(batchPhaseOuterKernel, dim3(1), dim3(1024), 0, s, b, outer, first);
+  GPH_HIP(hipGetLastError());
+}
+void launchBatchPhaseRestart(const BatchArgs& b, const float* rhs, float* x, float* r, float* r0,
+                             float* p, hipStream_t s) {
+  hipLaunchKernelGGL(batchPhaseStartKernel, dim3(phaseGrid(b)), dim3(kBatchThreads), 0, s, b, rhs, x,
+                     r, r0, p);
+  hipLaunchKernelGGL(batchFinKernel<float>, dim3(1), dim3(1024), 0, s, b, (int)kFinInit, 1);
   GPH_HIP(hipGetLastError());
 }
 

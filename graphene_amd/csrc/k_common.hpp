@@ -19,9 +19,11 @@ struct DevState {
   int32_t iter, converged, singular, done;
   float absTol, relTol, relRes;
   int32_t minIt, maxIt;
-  int32_t pad[3];
+  int32_t innerTotal;  // batched refinement: inner iterations of the finished inner solves
+  int32_t pad[2];
   uint32_t counter[8];
 };
+static_assert(sizeof(DevState) == 144, "DevState layout");
 
 // What the last block of a fused reduction does with the totals.
 enum FinOp : int32_t {

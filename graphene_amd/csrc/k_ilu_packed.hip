@@ -243,11 +243,11 @@ __global__ __launch_bounds__(kPackedThreads) void iluPackedKernel(
     X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (checkDone && st->done) return;
-  if (p.groupSys) {  // batched: skip groups whose systems have all finished
-    bool done = true;
+  if (p.groupSys) {  // batched: skip groups with no inner solve running
+    bool run = false;
     for (int k = p.groupSys[2 * blockIdx.x]; k <= p.groupSys[2 * blockIdx.x + 1]; k++)
-      done = done && p.sysState[k].done;
-    if (done) return;
+      run = run || p.sysPhase[k] == 0;
+    if (!run) return;
   }
   const PackedLds L = packedLayout(p, sizeof(X));
   X* xs = reinterpret_cast<X*>(smem + L.xs);

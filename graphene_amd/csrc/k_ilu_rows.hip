@@ -8,10 +8,11 @@
 // (stream format in ilu_pack.hpp). Each lane holds one chunk of 8 entries of
 // a record: a short record (<= 16 entries) uses lane rows 0 and 1 (or 2 and
 // 3) of its column, a long record (<= 32 entries) all four rows in the order
-// 0, 1, 3, 2. A step's records are stored one after the other without
-// padding to the chunk size, so a lane loads the 8 entries from its chunk's
-// start and drops the products of entries past its record's end (they
-// belong to the next record); the step's loads read one contiguous range.
+// 0, 1, 3, 2. A step's records are stored one after the other, padded to
+// whole chunks or, in a dense stream, to even lengths: there a lane loads
+// the 8 entries from its chunk's start and drops the products of entries
+// past its record's end (they belong to the next record); either way the
+// step's loads read one contiguous range.
 // Every lane reads the x values of its entries' columns from LDS
 // (all at once: they belong to earlier levels) and forms its products; then
 // the row's partial sum passes from lane to lane in entry order
@@ -187,7 +188,10 @@ __device__ __forceinline__ void stageRows(int t0, int nt, const DT* __restrict__
   }
 }
 
-template <typename FT, typename X>
+// DENSE: records padded to even lengths, so a lane's 8-entry window may
+// extend past its record (those products are dropped); else records are
+// whole chunks.
+template <typename FT, typename X, bool DENSE>
 __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     RowsArgs p, const InvDiag<FT>* __restrict__ invD, const X* __restrict__ rhs,
     X* __restrict__ out, PreOp pre, DevState* st, int checkDone) {
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     out += j * p.rhsStride;
   }
   // batched solves: the tile's system has finished
-  if (p.tileSys && p.sysState[p.tileSys[tile]].done) return;
+  if (p.tileSys && p.sysPhase[p.tileSys[tile]] != 0) return;
   const RowsLds L = rowsLayout(p, sizeof(X), D);
   const int lane = threadIdx.x;
   X* xs = reinterpret_cast<X*>(smem + L.xs);
@@ -264,7 +268,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
   const int dilu = p.dilu;
   Words4 vr[D][NV], cr[D];
   uint32_t mr[D], hr[D];
-  int nr[D];  // entries of the lane's chunk inside its record (0-8)
+  int nr[DENSE ? D : 1];  // dense: entries of the lane's chunk inside its record
   X carry = (X)0;   // partial sum of a row continued in the next step
   int ldRow = 0;    // first record word of the next step to load
   uint32_t ldElem = 0;  // its first element
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
     const int ch = (m & kRowsLongBit) ? chLong : chShort;
     const int rem = (int)((m >> kRowsLenShift) & 63u) - kRowsChunk * ch;
     const bool has = rem > 0;
-    nr[u] = rem < 0 ? 0 : (rem > kRowsChunk ? kRowsChunk : rem);
+    if constexpr (DENSE) nr[u] = rem < 0 ? 0 : (rem > kRowsChunk ? kRowsChunk : rem);
     const uint32_t e =
         has ? ldElem + (m & 511u) + (uint32_t)(kRowsChunk * ch) : 0x8000000u;
     const int vo = (int)(e * sizeof(FT)), co = (int)(e * 2u);
@@ -322,14 +326,12 @@ __global__ __launch_bounds__(kRowsThreads) void iluRowsKernel(
       __builtin_amdgcn_sched_barrier(0);
       X prod[kRowsChunk];
 #pragma unroll
-      for (int j = 0; j < kRowsChunk; j++) {
-        // entries past the record's end belong to other records: +0 instead
-        // (float64 records are padded to whole chunks)
-        const X q = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
-        if constexpr (sizeof(FT) == 8)
-          prod[j] = q;
-        else
-          prod[j] = j < nr[u] ? q : (X)0;
+      for (int j = 0; j < kRowsChunk; j++) prod[j] = cvt<X>(rowVal<FT>(vr[u], j)) * xv[j];
+      // dense records: entries past a record's end belong to other records,
+      // +0 instead
+      if constexpr (DENSE) {
+#pragma unroll
+        for (int j = 0; j < kRowsChunk; j++) prod[j] = j < nr[u] ? prod[j] : (X)0;
       }
       // chunk 0 (rows 0, 2), chunk 1 (rows 1, 3), in entry order
       X acc = cont ? carry : bwd ? (X)0 : pX;
@@ -367,7 +369,8 @@ template <typename FT, typename X>
 static void rowsTyped(const RowsArgs& p, const void* invD, const void* rhs, void* out,
                       const PreOp& pre, DevState* st, int checkDone, hipStream_t s) {
   const size_t shm = rowsLayout(p, sizeof(X), rowsDepth<FT>()).total;
-  auto k = iluRowsKernel<FT, X>;
+  if (p.dense && sizeof(FT) == 8) fail(kUnsupported, "dense ILU(0) records: float64 factors");
+  auto k = p.dense ? iluRowsKernel<FT, X, sizeof(FT) != 8> : iluRowsKernel<FT, X, false>;
   if (shm > 64 * 1024)
     GPH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)shm));

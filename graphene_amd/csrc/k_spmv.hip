@@ -189,6 +189,10 @@ template <typename VT, typename XT, typename BT, typename OT, typename AT,
 __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
   __shared__ AT redLds[2 * kSpmvThreads / 64];
   if (a.checkDone && a.state->done) return;
+  // batched refinement: the residual of the systems whose outer step is due
+  // (phase 1) is skipped at once when none is (the batch state's iter
+  // counts them)
+  if (SKIP && a.runPhase == 1 && a.state && a.state->iter == 0) return;
   // each block owns one contiguous range of slice groups (4 slices = 4
   // waves per group); blocks are remapped so an XCD sweeps adjacent ranges
   const int lb = a.remap ? xcdRemap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
@@ -212,11 +216,11 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
       const int g = c0 + i;
       const int slice = g * W + wave;
       bool sliceOk = g < g1 && slice < a.numSlices;
-      if (SKIP && sliceOk) {  // batched: skip slices of finished systems
-        bool done = true;
+      if (SKIP && sliceOk) {  // batched: only slices of systems in the run phase
+        bool run = false;
         for (int k = a.sliceSys[2 * slice]; k <= a.sliceSys[2 * slice + 1]; k++)
-          done = done && a.sysState[k].done;
-        sliceOk = !done;
+          run = run || a.sysPhase[k] == a.runPhase;
+        sliceOk = run;
       }
       const int slot = slice * 64 + lane;
       int row = 0, len = 0;

@@ -51,10 +51,12 @@ struct SpmvArgs {
   const uint32_t* sliceColW = nullptr; // first column word of each slice
   const uint16_t* scol = nullptr;      // column words
   const void* sval = nullptr;
-  // optional (batched solves): slice s is skipped when the systems
-  // sliceSys[2s] .. sliceSys[2s+1] are all finished (sysState[k].done)
-  const DevState* sysState = nullptr;
+  // optional (batched solves): slice s is processed only when one of the
+  // systems sliceSys[2s] .. sliceSys[2s+1] is in phase runPhase
+  // (sysPhase[k]: 0 inner solve running, 1 outer step due, 2 finished)
+  const int32_t* sysPhase = nullptr;
   const int32_t* sliceSys = nullptr;
+  int32_t runPhase = 0;
 };
 
 enum SpmvMode : int32_t { kModeSpmv = 0, kModeResidual = 1 };
@@ -190,8 +192,9 @@ struct PackedArgs {
   int32_t dilu = 0;  // diagonal-based ILU: x_i = t*invD_i / x_i + t*invD_i
   PackedDirArgs fwd, bwd;
   // optional (batched solves): group g is skipped when the systems
-  // groupSys[2g] .. groupSys[2g+1] are all finished (sysState[k].done)
-  const DevState* sysState = nullptr;
+  // groupSys[2g] .. groupSys[2g+1] have no inner solve running
+  // (sysPhase[k] != 0)
+  const int32_t* sysPhase = nullptr;
   const int32_t* groupSys = nullptr;
 };
 constexpr int kPackedThreads = 64;  // one wave per tile
@@ -231,6 +234,7 @@ void launchIluLevel(const LevelArgs& p, DType ft, const void* rhs, void* out,
 struct RowsArgs {
   int32_t T = 0, maxRows = 0, maxSteps = 0, maxRecs = 0;
   int32_t dilu = 0;
+  int32_t dense = 0;  // records padded to even lengths (else whole chunks)
   const int32_t* tileStart = nullptr;  // T+1
   const uint32_t* elemOff = nullptr;   // T+1 element offsets of the tiles' streams
   const int32_t* hdrOff = nullptr;     // T+1 offsets of the tiles' step headers
@@ -240,8 +244,8 @@ struct RowsArgs {
   const uint16_t* col = nullptr;       // column byte offsets (4 * local id)
   const void* val = nullptr;           // factor values
   // optional (batched solves): tile t is skipped when system tileSys[t]
-  // has finished (sysState[tileSys[t]].done)
-  const DevState* sysState = nullptr;
+  // has no inner solve running (sysPhase[tileSys[t]] != 0)
+  const int32_t* sysPhase = nullptr;
   const int32_t* tileSys = nullptr;
   // several right-hand sides of the same matrix in one launch: vector j is
   // rhs/out + j * rhsStride; it is skipped when rhsState[j].done (optional)
@@ -320,6 +324,12 @@ struct BatchArgs {
   DevState* st = nullptr;             // S inner-solver states
   DevState* glob = nullptr;           // done = no system running
   void* partials = nullptr;           // 2*C sums
+  // per system: 0 inner solve running, 1 outer (refinement) step due,
+  // 2 finished (optional: the skip masks of SpMV slices and ILU tiles)
+  int32_t* phase = nullptr;
+  // refinement with the outer step on the device: a finished inner solve
+  // moves its system to phase 1 and glob->done is set by the outer step
+  int32_t deviceOuter = 0;
 };
 template <int K>
 struct BatchDot {
@@ -342,6 +352,21 @@ void launchBatchAccumulate(const BatchArgs& b, const DevState* outer, double* ph
                            const float* d, hipStream_t s);
 void launchBatchOuter(const BatchArgs& b, DevState* outer, const double* r64, int first,
                       hipStream_t s);
+// Device-side refinement step of the systems in phase 1 (their inner solve
+// has finished): phi += d (launchBatchPhaseAccumulate); after the FP64
+// residual of their rows: per-system norm, refinement convergence, phase 2
+// for the finished ones and glob->done when every system is (outer step,
+// launchBatchPhaseOuter; first: the initial residual of every system); the
+// inner restart of those still refining: d = 0, r = r0 = p = fl32(residual)
+// and the inner initialisation, phase 0 (launchBatchPhaseRestart).
+// phase[s] = 2 if the inner state of system s is finished, else 0 (start of
+// an inner solve driven by the host)
+void launchBatchPhaseSync(const BatchArgs& b, hipStream_t s);
+void launchBatchPhaseAccumulate(const BatchArgs& b, double* phi, const float* d, hipStream_t s);
+void launchBatchPhaseOuter(const BatchArgs& b, DevState* outer, const double* r64, int first,
+                           hipStream_t s);
+void launchBatchPhaseRestart(const BatchArgs& b, const float* rhs, float* x, float* r, float* r0,
+                             float* p, hipStream_t s);
 void launchBatchOuterMask(const BatchArgs& b, const DevState* outer, hipStream_t s);
 // dst[k] = idx[k] < 0 ? 0 : src[idx[k]]  (same element type t)
 void launchGather(const void* src, DType t, const int32_t* idx, void* dst,

@@ -169,7 +169,7 @@ class IluSolver : public Solver {
     Solver::applyDeviceMulti(rhs, out, k, stride, states, st, s);
   }
 
-  void setSystemSkip(const DevState* states,
+  void setSystemSkip(const int32_t* phases,
                      const std::vector<int64_t>& systemRowStart) override {
     if (rows_) {  // the system of every tile (systems are tile ranges)
       const std::vector<int64_t>& ts = A_.hostMatrix().tileStart;
@@ -182,7 +182,7 @@ class IluSolver : public Solver {
       tileSys_.upload(sys.data(), sys.size(), A_.stream());
       GPH_HIP(hipStreamSynchronize(A_.stream()));
       rargs_.tileSys = tileSys_.as<int32_t>();
-      rargs_.sysState = states;
+      rargs_.sysPhase = phases;
       return;
     }
     if (!packed_) return;
@@ -201,7 +201,7 @@ class IluSolver : public Solver {
     groupSys_.upload(gs.data(), gs.size(), A_.stream());
     GPH_HIP(hipStreamSynchronize(A_.stream()));
     pargs_.groupSys = groupSys_.as<int32_t>();
-    pargs_.sysState = states;
+    pargs_.sysPhase = phases;
   }
 
   size_t factorBytes() const {
@@ -301,7 +301,17 @@ class IluSolver : public Solver {
     GPH_HIP(hipMemcpyAsync(lu.data(), luD, (size_t)nnz * es, hipMemcpyDeviceToHost, s));
     GPH_HIP(hipStreamSynchronize(s));
     (void)n;
-    IluRowStream R = packIluRows(A_.hostMatrix(), lu.data(), (int)es);
+    // dense records when rows average more than one chunk of in-tile
+    // entries per direction
+    const host::HostMatrix& hm = A_.hostMatrix();
+    int64_t inTile = 0;
+    for (int32_t t = 0; t < hm.numTiles; t++)
+      for (int64_t i = hm.tileStart[(size_t)t]; i < hm.tileStart[(size_t)t + 1]; i++)
+        for (int64_t a = hm.rowPtr[(size_t)i]; a < hm.rowPtr[(size_t)i + 1]; a++)
+          inTile += hm.col[(size_t)a] >= hm.tileStart[(size_t)t] &&
+                    hm.col[(size_t)a] < hm.tileStart[(size_t)t + 1];
+    const bool dense = inTile > 2 * (int64_t)kRowsPad * n;
+    IluRowStream R = packIluRows(hm, lu.data(), (int)es, dense);
     if (!R.ok) return;
     rcol_.upload(R.col.data(), R.col.size(), s);
     rval_.upload(R.val.data(), R.val.size(), s);
@@ -318,6 +328,7 @@ class IluSolver : public Solver {
     rargs_.maxRecs = (int32_t)R.maxRecs;
     rargs_.metaOff = rmoff_.as<int64_t>();
     rargs_.dilu = cfg_->diagonalBased ? 1 : 0;
+    rargs_.dense = R.dense ? 1 : 0;
     rargs_.elemOff = relem_.as<uint32_t>();
     rargs_.hdrOff = rhoff_.as<int32_t>();
     rargs_.hdr = rhdr_.as<uint32_t>();

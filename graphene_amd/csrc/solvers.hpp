@@ -52,11 +52,12 @@ class Solver {
   // synchronizes before it hands results out.
   virtual void setDeferredSync(bool v) { (void)v; }
   // Batched solves: the matrix is block-diagonal with system k on rows
-  // systemRowStart[k] .. systemRowStart[k+1]-1; work of systems whose
-  // states[k].done is set may be skipped by applyDevice (optional).
-  virtual void setSystemSkip(const gph::DevState* states,
+  // systemRowStart[k] .. systemRowStart[k+1]-1; work of systems with no
+  // inner solve running (phases[k] != 0) may be skipped by applyDevice
+  // (optional).
+  virtual void setSystemSkip(const int32_t* phases,
                              const std::vector<int64_t>& systemRowStart) {
-    (void)states;
+    (void)phases;
     (void)systemRowStart;
   }
   virtual void applyDevice(const Vec& rhs, const Vec& out, const gph::PreOp& pre,

@@ -316,10 +316,15 @@ int gph_server_run(gph_server* s, int64_t n, const int32_t* system,
  * entry coupling two systems. json: IterativeRefinement (float64 extended,
  * float32 working type) around PBiCGStab, or PBiCGStab alone, each with an
  * ILU preconditioner; every system follows the single-system algorithm from
- * x = 0 with its own scalars and convergence. The inner iterations run as a
- * captured HIP graph; the ILU substitution kernel (and the packed kernel's
- * rows per wave) is picked by timing at creation unless the ILU
- * configuration sets "substitution" or "rowsPerWave". */
+ * x = 0 with its own scalars and convergence. The iterations run as a
+ * captured HIP graph replayed until every system has finished. With
+ * refinement the outer step (phi += d, FP64 residual, norm, convergence,
+ * inner restart) runs on the device too, per system as soon as its inner
+ * solve has finished; "outerStep": "host" in the top-level configuration
+ * instead ends every system's inner solve before the outer steps, which the
+ * host drives. The ILU substitution kernel (and the packed kernel's rows per
+ * wave) is picked by timing at creation unless the ILU configuration sets
+ * "substitution" or "rowsPerWave". */
 typedef struct gph_batch gph_batch;
 int gph_batch_create(const gph_matrix* A, int32_t n_systems,
                      const int32_t* system_tile_start, const char* json,

@@ -113,6 +113,30 @@ def test_batch_repeat_identical(batch):
     assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
 
 
+@pytest.mark.parametrize("sub", ["rows", "packed"])
+def test_batch_outer_step_device_equals_host(batch, sub):
+    """The refinement's outer step on the device (each system restarts its
+    inner solve as soon as its outer step is done) and driven by the host
+    (every system's inner solve ends first) give every system the same
+    operations, hence the same bits and iteration counts."""
+    B = batch
+    outs = []
+    for outer in ("device", "host"):
+        bt = g.Batch(B["A"], B["ts"], dict(mpir_cfg(sub=sub), outerStep=outer))
+        x = g.DeviceVector(B["h"].nrows, np.float64)
+        stats, _ = bt.solve(x, g.DeviceVector.from_numpy(B["b"]))
+        outs.append((x.numpy(), [(s.iterations, s.innerIterations, s.finalResidual)
+                                 for s in stats]))
+    assert outs[0][1] == outs[1][1]
+    assert np.array_equal(outs[0][0].view(np.uint64), outs[1][0].view(np.uint64))
+
+
+def test_batch_rejects_bad_outer_step(batch):
+    B = batch
+    with pytest.raises(g.GrapheneError, match="outerStep"):
+        g.Batch(B["A"], B["ts"], dict(mpir_cfg(), outerStep="gpu"))
+
+
 def test_batch_pbicgstab_only(batch):
     B = batch
     cfg = {"type": "PBiCGStab", "relResidual": 1e-6, "maxIterations": 500,
