@@ -98,6 +98,10 @@ _SIGS = {
     "gph_matrix_create": ([_P, C.c_int, C.c_int32, C.POINTER(_P)], C.c_int),
     "gph_matrix_destroy": ([_P], None),
     "gph_matrix_set_sliced": ([_P, C.c_int32], C.c_int),
+    "gph_matrix_set_stream": ([_P, _P, C.c_int32], C.c_int),
+    "gph_stream_create": ([C.POINTER(_P)], C.c_int),
+    "gph_stream_destroy": ([_P], C.c_int),
+    "gph_stream_synchronize": ([_P], C.c_int),
     "gph_matrix_device_bytes": ([_P, C.POINTER(C.c_size_t)], C.c_int),
     "gph_spmv": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int], C.c_int),
     "gph_residual": ([_P, C.c_int, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int, _P], C.c_int),
@@ -400,6 +404,24 @@ class DeviceVector:
         return out
 
 
+class Stream:
+    """A HIP stream (gph_stream_*), for ordering a matrix's work after other
+    work on the same stream."""
+
+    def __init__(self):
+        p = _P()
+        _check(lib().gph_stream_create(C.byref(p)))
+        self.ptr = p
+
+    def synchronize(self):
+        _check(lib().gph_stream_synchronize(self.ptr))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.gph_stream_destroy(self.ptr)
+            self.ptr = None
+
+
 class SolverStats:
     """lg/matrix/solver/SolverStats.hpp:29-69."""
 
@@ -500,6 +522,13 @@ class Matrix:
         b = C.c_size_t()
         _check(lib().gph_matrix_device_bytes(self._A, C.byref(b)))
         return b.value
+
+    def set_stream(self, stream, synchronous=True):
+        """Enqueue this matrix's work (and its solvers') on `stream` (a Stream
+        or None for the matrix's own); synchronous=False: spmv and residual
+        return once enqueued."""
+        _check(lib().gph_matrix_set_stream(self._A, stream.ptr if stream else None,
+                                           int(bool(synchronous))))
 
     def createUninitializedVector(self, dtype=np.float32, withHalo=False):
         """Matrix::createUninitializedVector (halo cells need no storage here)."""

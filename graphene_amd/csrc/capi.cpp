@@ -20,6 +20,7 @@ struct gph_host_matrix {
 };
 struct gph_matrix {
   std::unique_ptr<DeviceMatrix> A;
+  bool async = false;  // gph_spmv / gph_residual return without synchronizing
 };
 struct gph_server {
   std::unique_ptr<graphene::serve::Server> S;
@@ -430,6 +431,31 @@ int gph_matrix_create(const gph_host_matrix* h, gph_format fmt,
   });
 }
 void gph_matrix_destroy(gph_matrix* A) { delete A; }
+int gph_matrix_set_stream(gph_matrix* A, void* stream, int32_t synchronous) {
+  return guard([&] {
+    need(A, "matrix");
+    GPH_HIP(hipStreamSynchronize(A->A->stream()));  // work already enqueued ends first
+    A->A->setStream(static_cast<hipStream_t>(stream));
+    A->async = stream != nullptr && !synchronous;
+  });
+}
+int gph_stream_create(void** stream) {
+  return guard([&] {
+    need(stream, "stream");
+    hipStream_t s = nullptr;
+    GPH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+  });
+}
+int gph_stream_destroy(void* stream) {
+  return guard([&] {
+    need(stream, "stream");
+    GPH_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  });
+}
+int gph_stream_synchronize(void* stream) {
+  return guard([&] { GPH_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream))); });
+}
 int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced) {
   return guard([&] {
     need(A, "matrix");
@@ -453,7 +479,7 @@ int gph_spmv(const gph_matrix* A, gph_type value_type, gph_type x_type,
     hipStream_t s = A->A->stream();
     A->A->spmv(Vec{const_cast<void*>(x), dt(x_type)}, Vec{y, dt(y_type)},
                dt(acc_type), dt(value_type), s);
-    GPH_HIP(hipStreamSynchronize(s));
+    if (!A->async) GPH_HIP(hipStreamSynchronize(s));
   });
 }
 int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
@@ -468,7 +494,7 @@ int gph_residual(const gph_matrix* A, gph_type value_type, gph_type x_type,
     A->A->residual(Vec{const_cast<void*>(x), dt(x_type)},
                    Vec{const_cast<void*>(b), dt(b_type)}, Vec{r, dt(r_type)},
                    dt(acc_type), dt(value_type), s, r64);
-    GPH_HIP(hipStreamSynchronize(s));
+    if (!A->async) GPH_HIP(hipStreamSynchronize(s));
   });
 }
 int gph_vector_norm(const gph_matrix* A, gph_type t, const void* x,

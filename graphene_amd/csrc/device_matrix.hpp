@@ -36,7 +36,10 @@ class DeviceMatrix {
   int32_t n() const { return n_; }
   int64_t nnzOff() const { return nnz_; }
   int32_t numTiles() const { return T_; }
-  hipStream_t stream() const { return stream_; }
+  // the stream of the matrix's work and of the solvers built on it: the
+  // caller's (setStream) or the matrix's own
+  hipStream_t stream() const { return user_ ? user_ : stream_; }
+  void setStream(hipStream_t s) { user_ = s; }
 
   // Row-gather arguments for values stored in `vt` (copies in float16 /
   // float64 are made on first use).
@@ -88,6 +91,7 @@ class DeviceMatrix {
   int32_t n_ = 0, T_ = 0;
   int64_t nnz_ = 0;
   hipStream_t stream_ = nullptr;
+  hipStream_t user_ = nullptr;  // caller's stream (not owned)
 
   gph::DevBuf rowPtr_, col_, off32_, diag32_;
   mutable gph::DevBuf off64_, diag64_, off16_, diag16_;

@@ -186,6 +186,11 @@ int gph_memcpy_htod(void* dst, const void* src, size_t bytes);
 int gph_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int gph_memset(void* dst, int value, size_t bytes);
 int gph_synchronize(void);
+/* No reference counterpart: HIP streams for callers without HIP bindings
+ * (a void* is a hipStream_t). */
+int gph_stream_create(void** stream);
+int gph_stream_destroy(void* stream);
+int gph_stream_synchronize(void* stream);
 /* out = (out_type) in, n elements, one streaming kernel on the default stream
  * (Expression::cast, libgraphene/dsl/tensor/Expression.hpp:152). With
  * in_type == out_type it is a plain copy kernel. */
@@ -204,6 +209,15 @@ void gph_matrix_destroy(gph_matrix* A);
  * the form is chosen per matrix: sliced ELL, except for rows of strongly
  * varying length whose columns mostly span more than 2^16 (row blocks). */
 int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced);
+/* No reference counterpart (the reference runs on a Poplar engine): the
+ * stream on which A's operations and those of solvers created on A are
+ * enqueued, a caller's hipStream_t, or NULL for the matrix's own stream;
+ * A's work is then ordered after the caller's earlier work on that stream.
+ * synchronous = 0: gph_spmv and gph_residual return once their kernels are
+ * enqueued (the caller synchronizes the stream before reading y or r);
+ * every other call returns after its work is done. The stream must outlive
+ * its use by A. */
+int gph_matrix_set_stream(gph_matrix* A, void* stream, int32_t synchronous);
 int gph_matrix_device_bytes(const gph_matrix* A, size_t* bytes);
 
 /* y = A*x (Matrix::operator*, lg/matrix/Matrix.hpp:95-97; MatrixBase::spmv,

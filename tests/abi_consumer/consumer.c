@@ -4,7 +4,8 @@
  * 144-183: generate a Poisson matrix, copy it to the device, b = A x, solve).
  *
  *   consumer host   host-side calls only (no GPU needed)
- *   consumer gpu    also the device calls: SpMV and a PBiCGStab + ILU(0) solve
+ *   consumer gpu    also the device calls: SpMV, a PBiCGStab + ILU(0) solve,
+ *                   and an SpMV on a stream of the caller's
  *
  * Prints "consumer ok" and exits 0 on success. */
 #include <math.h>
@@ -92,6 +93,24 @@ int main(int argc, char** argv) {
     for (int64_t i = 0; i < n; i++) err = fmax(err, fabs(hx[i] - 1.0));
     if (!st.converged || st.singular || st.iterations < 1 || err > 1e-4)
       return fail("solve");
+    /* the same SpMV on a stream of the caller's, returning once enqueued */
+    void* stream = NULL;
+    CHECK(gph_stream_create(&stream));
+    CHECK(gph_matrix_set_stream(A, stream, 0));
+    CHECK(gph_memset(b, 0, (size_t)n * 4));
+    CHECK(gph_spmv(A, GPH_FLOAT32, GPH_FLOAT32, x, GPH_FLOAT32, b, GPH_FLOAT32));
+    CHECK(gph_stream_synchronize(stream));
+    CHECK(gph_memcpy_dtoh(hy, b, (size_t)n * 4));
+    CHECK(gph_matrix_set_stream(A, NULL, 1));
+    CHECK(gph_stream_destroy(stream));
+    /* x is the solution (1 to 1e-4): A x gives the row sums again */
+    for (int64_t k = 0; k < nz; k++)
+      for (int64_t j = 0; j < ny; j++)
+        for (int64_t i = 0; i < nx; i++) {
+          const int nb = (i > 0) + (i < nx - 1) + (j > 0) + (j < ny - 1) + (k > 0) + (k < nz - 1);
+          if (fabs(hy[(k * ny + j) * nx + i] - (6 - nb)) > 1e-2)
+            return fail("SpMV on the caller's stream");
+        }
     gph_solver_destroy(S);
     CHECK(gph_free(x));
     CHECK(gph_free(b));
