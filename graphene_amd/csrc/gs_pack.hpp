@@ -15,7 +15,11 @@
 // of a row this sweep has already updated (in-tile j < i forward, j > i
 // backward: read from the sweep's own copy); clear, the tile-major column
 // (the previous x). The entries of a row keep their stored order, which is
-// the order the sweep subtracts them in.
+// the order the sweep subtracts them in. Within a level the stream holds
+// the entries in gather order (previous-x columns ascending, then the
+// values this sweep wrote), each with its jagged-diagonal position in the
+// level (pos), where its product is staged: the gathers of neighbouring
+// lanes share cache lines, the subtraction order is unchanged.
 #pragma once
 
 #include <cstdint>
@@ -34,9 +38,11 @@ struct GsStream {
   std::vector<uint32_t> slot;    // per slot: local row << 8 | length
   std::vector<uint32_t> code;    // per entry: x source
   std::vector<int32_t> src;      // per entry: CRS position of its value
+  std::vector<uint16_t> pos;     // per entry: jagged-diagonal position in its level
   int32_t maxRows = 0;           // largest tile
   int32_t maxLevelEntries = 0;   // entries of the largest level
-  bool ok = false;  // false: rows longer than 255 or tiles over 2^24 rows, or over 32-bit offsets
+  bool ok = false;  // false: rows longer than 255, tiles over 2^24 rows, levels over 2^16
+                    // entries, or over 32-bit offsets
 };
 
 GsStream packGs(const host::HostMatrix& m, bool backward);

@@ -7,9 +7,11 @@
 // j < i (forward) or j > i (backward), the previous x otherwise (halo
 // included). One workgroup per tile, the values this sweep writes in LDS.
 // Per level (stream format in gs_pack.hpp) two phases: every thread takes
-// consecutive entries of the level's stream and stores its product
-// a_ij * x_j in LDS (x_j from LDS or the previous x; the stream loads are
-// coalesced and no lane idles on a short row); then one thread per row
+// consecutive entries of the level's stream (in gather order: previous-x
+// columns ascending) and stores its product a_ij * x_j at the entry's
+// position in LDS (x_j from LDS or the previous x; the stream loads are
+// coalesced, the gathers of neighbouring lanes share cache lines, and no
+// lane idles on a short row); then one thread per row
 // subtracts its products from b_i in the row's stored order and writes
 // x_i. The result is written to xout at the end (xout != xin).
 #include "kernels.hpp"
@@ -17,6 +19,9 @@
 namespace gph {
 
 static size_t gsMaxShared() { return 160 * 1024 - 4096; }
+
+// entries per thread and step of a level's product phase
+constexpr int kGsStep = 4;
 
 // LDS: the x values this sweep writes, the level's products, the level's
 // diagonal starts (kGsMaxWidth + 1 ints)
@@ -42,6 +47,7 @@ __global__ __launch_bounds__(kGsThreads) void gsChunkKernel(GsArgs a,
   const int tid = threadIdx.x;
   const float* __restrict__ val = a.val;
   const uint32_t* __restrict__ code = a.code;
+  const uint16_t* __restrict__ pos = a.pos;
   const int32_t* __restrict__ dg = a.diagStart;
   const int l0 = a.lvlBase[tile], nL = a.lvlBase[tile + 1] - l0;
   for (int L = 0; L < nL; L++) {
@@ -57,24 +63,26 @@ __global__ __launch_bounds__(kGsThreads) void gsChunkKernel(GsArgs a,
       di = cvt<X>(diag[i]);
     }
     if (tid <= lv.w) doff[tid] = dg[lv.z + tid] - e0;
-    // 4 entries per thread and step: the code and value loads, the x
+    // kGsStep entries per thread and step: the code and value loads, the x
     // reads, then the products (indices past the level read its last entry)
-    for (int e = e0 + tid; e < e1; e += 4 * kGsThreads) {
-      uint32_t c[4];
-      float v[4];
+    for (int e = e0 + tid; e < e1; e += kGsStep * kGsThreads) {
+      uint32_t c[kGsStep];
+      float v[kGsStep];
+      int ps[kGsStep];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < kGsStep; k++) {
         const int ek = min(e + k * kGsThreads, e1 - 1);
         c[k] = code[ek];
         v[k] = val[ek];
+        ps[k] = pos[ek];
       }
-      X xv[4];
+      X xv[kGsStep];
 #pragma unroll
-      for (int k = 0; k < 4; k++)
+      for (int k = 0; k < kGsStep; k++)
         xv[k] = (c[k] & kGsFreshBit) ? xs[c[k] & ~kGsFreshBit] : xin[c[k]];
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (e + k * kGsThreads < e1) prod[e + k * kGsThreads - e0] = cvt<X>(v[k]) * xv[k];
+      for (int k = 0; k < kGsStep; k++)
+        if (e + k * kGsThreads < e1) prod[ps[k]] = cvt<X>(v[k]) * xv[k];
     }
     __syncthreads();
     for (int q = tid; q < lv.y; q += kGsThreads) {

@@ -117,22 +117,28 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
 #pragma unroll
   for (int s = 0; s < S; s++) {
     const int32_t* __restrict__ ptr = a.ptr[s];
-    const int32_t* __restrict__ col = a.col[s];
-    const VT* __restrict__ val = static_cast<const VT*>(a.val[s]);
+    // column-ordered chunks (one stream): products go to their entry's
+    // position in the chunk
+    const bool srt = S == 1 && a.gcol != nullptr;
+    const int32_t* __restrict__ col = srt ? a.gcol : a.col[s];
+    const VT* __restrict__ val = static_cast<const VT*>(srt ? a.gval : a.val[s]);
+    const uint16_t* __restrict__ pos = a.gpos;
     const int e0 = ptr[r0], e1 = ptr[r1];
     const int mb = has ? ptr[row] : 0;
     const int me = has ? ptr[row + 1] : 0;
     for (int c0 = e0; c0 < e1; c0 += kSpmvCap) {
       const int cn = min(kSpmvCap, e1 - c0);
       constexpr int U = kSpmvCap / kSpmvThreads;
-      int ci[U];
+      int ci[U], pk[U];
       VT vv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int k = tid + u * kSpmvThreads;
+        pk[u] = k;
         if (k < cn) {
           ci[u] = col[c0 + k];
           vv[u] = val[c0 + k];
+          if (srt) pk[u] = pos[c0 + k];
         }
       }
       XT xv[U];
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(kSpmvThreads) void spmvKernel(SpmvArgs a) {
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int k = tid + u * kSpmvThreads;
-        if (k < cn) prod[k] = cvt<AT>(vv[u]) * (AT)xv[u];
+        if (k < cn) prod[pk[u]] = cvt<AT>(vv[u]) * (AT)xv[u];
       }
       __syncthreads();
       const int lo = max(mb, c0), hi = min(me, c0 + cn);

@@ -29,6 +29,13 @@ struct SpmvArgs {
   const int32_t* col[2] = {nullptr, nullptr};
   const void* val[2] = {nullptr, nullptr};
   const void* diag = nullptr;
+  // Row-block form, one stream (CRS): each staging chunk's entries ordered
+  // by column, with each entry's position in its chunk, so the x gathers
+  // of neighbouring lanes share cache lines; used instead of col[0] /
+  // val[0] for the products when set (the row sums keep the stored order)
+  const int32_t* gcol = nullptr;
+  const uint16_t* gpos = nullptr;
+  const void* gval = nullptr;
   const void* x = nullptr;
   const void* b = nullptr;      // residual right-hand side
   void* out = nullptr;
@@ -307,6 +314,7 @@ struct GsArgs {
   const uint32_t* slot = nullptr;      // per slot: tile-local row << 8 | length
   const uint32_t* code = nullptr;      // per entry: x source
   const float* val = nullptr;          // per entry: matrix value
+  const uint16_t* pos = nullptr;       // per entry: position of its product in the level
 };
 size_t gsStreamSharedBytes(const GsArgs& a, DType xt);
 void launchGsStream(const GsArgs& a, const void* diag, const void* b, const void* xin,

@@ -560,6 +560,7 @@ class GaussSeidelSolver : public Solver {
     buf[2].upload(g.diag.data(), g.diag.size(), s);
     buf[3].upload(g.slot.data(), g.slot.size(), s);
     buf[4].upload(g.code.data(), g.code.size(), s);
+    buf[6].upload(g.pos.data(), g.pos.size(), s);
     DevBuf idx;
     idx.upload(g.src.data(), g.src.size(), s);
     buf[5].alloc(g.src.size() * sizeof(float));
@@ -575,6 +576,7 @@ class GaussSeidelSolver : public Solver {
     a.slot = buf[3].as<uint32_t>();
     a.code = buf[4].as<uint32_t>();
     a.val = buf[5].as<float>();
+    a.pos = buf[6].as<uint16_t>();
     a.maxLevelEntries = g.maxLevelEntries;
     static_assert(gph::kGsMaxWidth == 255, "stream row length field");
     if (gph::gsStreamSharedBytes(a, DType::F32) > kMaxGsShared) a.T = 0;  // row-block kernel
@@ -584,7 +586,7 @@ class GaussSeidelSolver : public Solver {
   std::shared_ptr<gaussseidel::Configuration> cfg_;
   Workspace ws_;
   gph::GsArgs fwd_, bwd_;
-  DevBuf fbuf_[6], bbuf_[6];
+  DevBuf fbuf_[7], bbuf_[7];
 };
 
 // ---------------------------------------------------------------------------
