@@ -49,7 +49,8 @@ DeviceMatrix::DeviceMatrix(std::shared_ptr<const host::HostMatrix> host,
   // far columns) gather x in column order (4M power-law rows, blocks of up
   // to 256 rows: 173 us; in stored order 237 us with 64-row blocks, the
   // best size there, 284 us with 256)
-  if (!(sellCrs_.numSlices > 0 && sellCrs_.preferred)) buildColumnOrder(br);
+  if (fmt_ == MatrixFormat::CRS && !(sellCrs_.numSlices > 0 && sellCrs_.preferred))
+    buildColumnOrder(br);
 
   if (fmt_ == MatrixFormat::LDU) {
     host::LDUHost L = host::buildLDU(h);

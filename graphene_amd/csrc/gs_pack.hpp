@@ -22,6 +22,7 @@
 // lanes share cache lines, the subtraction order is unchanged.
 #pragma once
 
+#include <climits>
 #include <cstdint>
 #include <vector>
 
@@ -45,6 +46,9 @@ struct GsStream {
                     // entries, or over 32-bit offsets
 };
 
-GsStream packGs(const host::HostMatrix& m, bool backward);
+// Levels of more than maxLevelEntries entries are split into consecutive
+// levels of at most that many (or of one row).
+GsStream packGs(const host::HostMatrix& m, bool backward,
+                int64_t maxLevelEntries = INT64_MAX);
 
 }  // namespace graphene::matrix

@@ -552,7 +552,13 @@ class GaussSeidelSolver : public Solver {
   void buildStream(bool backward, gph::GsArgs& a, DevBuf* buf) {
     if ((size_t)(A_.tileArgs().maxTileRows + 1) * 8 > kMaxGsShared) return;
     static_assert(kGsFresh == gph::kGsFreshBit, "stream format");
-    GsStream g = packGs(A_.hostMatrix(), backward);
+    // levels capped so that 8 workgroups (the most of 256 threads a CU
+    // holds) fit a CU's LDS with float32 vectors: no tile of the last round
+    // of a 16-tiles-per-CU launch waits for an LDS slot
+    const int64_t maxRows = A_.tileArgs().maxTileRows;
+    const int64_t perBlock = 160 * 1024 / 8 - (gph::kGsMaxWidth + 1) * 4;
+    const int64_t cap = perBlock / 4 - maxRows >= 512 ? perBlock / 4 - maxRows : INT64_MAX;
+    GsStream g = packGs(A_.hostMatrix(), backward, cap);
     if (!g.ok) return;
     hipStream_t s = A_.stream();
     buf[0].upload(g.lvlBase.data(), g.lvlBase.size(), s);

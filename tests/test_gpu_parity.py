@@ -101,6 +101,34 @@ def test_spmv_wide_and_narrow_slices(name):
         assert np.array_equal((A_ldu * dev(x)).numpy(), o.ldu_spmv_f32(U, x))
 
 
+def test_spmv_column_ordered_row_blocks():
+    """An irregular matrix (power-law, 600,000 rows, 10 % uniformly random
+    columns) runs the row-block kernel with its chunks in column order:
+    bit-exact against the oracle with FP32, FP64 and FP16 values, FP64 x and
+    the mixed residual, on the matrix's own stream and on a caller stream."""
+    h, L = powerlaw_case(600000, T=600)
+    A = h.copyToTile(g.CRS)
+    x = o.uniform_pm1(41, h.nrows)
+    x64 = x.astype(np.float64) * 1.0000001
+    b = o.uniform_pm1(42, h.nrows)
+    L16 = dict(L, off=L["off"].astype(np.float16).astype(np.float32),
+               diag=L["diag"].astype(np.float16).astype(np.float32))
+    ref32, ref64 = o.spmv(L, x), o.spmv(L, x64)
+    refr32, refr64 = o.residual_mixed(L, x64 / 3.0, b)
+    ref16 = o.spmv(L16, x)
+    st = g.Stream()
+    for stream in (None, st):
+        A.set_stream(stream)
+        assert np.array_equal((A * dev(x)).numpy().view(np.uint32), ref32.view(np.uint32))
+        assert np.array_equal(A.spmv(dev(x64)).numpy(), ref64)
+        assert np.array_equal(A.spmv(dev(x), value_type=g.FLOAT16).numpy(), ref16)
+        r64 = g.DeviceVector(h.nrows, np.float64)
+        r = A.residual(dev(x64 / 3.0), dev(b), destType=g.FLOAT32,
+                       intermediateType=g.FLOAT64, r64=r64)
+        assert np.array_equal(r.numpy(), refr32) and np.array_equal(r64.numpy(), refr64)
+    A.set_stream(None)
+
+
 def test_spmv_f64(case):
     name, h, L, A = case
     x = o.uniform_pm1(12, h.nrows).astype(np.float64) * 1.0000001
