@@ -212,7 +212,23 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
   using In = RowIn<XT, BT, MODE, RED>;
   RedAcc<AT, RED> acc;
   constexpr int CH = sellChunk<AT, MODE, RED>();
-  for (int c0 = g0; c0 < g1; c0 += CH) {
+  // SKIP: the block's groups in rounds of 64 (256 slices); each thread
+  // first tests one slice of the round (are any of its systems in the run
+  // phase), all at once, and a round without such a slice is passed over
+  __shared__ uint8_t runLds[SKIP ? kSpmvThreads : 1];
+  const int rg = SKIP ? kSpmvThreads / W : g1 - g0;
+  for (int r0 = g0; r0 < g1; r0 += rg) {
+  const int r1 = min(g1, r0 + rg);
+  if constexpr (SKIP) {
+    const int sl = r0 * W + (int)threadIdx.x;
+    bool run = false;
+    if (sl < r1 * W && sl < a.numSlices)
+      for (int k = a.sliceSys[2 * sl]; k <= a.sliceSys[2 * sl + 1]; k++)
+        run = run || a.sysPhase[k] == a.runPhase;
+    runLds[threadIdx.x] = run ? 1 : 0;
+    if (!__syncthreads_or(run ? 1 : 0)) continue;
+  }
+  for (int c0 = r0; c0 < r1; c0 += CH) {
     int rows[CH];
     bool hass[CH];
     AT sums[CH];
@@ -221,13 +237,8 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
     for (int i = 0; i < CH; i++) {
       const int g = c0 + i;
       const int slice = g * W + wave;
-      bool sliceOk = g < g1 && slice < a.numSlices;
-      if (SKIP && sliceOk) {  // batched: only slices of systems in the run phase
-        bool run = false;
-        for (int k = a.sliceSys[2 * slice]; k <= a.sliceSys[2 * slice + 1]; k++)
-          run = run || a.sysPhase[k] == a.runPhase;
-        sliceOk = run;
-      }
+      bool sliceOk = g < r1 && slice < a.numSlices;
+      if (SKIP && sliceOk) sliceOk = runLds[slice - r0 * W] != 0;
       const int slot = slice * 64 + lane;
       int row = 0, len = 0;
       if (sliceOk) {
@@ -280,6 +291,8 @@ __global__ __launch_bounds__(kSpmvThreads) void sellKernel(SpmvArgs a) {
 #pragma unroll
     for (int i = 0; i < CH; i++)
       spmvRowOut<XT, BT, OT, AT, MODE, RED, true>(a, rows[i], hass[i], sums[i], ins[i], acc);
+  }
+  if constexpr (SKIP) __syncthreads();  // the round's flags are read
   }
   spmvFinish<AT, RED, kSpmvThreads>(a, acc, redLds);
 }
