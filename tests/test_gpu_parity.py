@@ -274,6 +274,36 @@ def test_gauss_seidel_bit_exact(case, symmetric):
     assert np.array_equal(x.numpy(), ref), name
 
 
+def test_gauss_seidel_split_levels():
+    """One 3000-row power-law tile: its largest forward levels hold more
+    entries than the sweep kernel stages at once (4864 float32 slots less
+    the tile's 3000 rows), so they are streamed as several consecutive
+    levels; the sweeps stay bit-exact against the oracle."""
+    h, L = powerlaw_case(3000, T=1)
+    rp, cl = L["row_ptr"], L["col_local"]
+    lvl = np.zeros(h.nrows, np.int64)
+    for i in range(h.nrows):
+        lo = cl[rp[i]:rp[i + 1]]
+        lo = lo[lo < i]
+        lvl[i] = 1 + lvl[lo].max() if len(lo) else 0
+    per_level = np.bincount(lvl, weights=np.diff(rp))
+    assert per_level.max() > 4864 - h.nrows
+    A = h.copyToTile()
+    b = o.uniform_pm1(18, h.nrows)
+    x0 = o.uniform_pm1(19, h.nrows) * 0.5
+    for symmetric in (False, True):
+        S = g.Solver(A, {"type": "GaussSeidel", "numFixedIterations": 2,
+                         "symmetric": symmetric})
+        x = dev(x0)
+        S.solve(x, dev(b))
+        ref = x0.copy()
+        for _ in range(2):
+            ref = o.gs_sweep(L, b, ref)
+            if symmetric:
+                ref = o.gs_sweep(L, b, ref, backward=True)
+        assert np.array_equal(x.numpy(), ref), symmetric
+
+
 def _pbicgstab_cfg(rel, precond="ILU", working="float32", maxit=1000):
     c = {"type": "PBiCGStab", "relResidual": rel, "maxIterations": maxit,
          "workingType": working}
