@@ -207,7 +207,9 @@ void gph_matrix_destroy(gph_matrix* A);
 /* No reference counterpart: kernel form of the row gather, 1 = sliced ELL,
  * 0 = row blocks staged in LDS. Both give identical results. Without a call
  * the form is chosen per matrix: sliced ELL, except for rows of strongly
- * varying length whose columns mostly span more than 2^16 (row blocks). */
+ * varying length whose columns mostly span more than 2^16 (row blocks; a
+ * CRS matrix of that kind also keeps its entries in column order per row
+ * block, 10 more bytes per off-diagonal). */
 int gph_matrix_set_sliced(gph_matrix* A, int32_t sliced);
 /* No reference counterpart (the reference runs on a Poplar engine): the
  * stream on which A's operations and those of solvers created on A are
