@@ -10,6 +10,7 @@
 #include <sstream>
 
 #include "gph_common.hpp"
+#include "parallel.hpp"
 
 namespace graphene::matrix::host {
 
@@ -508,52 +509,118 @@ static GlobalCRS convertToCRS(TripletMatrix&& coo) {
   if (coo.ncols != n) fail(kInvalid, "matrix must be square");
   if (n <= 0) fail(kInvalid, "matrix must have at least one row");
   if (n >= (int64_t)INT32_MAX) fail(kInvalid, "too many rows for int32 ids");
-  const size_t ne = coo.size();
+  const int64_t ne = (int64_t)coo.size();
+  const int64_t* rows = coo.rows.data();
+  const int64_t* cols = coo.cols.data();
+  // index checks, and whether the triplets are already in row order (the
+  // generators and unions of them emit them so)
+  constexpr int64_t kGrain = 1 << 20;
+  const int64_t nch = (ne + kGrain - 1) / kGrain;
+  std::vector<uint8_t> unsortedChunk((size_t)std::max<int64_t>(nch, 1), 0);
+  gph::parallelFor(nch, 1, [&](int64_t k) {
+    const int64_t b = k * kGrain, e = std::min(ne, b + kGrain);
+    for (int64_t i = b; i < e; i++) {
+      if (rows[i] < 0 || rows[i] >= n || cols[i] < 0 || cols[i] >= n)
+        fail(kInvalid, "triplet index out of range");
+      if (i > 0 && rows[i] < rows[i - 1]) unsortedChunk[(size_t)k] = 1;
+    }
+  });
+  const bool sorted =
+      std::find(unsortedChunk.begin(), unsortedChunk.end(), 1) == unsortedChunk.end();
+  // cnt[r]: first triplet of row r in `ord` (the identity when sorted)
   std::vector<int64_t> cnt((size_t)n + 1, 0);
-  for (size_t e = 0; e < ne; e++) {
-    int64_t r = coo.rows[e], c = coo.cols[e];
-    if (r < 0 || r >= n || c < 0 || c >= n)
-      fail(kInvalid, "triplet index out of range");
-    cnt[(size_t)r + 1]++;
-  }
-  for (int64_t r = 0; r < n; r++) cnt[r + 1] += cnt[r];
-  std::vector<int64_t> ord(ne);
-  {
+  std::vector<int64_t> ord;
+  if (sorted) {
+    gph::parallelFor(nch, 1, [&](int64_t k) {
+      const int64_t b = k * kGrain, e = std::min(ne, b + kGrain);
+      for (int64_t i = b; i < e; i++) {
+        const int64_t prev = i == 0 ? -1 : rows[i - 1];
+        for (int64_t r = prev + 1; r <= rows[i]; r++) cnt[(size_t)r] = i;
+      }
+    });
+    for (int64_t r = ne > 0 ? rows[ne - 1] + 1 : 0; r <= n; r++) cnt[(size_t)r] = ne;
+  } else {
+    for (int64_t e = 0; e < ne; e++) cnt[(size_t)rows[e] + 1]++;
+    for (int64_t r = 0; r < n; r++) cnt[(size_t)r + 1] += cnt[(size_t)r];
+    ord.resize((size_t)ne);
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-    for (size_t e = 0; e < ne; e++) ord[(size_t)fill[coo.rows[e]]++] = (int64_t)e;
+    for (int64_t e = 0; e < ne; e++) ord[(size_t)fill[(size_t)rows[e]]++] = e;
   }
+  // per row: entries in column order (stable), the diagonal summed apart,
+  // duplicates summed; row ranges in parallel, then concatenated
   GlobalCRS g;
   g.rowPtr.assign((size_t)n + 1, 0);
   g.diag.assign((size_t)n, 0.0f);
-  g.col.reserve(ne);
-  g.off.reserve(ne);
-  std::vector<std::pair<int64_t, float>> rowE;
-  for (int64_t r = 0; r < n; r++) {
-    rowE.clear();
-    for (int64_t q = cnt[r]; q < cnt[r + 1]; q++) {
-      int64_t e = ord[(size_t)q];
-      rowE.emplace_back(coo.cols[e], coo.vals[e]);
-    }
-    std::stable_sort(rowE.begin(), rowE.end(),
-                     [](const auto& a, const auto& b) { return a.first < b.first; });
-    for (size_t q = 0; q < rowE.size(); q++) {
-      int64_t c = rowE[q].first;
-      float v = rowE[q].second;
-      if (c == r) {
-        g.diag[(size_t)r] = g.diag[(size_t)r] + v;
-        continue;
+  constexpr int64_t kRowGrain = 1 << 14;
+  const int64_t nrc = (n + kRowGrain - 1) / kRowGrain;
+  struct Piece {
+    std::vector<int32_t> col;
+    std::vector<float> off;
+  };
+  std::vector<Piece> pieces((size_t)nrc);
+  gph::parallelFor(nrc, 1, [&](int64_t k) {
+    const int64_t r0 = k * kRowGrain, r1 = std::min(n, r0 + kRowGrain);
+    Piece& pc = pieces[(size_t)k];
+    pc.col.reserve((size_t)(cnt[(size_t)r1] - cnt[(size_t)r0]));
+    pc.off.reserve((size_t)(cnt[(size_t)r1] - cnt[(size_t)r0]));
+    std::vector<std::pair<int64_t, float>> rowE;
+    for (int64_t r = r0; r < r1; r++) {
+      rowE.clear();
+      for (int64_t q = cnt[(size_t)r]; q < cnt[(size_t)r + 1]; q++) {
+        const int64_t e = sorted ? q : ord[(size_t)q];
+        rowE.emplace_back(cols[e], coo.vals[(size_t)e]);
       }
-      if ((int64_t)g.col.size() > g.rowPtr[(size_t)r] && g.col.back() == c) {
-        g.off.back() = g.off.back() + v;
-        continue;
+      // stable: insertion for short rows (no allocation), merge sort beyond
+      if (rowE.size() <= 64) {
+        for (size_t q = 1; q < rowE.size(); q++) {
+          const auto x = rowE[q];
+          size_t z = q;
+          for (; z > 0 && rowE[z - 1].first > x.first; z--) rowE[z] = rowE[z - 1];
+          rowE[z] = x;
+        }
+      } else {
+        std::stable_sort(rowE.begin(), rowE.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
       }
-      g.col.push_back((int32_t)c);
-      g.off.push_back(v);
+      const size_t rowStart = pc.col.size();
+      for (size_t q = 0; q < rowE.size(); q++) {
+        const int64_t c = rowE[q].first;
+        const float v = rowE[q].second;
+        if (c == r) {
+          g.diag[(size_t)r] = g.diag[(size_t)r] + v;
+          continue;
+        }
+        if (pc.col.size() > rowStart && pc.col.back() == c) {
+          pc.off.back() = pc.off.back() + v;
+          continue;
+        }
+        pc.col.push_back((int32_t)c);
+        pc.off.push_back(v);
+      }
+      g.rowPtr[(size_t)r + 1] = (int64_t)(pc.col.size() - rowStart);
     }
-    g.rowPtr[(size_t)r + 1] = (int64_t)g.col.size();
-  }
-  if (g.rowPtr[(size_t)n] >= (int64_t)INT32_MAX)
+  });
+  std::vector<int64_t> base((size_t)nrc + 1, 0);
+  for (int64_t k = 0; k < nrc; k++)
+    base[(size_t)k + 1] = base[(size_t)k] + (int64_t)pieces[(size_t)k].col.size();
+  if (base[(size_t)nrc] >= (int64_t)INT32_MAX)
     fail(kInvalid, "too many non-zeros for int32 offsets");
+  g.col.resize((size_t)base[(size_t)nrc]);
+  g.off.resize((size_t)base[(size_t)nrc]);
+  gph::parallelFor(nrc, 1, [&](int64_t k) {
+    const int64_t r0 = k * kRowGrain, r1 = std::min(n, r0 + kRowGrain);
+    Piece& pc = pieces[(size_t)k];
+    std::copy(pc.col.begin(), pc.col.end(), g.col.begin() + base[(size_t)k]);
+    std::copy(pc.off.begin(), pc.off.end(), g.off.begin() + base[(size_t)k]);
+    // row lengths to offsets (rowPtr[r0] is the piece's base)
+    int64_t acc = base[(size_t)k];
+    for (int64_t r = r0; r < r1; r++) {
+      acc += g.rowPtr[(size_t)r + 1];
+      g.rowPtr[(size_t)r + 1] = acc;
+    }
+    Piece().col.swap(pc.col);
+    Piece().off.swap(pc.off);
+  });
   coo = TripletMatrix();
   return g;
 }
@@ -575,6 +642,7 @@ static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
   m.numSeparatorRegions.assign((size_t)T, 0);
   m.haloStart.assign((size_t)T + 1, 0);
   m.haloGlobal.clear();
+  const int32_t* rowTile = m.rowToTile.data();
 
   if (T == 1) {
     // CRSHostMatrix.cpp:151-158: a single tile holds every row, all interior.
@@ -589,141 +657,178 @@ static void computeLayout(HostMatrix& m, const GlobalCRS& g) {
     };
     std::vector<std::vector<int64_t>> interior((size_t)T);
     std::vector<std::vector<Region>> seps((size_t)T);
-    std::vector<std::map<std::vector<int32_t>, int32_t>> lookup((size_t)T);
     // Destination tiles of a row: tiles owning its columns
     // (CRSHostMatrix.cpp:289-299) united with tiles whose rows reference it;
     // identical for structurally symmetric patterns, and well defined for
     // nonsymmetric ones (where the reference's local addressing throws).
+    // The referencing tiles of each row (a transpose; their order within a
+    // row does not matter, the destination set is sorted below).
     std::vector<int64_t> refPtr((size_t)n + 1, 0);
-    for (int64_t r = 0; r < n; r++)
-      for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++)
-        if (m.rowToTile[(size_t)g.col[(size_t)a]] != m.rowToTile[(size_t)r])
-          refPtr[(size_t)g.col[(size_t)a] + 1]++;
+    constexpr int64_t kRows = 1 << 14;
+    const int64_t nrc = (n + kRows - 1) / kRows;
+    gph::parallelFor(nrc, 1, [&](int64_t k) {
+      for (int64_t r = k * kRows; r < std::min(n, (k + 1) * kRows); r++)
+        for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
+          const int32_t c = g.col[(size_t)a];
+          if (rowTile[c] != rowTile[r])
+            __atomic_fetch_add(&refPtr[(size_t)c + 1], 1, __ATOMIC_RELAXED);
+        }
+    });
     for (int64_t r = 0; r < n; r++) refPtr[(size_t)r + 1] += refPtr[(size_t)r];
     std::vector<int32_t> refTile((size_t)refPtr[(size_t)n]);
     {
       std::vector<int64_t> fill(refPtr.begin(), refPtr.end() - 1);
-      for (int64_t r = 0; r < n; r++)
+      gph::parallelFor(nrc, 1, [&](int64_t k) {
+        for (int64_t r = k * kRows; r < std::min(n, (k + 1) * kRows); r++)
+          for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
+            const int32_t c = g.col[(size_t)a];
+            if (rowTile[c] != rowTile[r])
+              refTile[(size_t)__atomic_fetch_add(&fill[(size_t)c], 1, __ATOMIC_RELAXED)] =
+                  rowTile[r];
+          }
+      });
+    }
+    // rows of each tile in ascending order
+    std::vector<int64_t> tileRowPtr((size_t)T + 1, 0), tileRows((size_t)n);
+    for (int64_t r = 0; r < n; r++) tileRowPtr[(size_t)rowTile[r] + 1]++;
+    for (int32_t t = 0; t < T; t++) tileRowPtr[(size_t)t + 1] += tileRowPtr[(size_t)t];
+    {
+      std::vector<int64_t> fill(tileRowPtr.begin(), tileRowPtr.end() - 1);
+      for (int64_t r = 0; r < n; r++) tileRows[(size_t)fill[(size_t)rowTile[r]]++] = r;
+    }
+    // per tile: interior rows, then separator regions (rows with the same
+    // destination set) in order of first appearance
+    gph::parallelFor(T, 1, [&](int64_t tt) {
+      const int32_t t = (int32_t)tt;
+      std::map<std::vector<int32_t>, int32_t> lookup;
+      std::vector<int32_t> key;
+      for (int64_t q = tileRowPtr[(size_t)t]; q < tileRowPtr[(size_t)t + 1]; q++) {
+        const int64_t r = tileRows[(size_t)q];
+        key.clear();
         for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
-          int64_t c = g.col[(size_t)a];
-          if (m.rowToTile[(size_t)c] != m.rowToTile[(size_t)r])
-            refTile[(size_t)fill[(size_t)c]++] = m.rowToTile[(size_t)r];
+          const int32_t ct = rowTile[g.col[(size_t)a]];
+          if (ct != t) key.push_back(ct);
         }
-    }
-    std::vector<int32_t> key;
-    for (int64_t r = 0; r < n; r++) {
-      int32_t t = m.rowToTile[(size_t)r];
-      key.clear();
-      for (int64_t a = g.rowPtr[(size_t)r]; a < g.rowPtr[(size_t)r + 1]; a++) {
-        int32_t ct = m.rowToTile[(size_t)g.col[(size_t)a]];
-        if (ct != t) key.push_back(ct);
+        for (int64_t z = refPtr[(size_t)r]; z < refPtr[(size_t)r + 1]; z++)
+          key.push_back(refTile[(size_t)z]);
+        if (key.empty()) {
+          interior[(size_t)t].push_back(r);
+          continue;
+        }
+        std::sort(key.begin(), key.end());
+        key.erase(std::unique(key.begin(), key.end()), key.end());
+        auto it = lookup.find(key);
+        int32_t reg;
+        if (it == lookup.end()) {
+          reg = (int32_t)seps[(size_t)t].size();
+          seps[(size_t)t].push_back(Region{key, {}});
+          lookup.emplace(key, reg);
+        } else {
+          reg = it->second;
+        }
+        seps[(size_t)t][(size_t)reg].cells.push_back(r);
       }
-      for (int64_t q = refPtr[(size_t)r]; q < refPtr[(size_t)r + 1]; q++)
-        key.push_back(refTile[(size_t)q]);
-      if (key.empty()) {
-        interior[(size_t)t].push_back(r);
-        continue;
-      }
-      std::sort(key.begin(), key.end());
-      key.erase(std::unique(key.begin(), key.end()), key.end());
-      auto& lk = lookup[(size_t)t];
-      auto it = lk.find(key);
-      int32_t reg;
-      if (it == lk.end()) {
-        reg = (int32_t)seps[(size_t)t].size();
-        seps[(size_t)t].push_back(Region{key, {}});
-        lk.emplace(key, reg);
-      } else {
-        reg = it->second;
-      }
-      seps[(size_t)t][(size_t)reg].cells.push_back(r);
-    }
-    std::vector<std::vector<int64_t>> halo((size_t)T);
+    });
+    // halo of tile d: the regions that list d, by source tile, then region
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> into((size_t)T);
     for (int32_t s = 0; s < T; s++)
-      for (auto& R : seps[(size_t)s])
-        for (int32_t d : R.dst)
-          halo[(size_t)d].insert(halo[(size_t)d].end(), R.cells.begin(),
-                                 R.cells.end());
-    int64_t pos = 0;
+      for (int32_t reg = 0; reg < (int32_t)seps[(size_t)s].size(); reg++)
+        for (int32_t d : seps[(size_t)s][(size_t)reg].dst) into[(size_t)d].push_back({s, reg});
+    std::vector<int64_t> haloSize((size_t)T, 0);
+    for (int32_t d = 0; d < T; d++)
+      for (const auto& sr : into[(size_t)d])
+        haloSize[(size_t)d] += (int64_t)seps[(size_t)sr.first][(size_t)sr.second].cells.size();
+    int64_t pos = 0, h = 0;
     for (int32_t t = 0; t < T; t++) {
       m.tileStart[(size_t)t] = pos;
       m.numInterior[(size_t)t] = (int32_t)interior[(size_t)t].size();
       m.numSeparatorRegions[(size_t)t] = (int32_t)seps[(size_t)t].size();
-      for (int64_t r : interior[(size_t)t]) {
-        m.newToOld[(size_t)pos] = r;
-        m.oldToNew[(size_t)r] = pos++;
-      }
-      for (auto& R : seps[(size_t)t])
-        for (int64_t r : R.cells) {
-          m.newToOld[(size_t)pos] = r;
-          m.oldToNew[(size_t)r] = pos++;
-        }
+      pos += tileRowPtr[(size_t)t + 1] - tileRowPtr[(size_t)t];
       if (m.tileStart[(size_t)t] == pos)
         fail(kInvalid, "tile " + std::to_string(t) + " holds no rows");
+      m.haloStart[(size_t)t] = h;
+      h += haloSize[(size_t)t];
     }
     m.tileStart[(size_t)T] = pos;
-    int64_t h = 0;
-    for (int32_t t = 0; t < T; t++) {
-      m.haloStart[(size_t)t] = h;
-      m.haloGlobal.insert(m.haloGlobal.end(), halo[(size_t)t].begin(),
-                          halo[(size_t)t].end());
-      h += (int64_t)halo[(size_t)t].size();
-    }
     m.haloStart[(size_t)T] = h;
+    m.haloGlobal.resize((size_t)h);
+    gph::parallelFor(T, 1, [&](int64_t tt) {
+      const size_t t = (size_t)tt;
+      int64_t p = m.tileStart[t];
+      for (int64_t r : interior[t]) {
+        m.newToOld[(size_t)p] = r;
+        m.oldToNew[(size_t)r] = p++;
+      }
+      for (auto& R : seps[t])
+        for (int64_t r : R.cells) {
+          m.newToOld[(size_t)p] = r;
+          m.oldToNew[(size_t)r] = p++;
+        }
+      int64_t q = m.haloStart[t];
+      for (const auto& sr : into[t])
+        for (int64_t r : seps[(size_t)sr.first][(size_t)sr.second].cells)
+          m.haloGlobal[(size_t)q++] = r;
+    });
   }
 
-  // local addressing + values in local column order
+  // local addressing + values in local column order, tile by tile: row
+  // offsets first (row lengths in the new order), then each tile's rows
+  // with its halo ids looked up in a sorted copy of its halo list
   const int64_t nnz = g.rowPtr[(size_t)n];
   m.rowPtr.assign((size_t)n + 1, 0);
   m.col.resize((size_t)nnz);
   m.colLocal.resize((size_t)nnz);
   m.offDiag.resize((size_t)nnz);
   m.diag.resize((size_t)n);
-  std::vector<int64_t> haloLocal;
-  if (T > 1) haloLocal.assign((size_t)n, -1);
+  for (int64_t p = 0; p < n; p++) {
+    const int64_t gr = m.newToOld[(size_t)p];
+    m.rowPtr[(size_t)p + 1] = m.rowPtr[(size_t)p] + (g.rowPtr[(size_t)gr + 1] - g.rowPtr[(size_t)gr]);
+  }
   struct E {
     int64_t local;
     int32_t pos;
     float v;
   };
-  std::vector<E> es;
-  int64_t c = 0;
-  for (int32_t t = 0; t < T; t++) {
+  gph::parallelFor(T, 1, [&](int64_t tt) {
+    const int32_t t = (int32_t)tt;
     const int64_t t0 = m.tileStart[(size_t)t];
     const int64_t nt = m.tileStart[(size_t)t + 1] - t0;
-    if (T > 1)
+    std::vector<std::pair<int64_t, int64_t>> halo;  // (global row, local id)
+    if (T > 1) {
       for (int64_t q = m.haloStart[(size_t)t]; q < m.haloStart[(size_t)t + 1]; q++)
-        haloLocal[(size_t)m.haloGlobal[(size_t)q]] = nt + (q - m.haloStart[(size_t)t]);
+        halo.push_back({m.haloGlobal[(size_t)q], nt + (q - m.haloStart[(size_t)t])});
+      std::sort(halo.begin(), halo.end());
+    }
+    std::vector<E> es;
     for (int64_t lr = 0; lr < nt; lr++) {
       const int64_t p = t0 + lr;
       const int64_t gr = m.newToOld[(size_t)p];
       m.diag[(size_t)p] = g.diag[(size_t)gr];
       es.clear();
       for (int64_t a = g.rowPtr[(size_t)gr]; a < g.rowPtr[(size_t)gr + 1]; a++) {
-        int64_t gc = g.col[(size_t)a];
-        int64_t local;
-        if (T == 1 || m.rowToTile[(size_t)gc] == t)
+        const int64_t gc = g.col[(size_t)a];
+        int64_t local = -1;
+        if (T == 1 || rowTile[gc] == t) {
           local = m.oldToNew[(size_t)gc] - t0;
-        else
-          local = haloLocal[(size_t)gc];
+        } else {
+          auto it = std::lower_bound(halo.begin(), halo.end(), std::make_pair(gc, (int64_t)-1));
+          if (it != halo.end() && it->first == gc) local = it->second;
+        }
         if (local < 0) fail(kRuntime, "internal: column missing from the halo");
         es.push_back(E{local, (int32_t)m.oldToNew[(size_t)gc], g.off[(size_t)a]});
       }
       if (T > 1)
         std::sort(es.begin(), es.end(),
                   [](const E& x, const E& y) { return x.local < y.local; });
+      int64_t c = m.rowPtr[(size_t)p];
       for (const E& e : es) {
         m.col[(size_t)c] = e.pos;
         m.colLocal[(size_t)c] = (int32_t)e.local;
         m.offDiag[(size_t)c] = e.v;
         c++;
       }
-      m.rowPtr[(size_t)p + 1] = c;
     }
-    if (T > 1)
-      for (int64_t q = m.haloStart[(size_t)t]; q < m.haloStart[(size_t)t + 1]; q++)
-        haloLocal[(size_t)m.haloGlobal[(size_t)q]] = -1;
-  }
+  });
 }
 
 // Level sets (CRSHostMatrix.cpp:527-675): forward level of a row = 1 + the
@@ -744,9 +849,11 @@ static void computeLevels(HostMatrix& m) {
   m.upperColorStartOffset.assign((size_t)T + 1, 0);
   m.numUpperColors.assign((size_t)T, 0);
   std::vector<int32_t> ulev((size_t)n, 0);
-  int64_t parRows = 0;
-  std::vector<int64_t> cnt;
-  for (int32_t t = 0; t < T; t++) {
+  // per tile (independent): levels, rows grouped by level, level starts
+  std::vector<std::vector<int64_t>> cStart((size_t)T), uStart((size_t)T);
+  std::vector<int64_t> parTile((size_t)T, 0);
+  gph::parallelFor(T, 1, [&](int64_t tt) {
+    const int32_t t = (int32_t)tt;
     const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
     int32_t maxL = 0;
     for (int64_t p = t0; p < t1; p++) {
@@ -770,13 +877,14 @@ static void computeLevels(HostMatrix& m) {
       ulev[(size_t)p] = L;
       maxU = std::max(maxU, L);
     }
+    std::vector<int64_t> cnt;
     auto group = [&](const std::vector<int32_t>& lv, int32_t maxLev,
                      std::vector<int32_t>& rows, std::vector<int64_t>& starts,
                      bool countPar) {
       cnt.assign((size_t)maxLev + 2, 0);
       for (int64_t p = t0; p < t1; p++) cnt[(size_t)lv[(size_t)p] + 1]++;
       for (int32_t L = 0; L <= maxLev; L++) {
-        if (countPar && cnt[(size_t)L + 1] > 12) parRows += cnt[(size_t)L + 1];
+        if (countPar && cnt[(size_t)L + 1] > 12) parTile[(size_t)t] += cnt[(size_t)L + 1];
         cnt[(size_t)L + 1] += cnt[(size_t)L];
       }
       for (int32_t L = 0; L <= maxLev + 1; L++) starts.push_back(cnt[(size_t)L]);
@@ -784,12 +892,19 @@ static void computeLevels(HostMatrix& m) {
       for (int64_t p = t0; p < t1; p++)
         rows[(size_t)(t0 + fill[(size_t)lv[(size_t)p]]++)] = (int32_t)(p - t0);
     };
-    m.colorStartOffset[(size_t)t] = (int64_t)m.colorStart.size();
     m.numColors[(size_t)t] = maxL + 1;
-    group(m.level, maxL, m.colorSortAddr, m.colorStart, true);
-    m.upperColorStartOffset[(size_t)t] = (int64_t)m.upperColorStart.size();
+    group(m.level, maxL, m.colorSortAddr, cStart[(size_t)t], true);
     m.numUpperColors[(size_t)t] = maxU + 1;
-    group(ulev, maxU, m.upperLevelRows, m.upperColorStart, false);
+    group(ulev, maxU, m.upperLevelRows, uStart[(size_t)t], false);
+  });
+  int64_t parRows = 0;
+  for (int32_t t = 0; t < T; t++) {
+    m.colorStartOffset[(size_t)t] = (int64_t)m.colorStart.size();
+    m.colorStart.insert(m.colorStart.end(), cStart[(size_t)t].begin(), cStart[(size_t)t].end());
+    m.upperColorStartOffset[(size_t)t] = (int64_t)m.upperColorStart.size();
+    m.upperColorStart.insert(m.upperColorStart.end(), uStart[(size_t)t].begin(),
+                             uStart[(size_t)t].end());
+    parRows += parTile[(size_t)t];
   }
   m.colorStartOffset[(size_t)T] = (int64_t)m.colorStart.size();
   m.upperColorStartOffset[(size_t)T] = (int64_t)m.upperColorStart.size();

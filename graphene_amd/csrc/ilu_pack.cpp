@@ -6,6 +6,8 @@
 #include <cstring>
 #include <cstdio>
 
+#include "parallel.hpp"
+
 namespace graphene::matrix {
 
 // Consecutive tiles are grouped until a group holds about `targetRows`
@@ -39,111 +41,150 @@ static void packDirection(const host::HostMatrix& m,
   const std::vector<int64_t>& starts = backward ? m.upperColorStart : m.colorStart;
   const std::vector<int64_t>& startOff =
       backward ? m.upperColorStartOffset : m.colorStartOffset;
-  int64_t slab = 0;
-  std::vector<int64_t> lvRows;   // rows of the level (matrix ids)
-  std::vector<int32_t> lvEnts;   // their entries (CRS positions), row after row
-  std::vector<int64_t> lvOff;    // R+1 offsets into lvEnts
-  for (int32_t gi = 0; gi < G; gi++) {
-    const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
-    const int64_t g0 = m.tileStart[(size_t)ta];
-    const int64_t ng = m.tileStart[(size_t)tb] - g0;
-    if (ng + 1 > 65535) {
-      ok = false;
-      return;
-    }
-    d.slabOff[(size_t)gi] = (int32_t)slab;
-    d.lvlOff[(size_t)gi] = (int32_t)d.slabEnd.size();
-    int32_t nL = 0;
-    for (int32_t t = ta; t < tb; t++)
-      nL = std::max<int32_t>(
-          nL, (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1));
-    d.maxLevels = std::max(d.maxLevels, nL);
-    int64_t groupSlab = 0;
-    int64_t slot = 0;  // row slot within the group, level order
-    for (int32_t L = 0; L < nL; L++) {
-      lvRows.clear();
-      lvEnts.clear();
-      lvOff.assign(1, 0);
-      for (int32_t t = ta; t < tb; t++) {
-        const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
-        const int64_t* ls = starts.data() + startOff[(size_t)t];
-        const int32_t tL = (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1);
-        if (L >= tL) continue;
-        for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
-          const int64_t i = t0 + rows[(size_t)(t0 + q)];
-          const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
-          if (!backward) {
-            for (int64_t a = a0; a < a1; a++) {
-              const int64_t j = m.col[(size_t)a];
-              if (i < j || j < t0 || j >= t1) break;
-              lvEnts.push_back((int32_t)a);
-            }
-          } else {
-            for (int64_t a = a1 - 1; a >= a0; a--) {
-              const int64_t j = m.col[(size_t)a];
-              if (j < t0 || j >= t1) continue;
-              if (j < i) break;
-              lvEnts.push_back((int32_t)a);
-            }
-          }
-          lvRows.push_back(i);
-          lvOff.push_back((int64_t)lvEnts.size());
-        }
+  // groups in parts of consecutive groups, packed in parallel (a group's
+  // tables are relative to its own stream), then concatenated
+  struct Part {
+    std::vector<uint16_t> col;
+    std::vector<int32_t> src, slabEnd, rowStart, width;
+    std::vector<int64_t> slabs, levels;  // per group of the part
+    int32_t maxLevels = 0, maxLevelEntries = 0;
+    bool ok = true;
+  };
+  constexpr int32_t kGroupsPerPart = 8;
+  const int32_t nParts = (G + kGroupsPerPart - 1) / kGroupsPerPart;
+  std::vector<Part> parts((size_t)nParts);
+  gph::parallelFor(nParts, 1, [&](int64_t pi) {
+    Part& P = parts[(size_t)pi];
+    std::vector<int64_t> lvRows;   // rows of the level (matrix ids)
+    std::vector<int32_t> lvEnts;   // their entries (CRS positions), row after row
+    std::vector<int64_t> lvOff;    // R+1 offsets into lvEnts
+    for (int32_t gi = (int32_t)pi * kGroupsPerPart;
+         gi < std::min(G, (int32_t)(pi + 1) * kGroupsPerPart) && P.ok; gi++) {
+      const int32_t ta = groups[(size_t)gi], tb = groups[(size_t)gi + 1];
+      const int64_t g0 = m.tileStart[(size_t)ta];
+      const int64_t ng = m.tileStart[(size_t)tb] - g0;
+      if (ng + 1 > 65535) {
+        P.ok = false;
+        break;
       }
-      const int64_t R = (int64_t)lvRows.size();
-      int64_t W = 0;
-      for (int64_t r = 0; r < R; r++)
-        W = std::max<int64_t>(W, lvOff[(size_t)r + 1] - lvOff[(size_t)r]);
-      d.rowStart.push_back((int32_t)slot);
-      d.width.push_back((int32_t)W);
-      for (int64_t r = 0; r < R; r++)
-        d.rowLoc[(size_t)(g0 + slot + r)] = (uint16_t)(lvRows[(size_t)r] - g0);
-      slot += R;
-      // entry j of row r at position j*R + r; short rows padded with zero
-      // entries (value 0 at the zero slot ng), which subtract +0
-      for (int64_t j = 0; j < W; j++)
-        for (int64_t r = 0; r < R; r++) {
-          if (j < lvOff[(size_t)r + 1] - lvOff[(size_t)r]) {
-            const int32_t a = lvEnts[(size_t)(lvOff[(size_t)r] + j)];
-            d.col.push_back((uint16_t)(m.col[(size_t)a] - g0));
-            d.src.push_back(a);
-          } else {
-            d.col.push_back((uint16_t)ng);
-            d.src.push_back(-1);
+      const size_t lv0 = P.slabEnd.size();
+      int32_t nL = 0;
+      for (int32_t t = ta; t < tb; t++)
+        nL = std::max<int32_t>(
+            nL, (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1));
+      P.maxLevels = std::max(P.maxLevels, nL);
+      int64_t groupSlab = 0;
+      int64_t slot = 0;  // row slot within the group, level order
+      for (int32_t L = 0; L < nL; L++) {
+        lvRows.clear();
+        lvEnts.clear();
+        lvOff.assign(1, 0);
+        for (int32_t t = ta; t < tb; t++) {
+          const int64_t t0 = m.tileStart[(size_t)t], t1 = m.tileStart[(size_t)t + 1];
+          const int64_t* ls = starts.data() + startOff[(size_t)t];
+          const int32_t tL = (int32_t)(startOff[(size_t)t + 1] - startOff[(size_t)t] - 1);
+          if (L >= tL) continue;
+          for (int64_t q = ls[L]; q < ls[L + 1]; q++) {
+            const int64_t i = t0 + rows[(size_t)(t0 + q)];
+            const int64_t a0 = m.rowPtr[(size_t)i], a1 = m.rowPtr[(size_t)i + 1];
+            if (!backward) {
+              for (int64_t a = a0; a < a1; a++) {
+                const int64_t j = m.col[(size_t)a];
+                if (i < j || j < t0 || j >= t1) break;
+                lvEnts.push_back((int32_t)a);
+              }
+            } else {
+              for (int64_t a = a1 - 1; a >= a0; a--) {
+                const int64_t j = m.col[(size_t)a];
+                if (j < t0 || j >= t1) continue;
+                if (j < i) break;
+                lvEnts.push_back((int32_t)a);
+              }
+            }
+            lvRows.push_back(i);
+            lvOff.push_back((int64_t)lvEnts.size());
           }
         }
-      const int64_t e = W * R;
-      d.maxLevelEntries = std::max<int32_t>(d.maxLevelEntries, (int32_t)e);
-      const int64_t ns = (e + kPackSlab - 1) / kPackSlab;
-      for (int64_t p = e; p < ns * kPackSlab; p++) {  // pad the level
-        d.col.push_back((uint16_t)ng);
-        d.src.push_back(-1);
+        const int64_t R = (int64_t)lvRows.size();
+        int64_t W = 0;
+        for (int64_t r = 0; r < R; r++)
+          W = std::max<int64_t>(W, lvOff[(size_t)r + 1] - lvOff[(size_t)r]);
+        P.rowStart.push_back((int32_t)slot);
+        P.width.push_back((int32_t)W);
+        for (int64_t r = 0; r < R; r++)
+          d.rowLoc[(size_t)(g0 + slot + r)] = (uint16_t)(lvRows[(size_t)r] - g0);
+        slot += R;
+        // entry j of row r at position j*R + r; short rows padded with zero
+        // entries (value 0 at the zero slot ng), which subtract +0
+        for (int64_t j = 0; j < W; j++)
+          for (int64_t r = 0; r < R; r++) {
+            if (j < lvOff[(size_t)r + 1] - lvOff[(size_t)r]) {
+              const int32_t a = lvEnts[(size_t)(lvOff[(size_t)r] + j)];
+              P.col.push_back((uint16_t)(m.col[(size_t)a] - g0));
+              P.src.push_back(a);
+            } else {
+              P.col.push_back((uint16_t)ng);
+              P.src.push_back(-1);
+            }
+          }
+        const int64_t e = W * R;
+        P.maxLevelEntries = std::max<int32_t>(P.maxLevelEntries, (int32_t)e);
+        const int64_t ns = (e + kPackSlab - 1) / kPackSlab;
+        for (int64_t p = e; p < ns * kPackSlab; p++) {  // pad the level
+          P.col.push_back((uint16_t)ng);
+          P.src.push_back(-1);
+        }
+        groupSlab += ns;
+        P.slabEnd.push_back((int32_t)groupSlab);
       }
-      groupSlab += ns;
-      d.slabEnd.push_back((int32_t)groupSlab);
+      if (slot != ng) {
+        P.ok = false;
+        break;
+      }
+      P.rowStart.push_back((int32_t)ng);
+      P.width.push_back(0);
+      P.slabEnd.push_back(INT32_MAX);  // sentinel: the tables hold nL+1 entries
+      if (groupSlab % 2) {  // streams start on an even slab (pairSlabs)
+        for (int64_t p = 0; p < kPackSlab; p++) {
+          P.col.push_back((uint16_t)ng);
+          P.src.push_back(-1);
+        }
+        groupSlab++;
+      }
+      P.slabs.push_back(groupSlab);
+      P.levels.push_back((int64_t)(P.slabEnd.size() - lv0));
     }
-    if (slot != ng) {
+  });
+  int64_t slab = 0, lvl = 0;
+  for (int32_t pi = 0; pi < nParts; pi++) {
+    const Part& P = parts[(size_t)pi];
+    if (!P.ok) {
       ok = false;
       return;
     }
-    d.rowStart.push_back((int32_t)ng);
-    d.width.push_back(0);
-    d.slabEnd.push_back(INT32_MAX);  // sentinel: the tables hold nL+1 entries
-    if (groupSlab % 2) {  // streams start on an even slab (pairSlabs)
-      for (int64_t p = 0; p < kPackSlab; p++) {
-        d.col.push_back((uint16_t)ng);
-        d.src.push_back(-1);
+    d.maxLevels = std::max(d.maxLevels, P.maxLevels);
+    d.maxLevelEntries = std::max(d.maxLevelEntries, P.maxLevelEntries);
+    for (size_t q = 0; q < P.slabs.size(); q++) {
+      const size_t gi = (size_t)pi * kGroupsPerPart + q;
+      d.slabOff[gi] = (int32_t)slab;
+      d.lvlOff[gi] = (int32_t)lvl;
+      slab += P.slabs[q];
+      lvl += P.levels[q];
+      // The prefetch ring reads up to two depths past a group's last slab;
+      // those reads land in the next group's stream (values unused) or in the
+      // padding after the last group, so no bounds test is needed in the kernel.
+      if ((slab + 2 * kPackDepth) * kPackSlab > INT32_MAX) {
+        ok = false;
+        return;
       }
-      groupSlab++;
     }
-    // The prefetch ring reads up to two depths past a group's last slab;
-    // those reads land in the next group's stream (values unused) or in the
-    // padding after the last group, so no bounds test is needed in the kernel.
-    slab += groupSlab;
-    if ((slab + 2 * kPackDepth) * kPackSlab > INT32_MAX) {
-      ok = false;
-      return;
-    }
+  }
+  for (const Part& P : parts) {
+    d.col.insert(d.col.end(), P.col.begin(), P.col.end());
+    d.src.insert(d.src.end(), P.src.begin(), P.src.end());
+    d.slabEnd.insert(d.slabEnd.end(), P.slabEnd.begin(), P.slabEnd.end());
+    d.rowStart.insert(d.rowStart.end(), P.rowStart.begin(), P.rowStart.end());
+    d.width.insert(d.width.end(), P.width.begin(), P.width.end());
   }
   for (int64_t p = 0; p < 2 * kPackDepth * kPackSlab; p++) {
     d.col.push_back(0);
@@ -371,11 +412,6 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, bool
   o.elemOff.assign((size_t)T + 1, 0);
   o.hdrOff.assign((size_t)T + 1, 0);
   o.metaOff.assign((size_t)T + 1, 0);
-  o.col.reserve((size_t)m.nnzOff() + (size_t)m.nrows + kRowsTail);
-  o.val.reserve(((size_t)m.nnzOff() + (size_t)m.nrows + kRowsTail) * (size_t)es);
-  o.meta.reserve((size_t)2 * m.nrows);
-  std::vector<int64_t> rows, off;
-  std::vector<int32_t> ents, order;
   // one record: row r's entries [e0, e0 + l) (r < 0: none); a column holds
   // one long record (rec[0], `lng`) or up to two short ones
   struct Rec {
@@ -387,117 +423,160 @@ IluRowStream packIluRows(const host::HostMatrix& m, const void* lu, int es, bool
     bool lng = false;
     Rec rec[2];
   };
-  std::vector<Column> step;
-  for (int32_t t = 0; t < T; t++) {
-    o.elemOff[(size_t)t] = (uint32_t)o.col.size();
-    o.hdrOff[(size_t)t] = (int32_t)o.hdr.size();
-    o.metaOff[(size_t)t] = (int64_t)o.meta.size();
-    const int64_t t0 = m.tileStart[(size_t)t];
-    for (int dir = 0; dir < 2; dir++) {
-      const bool backward = dir == 1;
-      const int32_t nL = groupLevels(m, t, t + 1, backward);
-      for (int32_t L = 0; L < nL; L++) {
-        groupLevel(m, t, t + 1, L, backward, rows, ents, off);
-        const int32_t R = (int32_t)rows.size();
-        order.resize((size_t)R);
-        for (int32_t r = 0; r < R; r++) order[(size_t)r] = r;
-        auto len = [&](int32_t r) { return off[(size_t)r + 1] - off[(size_t)r]; };
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int32_t a, int32_t b) { return len(a) > len(b); });
-        int32_t nLong = 0;
-        while (nLong < R && len(order[(size_t)nLong]) > kRowsShortLen) nLong++;
-        int32_t next = nLong;  // next short row
-        auto shortRec = [&]() {
-          Rec c;
-          if (next < R) {
-            c.r = order[(size_t)next++];
-            c.l = len(c.r);
-          }
-          return c;
-        };
-        // fills the free columns with short rows and emits the step
-        auto emit = [&](std::vector<Column>& cols) {
-          for (Column& c : cols)
-            if (!c.lng && c.rec[0].r < 0) {
+  // tiles in parts of consecutive tiles, packed in parallel (a part's words
+  // are relative to its step and record starts), then concatenated
+  struct Part {
+    std::vector<uint16_t> col;
+    std::vector<uint8_t> val;
+    std::vector<uint32_t> hdr, meta;
+    std::vector<int64_t> elem0, hdr0, meta0;  // per tile of the part
+  };
+  constexpr int32_t kTilesPerPart = 16;
+  const int32_t nParts = (T + kTilesPerPart - 1) / kTilesPerPart;
+  std::vector<Part> parts((size_t)nParts);
+  gph::parallelFor(nParts, 1, [&](int64_t pi) {
+    Part& P = parts[(size_t)pi];
+    std::vector<int64_t> rows, off;
+    std::vector<int32_t> ents, order;
+    std::vector<Column> step;
+    for (int32_t t = (int32_t)pi * kTilesPerPart; t < std::min(T, (int32_t)(pi + 1) * kTilesPerPart);
+         t++) {
+      P.elem0.push_back((int64_t)P.col.size());
+      P.hdr0.push_back((int64_t)P.hdr.size());
+      P.meta0.push_back((int64_t)P.meta.size());
+      const int64_t t0 = m.tileStart[(size_t)t];
+      for (int dir = 0; dir < 2; dir++) {
+        const bool backward = dir == 1;
+        const int32_t nL = groupLevels(m, t, t + 1, backward);
+        for (int32_t L = 0; L < nL; L++) {
+          groupLevel(m, t, t + 1, L, backward, rows, ents, off);
+          const int32_t R = (int32_t)rows.size();
+          order.resize((size_t)R);
+          for (int32_t r = 0; r < R; r++) order[(size_t)r] = r;
+          auto len = [&](int32_t r) { return off[(size_t)r + 1] - off[(size_t)r]; };
+          std::stable_sort(order.begin(), order.end(),
+                           [&](int32_t a, int32_t b) { return len(a) > len(b); });
+          int32_t nLong = 0;
+          while (nLong < R && len(order[(size_t)nLong]) > kRowsShortLen) nLong++;
+          int32_t next = nLong;  // next short row
+          auto shortRec = [&]() {
+            Rec c;
+            if (next < R) {
+              c.r = order[(size_t)next++];
+              c.l = len(c.r);
+            }
+            return c;
+          };
+          // fills the free columns with short rows and emits the step
+          auto emit = [&](std::vector<Column>& cols) {
+            for (Column& c : cols)
+              if (!c.lng && c.rec[0].r < 0) {
+                c.rec[0] = shortRec();
+                c.rec[1] = shortRec();
+              }
+            while ((int32_t)cols.size() < kRowsColumns && next < R) {
+              Column c;
               c.rec[0] = shortRec();
               c.rec[1] = shortRec();
+              cols.push_back(c);
             }
-          while ((int32_t)cols.size() < kRowsColumns && next < R) {
-            Column c;
-            c.rec[0] = shortRec();
-            c.rec[1] = shortRec();
-            cols.push_back(c);
-          }
-          while (!cols.empty() && !cols.back().lng && cols.back().rec[0].r < 0) cols.pop_back();
-          // records in record-word order, each contiguous, padded to whole
-          // chunks or (dense) to an even length; a long record's two words
-          // share its entries
-          bool anyLong = false;
-          int64_t elems = 0;
-          for (const Column& c : cols) {
-            uint32_t w[2] = {0u, 0u};
-            for (int hh = 0; hh < (c.lng ? 1 : 2); hh++) {
-              const Rec& rc = c.rec[hh];
-              if (rc.r < 0) continue;
-              const int64_t stored = (rc.l + gran - 1) / gran * gran;
-              const int64_t slot = rows[(size_t)rc.r] - t0 + 1;
-              w[hh] = (uint32_t)elems | (uint32_t)stored << kRowsLenShift |
-                      (c.lng ? kRowsLong : 0u) | rc.flags | (uint32_t)slot << kRowsSlotShift;
-              for (int64_t e = 0; e < stored; e++) {
-                if (e < rc.l) {
-                  const int32_t a = ents[(size_t)(off[(size_t)rc.r] + rc.e0 + e)];
-                  o.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
-                  o.val.insert(o.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
-                } else {
-                  o.col.push_back(zero);
-                  o.val.insert(o.val.end(), (size_t)es, 0);
+            while (!cols.empty() && !cols.back().lng && cols.back().rec[0].r < 0) cols.pop_back();
+            // records in record-word order, each contiguous, padded to whole
+            // chunks or (dense) to an even length; a long record's two words
+            // share its entries
+            bool anyLong = false;
+            int64_t elems = 0;
+            for (const Column& c : cols) {
+              uint32_t w[2] = {0u, 0u};
+              for (int hh = 0; hh < (c.lng ? 1 : 2); hh++) {
+                const Rec& rc = c.rec[hh];
+                if (rc.r < 0) continue;
+                const int64_t stored = (rc.l + gran - 1) / gran * gran;
+                const int64_t slot = rows[(size_t)rc.r] - t0 + 1;
+                w[hh] = (uint32_t)elems | (uint32_t)stored << kRowsLenShift |
+                        (c.lng ? kRowsLong : 0u) | rc.flags | (uint32_t)slot << kRowsSlotShift;
+                for (int64_t e = 0; e < stored; e++) {
+                  if (e < rc.l) {
+                    const int32_t a = ents[(size_t)(off[(size_t)rc.r] + rc.e0 + e)];
+                    P.col.push_back((uint16_t)(4 * (m.col[(size_t)a] - t0 + 1)));
+                    P.val.insert(P.val.end(), luB + (size_t)a * es, luB + (size_t)(a + 1) * es);
+                  } else {
+                    P.col.push_back(zero);
+                    P.val.insert(P.val.end(), (size_t)es, 0);
+                  }
                 }
+                elems += stored;
               }
-              elems += stored;
+              if (c.lng) w[1] = w[0];
+              P.meta.push_back(w[0]);
+              P.meta.push_back(w[1]);
+              anyLong = anyLong || c.lng;
             }
-            if (c.lng) w[1] = w[0];
-            o.meta.push_back(w[0]);
-            o.meta.push_back(w[1]);
-            anyLong = anyLong || c.lng;
-          }
-          o.hdr.push_back((uint32_t)elems | (uint32_t)(2 * cols.size()) << 12 |
-                          (anyLong ? 1u << 30 : 0u) | (backward ? 1u << 31 : 0u));
-          cols.clear();
-        };
-        // long rows, one column each, kRowsColumns at a time; a row longer
-        // than kRowsMaxLen continues in the same column of the next steps
-        for (int32_t c0 = 0; c0 < nLong; c0 += kRowsColumns) {
-          const int32_t n = std::min<int32_t>(kRowsColumns, nLong - c0);
-          const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
-          for (int64_t k = 0; k < chunks; k++) {
-            for (int32_t q = 0; q < n; q++) {
-              const int32_t r = order[(size_t)(c0 + q)];
-              const int64_t nk = (len(r) + kRowsMaxLen - 1) / kRowsMaxLen;
-              Column c;
-              if (k < nk) {
-                c.lng = true;
-                c.rec[0].r = r;
-                c.rec[0].e0 = k * kRowsMaxLen;
-                c.rec[0].l = std::min<int64_t>(kRowsMaxLen, len(r) - c.rec[0].e0);
-                c.rec[0].flags = (k > 0 ? kRowsCont : 0u) | (k + 1 < nk ? kRowsPartial : 0u);
+            P.hdr.push_back((uint32_t)elems | (uint32_t)(2 * cols.size()) << 12 |
+                            (anyLong ? 1u << 30 : 0u) | (backward ? 1u << 31 : 0u));
+            cols.clear();
+          };
+          // long rows, one column each, kRowsColumns at a time; a row longer
+          // than kRowsMaxLen continues in the same column of the next steps
+          for (int32_t c0 = 0; c0 < nLong; c0 += kRowsColumns) {
+            const int32_t n = std::min<int32_t>(kRowsColumns, nLong - c0);
+            const int64_t chunks = (len(order[(size_t)c0]) + kRowsMaxLen - 1) / kRowsMaxLen;
+            for (int64_t k = 0; k < chunks; k++) {
+              for (int32_t q = 0; q < n; q++) {
+                const int32_t r = order[(size_t)(c0 + q)];
+                const int64_t nk = (len(r) + kRowsMaxLen - 1) / kRowsMaxLen;
+                Column c;
+                if (k < nk) {
+                  c.lng = true;
+                  c.rec[0].r = r;
+                  c.rec[0].e0 = k * kRowsMaxLen;
+                  c.rec[0].l = std::min<int64_t>(kRowsMaxLen, len(r) - c.rec[0].e0);
+                  c.rec[0].flags = (k > 0 ? kRowsCont : 0u) | (k + 1 < nk ? kRowsPartial : 0u);
+                }
+                step.push_back(c);
               }
-              step.push_back(c);
+              emit(step);
             }
-            emit(step);
           }
+          while (next < R) emit(step);
         }
-        while (next < R) emit(step);
       }
     }
-    o.maxSteps = std::max<int32_t>(o.maxSteps, (int32_t)o.hdr.size() - o.hdrOff[(size_t)t]);
-    o.maxRecs = std::max<int64_t>(o.maxRecs, (int64_t)o.meta.size() - o.metaOff[(size_t)t]);
-    if (o.col.size() > (size_t)UINT32_MAX - 2 * kRowsTail) return o;
+  });
+  int64_t nCol = 0, nHdr = 0, nMeta = 0;
+  for (const Part& P : parts) {
+    for (size_t q = 0; q < P.elem0.size(); q++) {
+      const size_t t = (size_t)(&P - parts.data()) * kTilesPerPart + q;
+      o.elemOff[t] = (uint32_t)(nCol + P.elem0[q]);
+      o.hdrOff[t] = (int32_t)(nHdr + P.hdr0[q]);
+      o.metaOff[t] = nMeta + P.meta0[q];
+      const int64_t h1 = q + 1 < P.hdr0.size() ? P.hdr0[q + 1] : (int64_t)P.hdr.size();
+      const int64_t m1 = q + 1 < P.meta0.size() ? P.meta0[q + 1] : (int64_t)P.meta.size();
+      o.maxSteps = std::max<int32_t>(o.maxSteps, (int32_t)(h1 - P.hdr0[q]));
+      o.maxRecs = std::max<int64_t>(o.maxRecs, m1 - P.meta0[q]);
+    }
+    nCol += (int64_t)P.col.size();
+    nHdr += (int64_t)P.hdr.size();
+    nMeta += (int64_t)P.meta.size();
   }
-  o.elemOff[(size_t)T] = (uint32_t)o.col.size();
-  o.hdrOff[(size_t)T] = (int32_t)o.hdr.size();
-  o.metaOff[(size_t)T] = (int64_t)o.meta.size();
-  o.col.insert(o.col.end(), (size_t)kRowsTail, zero);
-  o.val.insert(o.val.end(), (size_t)kRowsTail * es, 0);
+  if (nCol > (int64_t)UINT32_MAX - 2 * kRowsTail || nHdr > INT32_MAX) return o;
+  o.elemOff[(size_t)T] = (uint32_t)nCol;
+  o.hdrOff[(size_t)T] = (int32_t)nHdr;
+  o.metaOff[(size_t)T] = nMeta;
+  o.col.resize((size_t)nCol + kRowsTail, zero);
+  o.val.resize(((size_t)nCol + kRowsTail) * (size_t)es, 0);
+  o.hdr.resize((size_t)nHdr);
+  o.meta.resize((size_t)nMeta);
+  gph::parallelFor(nParts, 1, [&](int64_t pi) {
+    Part& P = parts[(size_t)pi];
+    const size_t t0 = (size_t)pi * kTilesPerPart;
+    std::copy(P.col.begin(), P.col.end(), o.col.begin() + o.elemOff[t0]);
+    std::copy(P.val.begin(), P.val.end(), o.val.begin() + (size_t)o.elemOff[t0] * (size_t)es);
+    std::copy(P.hdr.begin(), P.hdr.end(), o.hdr.begin() + o.hdrOff[t0]);
+    std::copy(P.meta.begin(), P.meta.end(), o.meta.begin() + o.metaOff[t0]);
+    Part().col.swap(P.col);
+    Part().val.swap(P.val);
+  });
   o.ok = true;
   return o;
 }
