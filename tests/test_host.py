@@ -215,3 +215,31 @@ def test_tile_layout_hand_derived_fixture():
     assert e["num_separator_regions"].tolist() == HAND_NUM_REGIONS
     hs = e["halo_start"]
     assert [e["halo_global"][hs[t]:hs[t + 1]].tolist() for t in range(3)] == HAND_HALO
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_structure_beyond_one_work_chunk(reverse):
+    """The host pipeline works in chunks on worker threads (1 Mi triplets,
+    16 Ki rows, tiles and tile parts): a 100,000-row power-law matrix in 16
+    tiles spans several chunks of each; bit-exact against the oracle layout
+    with the triplets in row order and reversed (the bucketing path), levels
+    too."""
+    n, T = 100000, 16
+    rows, cols, vals = g.generatePowerLawTriplets(n, seed=4)
+    if reverse:
+        rows, cols, vals = rows[::-1].copy(), cols[::-1].copy(), vals[::-1].copy()
+    assert len(vals) > (1 << 20)
+    h = g.HostMatrix.from_coo(n, rows, cols, vals, T)
+    e = h.export()
+    r2t = np.empty(n, np.int32)
+    for t in range(T):
+        r2t[e["new_to_old"][e["tile_start"][t]:e["tile_start"][t + 1]]] = t
+    L = oracle_layout(n, rows, cols, vals, T, r2t)
+    for k in ["tile_start", "new_to_old", "num_interior", "num_separator_regions",
+              "row_ptr", "col_local", "off", "diag", "halo_start", "halo_global"]:
+        assert np.array_equal(e[k], L[k]), k
+    assert np.array_equal(e["col"].astype(np.int64), L["col"])
+    lv = h.levels()
+    ol = o.levels(n, T, L["tile_start"], L["row_ptr"], L["col"])
+    for k in ["level", "color_sort_addr", "color_start", "num_colors"]:
+        assert np.array_equal(lv[k], ol[k]), k
