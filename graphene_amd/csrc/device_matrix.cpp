@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include "parallel.hpp"
+
 namespace graphene::matrix {
 
 using gph::DevBuf;
@@ -157,9 +159,9 @@ void DeviceMatrix::buildColumnOrder(const std::vector<int32_t>& br) {
   std::vector<int32_t> gc((size_t)nnz_);
   std::vector<uint16_t> gp((size_t)nnz_);
   std::vector<float> gv((size_t)nnz_);
-  std::vector<int32_t> idx;
-  for (size_t b = 0; b + 1 < br.size(); b++) {
-    const int64_t e0 = h.rowPtr[(size_t)br[b]], e1 = h.rowPtr[(size_t)br[b + 1]];
+  gph::parallelFor((int64_t)br.size() - 1, 64, [&](int64_t b) {
+    std::vector<int32_t> idx;
+    const int64_t e0 = h.rowPtr[(size_t)br[(size_t)b]], e1 = h.rowPtr[(size_t)br[(size_t)b + 1]];
     for (int64_t c0 = e0; c0 < e1; c0 += gph::kSpmvCap) {
       const int32_t cn = (int32_t)std::min<int64_t>(gph::kSpmvCap, e1 - c0);
       idx.resize((size_t)cn);
@@ -174,7 +176,7 @@ void DeviceMatrix::buildColumnOrder(const std::vector<int32_t>& br) {
         gv[(size_t)c0 + k] = h.offDiag[src];
       }
     }
-  }
+  });
   up(gcol_, gc, stream_);
   up(gpos_, gp, stream_);
   up(gval32_, gv, stream_);
@@ -206,28 +208,41 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
   for (int64_t r = 0; r < n; r++) perm[(size_t)r] = (int32_t)r;
   if (sorted) {
     const int64_t W = 4096;
-    for (int64_t w0 = 0; w0 < n; w0 += W) {
-      const int64_t w1 = std::min(n, w0 + W);
+    gph::parallelFor((n + W - 1) / W, 16, [&](int64_t wi) {
+      const int64_t w0 = wi * W, w1 = std::min(n, w0 + W);
       std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int32_t a, int32_t b) {
         return rowPtr[(size_t)a + 1] - rowPtr[(size_t)a] > rowPtr[(size_t)b + 1] - rowPtr[(size_t)b];
       });
-    }
+    });
   }
   std::vector<int32_t> sliceOff((size_t)numSlices + 1, 0);
   std::vector<uint16_t> len((size_t)numSlices * 64, 0);
-  int64_t slots = 0;
-  for (int64_t sl = 0; sl < numSlices; sl++) {
+  // per slice (in parallel): row lengths, width, column range
+  std::vector<int64_t> width((size_t)numSlices, 0);
+  std::vector<int32_t> loC((size_t)numSlices, 0), hiC((size_t)numSlices, 0);
+  gph::parallelFor(numSlices, 1024, [&](int64_t sl) {
     int64_t K = 0;
+    int32_t lo = INT32_MAX, hi = -1;
     for (int l = 0; l < 64; l++) {
       int32_t r = perm[(size_t)(sl * 64 + l)];
-      if (r < n) {
-        int64_t ln = rowPtr[(size_t)r + 1] - rowPtr[(size_t)r];
-        len[(size_t)(sl * 64 + l)] = (uint16_t)ln;
-        K = std::max(K, ln);
+      if (r >= n) continue;
+      int64_t ln = rowPtr[(size_t)r + 1] - rowPtr[(size_t)r];
+      len[(size_t)(sl * 64 + l)] = (uint16_t)ln;
+      K = std::max(K, ln);
+      for (int64_t e = rowPtr[(size_t)r]; e < rowPtr[(size_t)r + 1]; e++) {
+        lo = std::min(lo, cols[(size_t)e]);
+        hi = std::max(hi, cols[(size_t)e]);
       }
     }
+    if (hi < 0) lo = hi = 0;  // empty slice
+    width[(size_t)sl] = K;
+    loC[(size_t)sl] = lo;
+    hiC[(size_t)sl] = hi;
+  });
+  int64_t slots = 0;
+  for (int64_t sl = 0; sl < numSlices; sl++) {
     sliceOff[(size_t)sl] = (int32_t)slots;
-    slots += 64 * K;
+    slots += 64 * width[(size_t)sl];
     if (slots > INT32_MAX) return;
   }
   sliceOff[(size_t)numSlices] = (int32_t)slots;
@@ -237,17 +252,8 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
   uint64_t words = 0;
   int64_t wide = 0;
   for (int64_t sl = 0; sl < numSlices; sl++) {
-    int32_t lo = INT32_MAX, hi = -1;
-    for (int l = 0; l < 64; l++) {
-      int32_t r = perm[(size_t)(sl * 64 + l)];
-      if (r >= n) continue;
-      for (int64_t e = rowPtr[(size_t)r]; e < rowPtr[(size_t)r + 1]; e++) {
-        lo = std::min(lo, cols[(size_t)e]);
-        hi = std::max(hi, cols[(size_t)e]);
-      }
-    }
+    const int32_t lo = loC[(size_t)sl], hi = hiC[(size_t)sl];
     const int64_t sw = sliceOff[(size_t)sl + 1] - sliceOff[(size_t)sl];
-    if (hi < 0) lo = hi = 0;  // empty slice
     colW[(size_t)sl] = (uint32_t)words;
     if ((int64_t)hi - lo < 65536) {
       base[(size_t)sl] = lo;
@@ -261,7 +267,7 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
   colW[(size_t)numSlices] = (uint32_t)words;
   std::vector<uint16_t> c((size_t)std::max<uint64_t>(words, 2), 0);
   std::vector<float> v((size_t)std::max<int64_t>(slots, 1), 0.0f);
-  for (int64_t sl = 0; sl < numSlices; sl++) {
+  gph::parallelFor(numSlices, 1024, [&](int64_t sl) {
     const int32_t b = base[(size_t)sl];
     for (int l = 0; l < 64; l++) {
       int32_t r = perm[(size_t)(sl * 64 + l)];
@@ -284,7 +290,7 @@ void DeviceMatrix::buildSell(const std::vector<int64_t>& rowPtr,
         }
       }
     }
-  }
+  });
   hipStream_t s = stream_;
   out.sliceOff.upload(sliceOff.data(), sliceOff.size(), s);
   out.rowLen.upload(len.data(), len.size(), s);

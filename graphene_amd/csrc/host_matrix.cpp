@@ -976,57 +976,73 @@ LDUHost buildLDU(const HostMatrix& m) {
   LDUHost L;
   L.ownerStart.assign((size_t)n + 1, 0);
   L.neighbourStart.assign((size_t)n + 1, 0);
-  // faces: owner ascending, neighbour ascending
-  std::vector<std::pair<int32_t, float>> ups;
-  std::vector<int64_t> rowUpperBegin((size_t)n + 1, 0);
+  constexpr int64_t kRows = 4096;
+  // faces: owner ascending, neighbour ascending (rows in parallel: counts,
+  // offsets, then each row's upper entries sorted into place)
+  std::vector<int64_t> up((size_t)n + 1, 0), low((size_t)n + 1, 0);
+  gph::parallelFor(n, kRows, [&](int64_t l) {
+    int64_t cu = 0, cl = 0;
+    for (int64_t a = m.rowPtr[(size_t)l]; a < m.rowPtr[(size_t)l + 1]; a++) {
+      if (m.col[(size_t)a] > l) cu++;
+      if (m.col[(size_t)a] < l) cl++;
+    }
+    up[(size_t)l + 1] = cu;
+    low[(size_t)l + 1] = cl;
+  });
   for (int64_t l = 0; l < n; l++) {
-    L.ownerStart[(size_t)l] = (int32_t)L.upperAddr.size();
-    ups.clear();
+    up[(size_t)l + 1] += up[(size_t)l];
+    low[(size_t)l + 1] += low[(size_t)l];
+  }
+  L.numFaces = up[(size_t)n];
+  // every face (l, u) must appear as the lower entry (u, l): counts first
+  if (low[(size_t)n] != L.numFaces)
+    fail(kInvalid, "LDU format needs a structurally symmetric matrix");
+  if (L.numFaces >= (int64_t)INT32_MAX) fail(kInvalid, "too many faces for int32 ids");
+  for (int64_t l = 0; l <= n; l++) {
+    L.ownerStart[(size_t)l] = (int32_t)up[(size_t)l];
+    L.neighbourStart[(size_t)l] = (int32_t)low[(size_t)l];
+  }
+  L.lowerAddr.resize((size_t)L.numFaces);
+  L.upperAddr.resize((size_t)L.numFaces);
+  L.upper.resize((size_t)L.numFaces);
+  gph::parallelFor(n, kRows, [&](int64_t l) {
+    std::vector<std::pair<int32_t, float>> ups;
     for (int64_t a = m.rowPtr[(size_t)l]; a < m.rowPtr[(size_t)l + 1]; a++)
       if (m.col[(size_t)a] > l) ups.emplace_back(m.col[(size_t)a], m.offDiag[(size_t)a]);
     std::sort(ups.begin(), ups.end(),
               [](const auto& x, const auto& y) { return x.first < y.first; });
+    int64_t f = up[(size_t)l];
     for (auto& u : ups) {
-      L.lowerAddr.push_back((int32_t)l);
-      L.upperAddr.push_back(u.first);
-      L.upper.push_back(u.second);
+      L.lowerAddr[(size_t)f] = (int32_t)l;
+      L.upperAddr[(size_t)f] = u.first;
+      L.upper[(size_t)f] = u.second;
+      f++;
     }
-  }
-  L.ownerStart[(size_t)n] = (int32_t)L.upperAddr.size();
-  L.numFaces = (int64_t)L.upperAddr.size();
-  // losort: per neighbour cell, faces ordered by face index == owner order
-  std::vector<int64_t> cnt((size_t)n + 1, 0);
-  for (int64_t f = 0; f < L.numFaces; f++) cnt[(size_t)L.upperAddr[(size_t)f] + 1]++;
-  for (int64_t c = 0; c < n; c++) cnt[(size_t)c + 1] += cnt[(size_t)c];
-  for (int64_t c = 0; c <= n; c++) L.neighbourStart[(size_t)c] = (int32_t)cnt[(size_t)c];
+  });
+  // losort: per neighbour cell u, its faces (l, u) by face index, i.e. by
+  // owner l ascending; with them the lower coefficients A[u][l]
   L.neighbourColInd.assign((size_t)L.numFaces, 0);
   L.lower.assign((size_t)L.numFaces, 0.0f);
-  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-  for (int64_t f = 0; f < L.numFaces; f++)
-    L.neighbourColInd[(size_t)fill[(size_t)L.upperAddr[(size_t)f]]++] = (int32_t)f;
-  // lower coefficient A[u][l] of every face, in losort order
-  int64_t lowerCount = 0;
-  for (int64_t u = 0; u < n; u++) {
-    for (int64_t a = m.rowPtr[(size_t)u]; a < m.rowPtr[(size_t)u + 1]; a++) {
-      int64_t l = m.col[(size_t)a];
-      if (l >= u) continue;
-      lowerCount++;
+  gph::parallelFor(n, kRows, [&](int64_t u) {
+    std::vector<std::pair<int32_t, float>> lows;
+    for (int64_t a = m.rowPtr[(size_t)u]; a < m.rowPtr[(size_t)u + 1]; a++)
+      if (m.col[(size_t)a] < u) lows.emplace_back(m.col[(size_t)a], m.offDiag[(size_t)a]);
+    std::sort(lows.begin(), lows.end(),
+              [](const auto& x, const auto& y) { return x.first < y.first; });
+    int64_t s = low[(size_t)u];
+    for (auto& lw : lows) {
+      const int32_t l = lw.first;
       // face (l, u): search l's owned faces
-      int32_t f0 = L.ownerStart[(size_t)l], f1 = L.ownerStart[(size_t)l + 1];
+      const int32_t f0 = L.ownerStart[(size_t)l], f1 = L.ownerStart[(size_t)l + 1];
       auto it = std::lower_bound(L.upperAddr.begin() + f0, L.upperAddr.begin() + f1,
                                  (int32_t)u);
       if (it == L.upperAddr.begin() + f1 || *it != (int32_t)u)
         fail(kInvalid, "LDU format needs a structurally symmetric matrix");
-      int64_t f = it - L.upperAddr.begin();
-      // position of face f inside u's losort segment (faces ascending)
-      int32_t s0 = L.neighbourStart[(size_t)u], s1 = L.neighbourStart[(size_t)u + 1];
-      auto is = std::lower_bound(L.neighbourColInd.begin() + s0,
-                                 L.neighbourColInd.begin() + s1, (int32_t)f);
-      L.lower[(size_t)(is - L.neighbourColInd.begin())] = m.offDiag[(size_t)a];
+      L.neighbourColInd[(size_t)s] = (int32_t)(it - L.upperAddr.begin());
+      L.lower[(size_t)s] = lw.second;
+      s++;
     }
-  }
-  if (lowerCount != L.numFaces)
-    fail(kInvalid, "LDU format needs a structurally symmetric matrix");
+  });
   return L;
 }
 
