@@ -41,7 +41,7 @@ static size_t maxDynamicShared() { return 160 * 1024 - 4096; }
 // loop on one lane.
 // ---------------------------------------------------------------------------
 constexpr int kFactorWaves = 4;  // rows (waves) per block
-constexpr int kFactorBatch = 8;  // rows k loaded together
+constexpr int kFactorBatch = 4;  // rows k loaded together (128^3: 8 rows took 7.8 ms in FP64, 4 rows 5.8)
 
 template <typename FT>
 __device__ __forceinline__ FT readLane(FT v, int l) {
