@@ -1,22 +1,22 @@
-"""Each benchmark configuration's own workload, at full size, on the GPU.
+"""Each benchmark workload at full size, on the GPU.
 
-cfg0  32^3 Poisson-7, one tile, FP32 PBiCGStab + ILU(0) to relResidual 1e-6
+poisson32 (one tile): 32^3 Poisson-7, FP32 PBiCGStab + ILU(0) to relResidual 1e-6
       (the reference's own CPU-runnable case): SpMV bit-exact against the
       oracle, iteration count within 5 % of the oracle's.
-cfg1  128^3 27-point convection-diffusion (Peclet 10), 16x8x8-cell tiles:
+convdiff128: 128^3 27-point convection-diffusion (Peclet 10), 16x8x8-cell tiles:
       CRS and LDU SpMV bit-exact against the oracle at full size; MPIR
       (FP64 outer, FP32 PBiCGStab + block ILU(0) inner) to 1e-10 with the
       true FP64 residual checked and the outer/inner counts compared with
       one oracle MPIR run; forward error against an FP64 solve to 1e-14; the
       same refinement in twofloat32 to a true 1e-10.
-cfg2  4,194,304-row power-law matrix, 4096 contiguous tiles: CRS SpMV (both
+powerlaw4M: 4,194,304-row power-law matrix, 4096 contiguous tiles: CRS SpMV (both
       kernel forms) bit-exact against the oracle; PBiCGStab + symmetric
       Gauss-Seidel (2 sweeps) iteration count against the oracle, and the
       same inner solve under FP64 refinement to a true residual of 1e-8.
-cfg3  512 2D 9-point systems, log-uniform 1e3..2e5 rows, in one batch:
+batch512: 512 2D 9-point systems, log-uniform 1e3..2e5 rows, in one batch:
       every system's true FP64 residual <= 1e-10; three sampled systems
       equal their solo solves.
-cfg4  request server at 50, 200 and 800 requests/s over resident systems
+serving: request server at 50, 200 and 800 requests/s over resident systems
       of the three generators (1e4..2e6 rows, 1-8 right-hand sides): every
       request converges to 1e-10 and a sampled request's result is the
       same alone and in the mix.
@@ -53,8 +53,8 @@ ILU_INNER = {"type": "PBiCGStab", "relResidual": 1e-3, "maxIterations": 100,
              "preconditioner": {"type": "ILU", "factorType": "float32"}}
 
 
-# ---- cfg0 -------------------------------------------------------------------
-def test_cfg0_poisson32_pbicgstab_ilu0():
+# ---- poisson32 -------------------------------------------------------------------
+def test_poisson32_one_tile_pbicgstab_ilu0():
     h, L = poisson_case(32, 32, 32)
     assert h.nrows == 32768 and h.nnz_off == 190464 and h.num_tiles == 1
     A = h.copyToTile()
@@ -68,15 +68,15 @@ def test_cfg0_poisson32_pbicgstab_ilu0():
     st = g.Solver(A, cfg).solve(xg, dev(b))
     S = o.System(L)
     _, sto = S.pbicgstab(np.zeros(h.nrows, np.float32), b, S.pbicgstab_cfg(1e-6, 1000))
-    print(f"cfg0: GPU {st}, oracle {sto.iterations} iterations", flush=True)
+    print(f"poisson32: GPU {st}, oracle {sto.iterations} iterations", flush=True)
     assert st.converged and sto.converged
     assert abs(st.iterations - sto.iterations) <= max(1, 0.05 * sto.iterations)
     assert true_rel(A, dev(xg.numpy().astype(np.float64)), dev(b)) <= 1e-5
 
 
-# ---- cfg1 -------------------------------------------------------------------
+# ---- convdiff128 -------------------------------------------------------------------
 @pytest.fixture(scope="module")
-def cfg1():
+def convdiff128():
     n = 128
     rows, cols, vals = g.generateConvectionDiffusion27Triplets(n, n, n, 10.0, 1)
     r2t, T = g.partition_grid3d(n, n, n, 16, 8, 8)
@@ -88,8 +88,8 @@ def cfg1():
     return dict(h=h, L=L, b=b, T=T)
 
 
-def test_cfg1_spmv_full_size_bit_exact(cfg1):
-    h, L = cfg1["h"], cfg1["L"]
+def test_convdiff128_spmv_full_size_bit_exact(convdiff128):
+    h, L = convdiff128["h"], convdiff128["L"]
     x = o.uniform_pm1(41, h.nrows)
     ref = o.spmv(L, x).view(np.uint32)
     A = h.copyToTile(g.CRS)
@@ -107,15 +107,15 @@ def test_cfg1_spmv_full_size_bit_exact(cfg1):
                           o.ldu_residual_mixed(U, x64, bb)[0])
 
 
-def test_cfg1_mpir_against_oracle(cfg1):
-    h, L, b = cfg1["h"], cfg1["L"], cfg1["b"]
+def test_convdiff128_mpir_against_oracle(convdiff128):
+    h, L, b = convdiff128["h"], convdiff128["L"], convdiff128["b"]
     A = h.copyToTile(g.LDU)
     phi = dev(np.zeros(h.nrows, np.float64))
     st = g.Solver(A, mpir_cfg(ILU_INNER)).solve(phi, dev(b))
     rel = true_rel(A, phi, dev(b))
     S = o.System(L)
     _, sto, inner = S.mpir(b, 1e-10, 50, inner=S.pbicgstab_cfg(1e-3, 100))
-    print(f"cfg1 MPIR: GPU {st} (true rel. residual {rel:.3e}), oracle "
+    print(f"convdiff128 MPIR: GPU {st} (true rel. residual {rel:.3e}), oracle "
           f"{sto.iterations} outer / {inner} inner", flush=True)
     assert st.converged and sto.converged
     assert rel <= 1e-10
@@ -131,15 +131,15 @@ def test_cfg1_mpir_against_oracle(cfg1):
     assert s64.converged and true_rel(A, x64, dev(b)) <= 1e-13
     xr, xm = x64.numpy(), phi.numpy()
     fwd = float(np.linalg.norm(xm - xr) / np.linalg.norm(xr))
-    print(f"cfg1 forward error of MPIR vs FP64 (1e-14): {fwd:.3e}", flush=True)
+    print(f"convdiff128 forward error of MPIR vs FP64 (1e-14): {fwd:.3e}", flush=True)
     assert fwd <= 1e-8
 
 
-def test_cfg1_mpir_twofloat(cfg1):
+def test_convdiff128_mpir_twofloat(convdiff128):
     """The same refinement with twofloat32 extended precision (the solution
     in double words): converged, true FP64 residual of hi + lo <= 1e-10, the
     outer count of the FP64-extended refinement +-1."""
-    h, b = cfg1["h"], cfg1["b"]
+    h, b = convdiff128["h"], convdiff128["b"]
     A = h.copyToTile(g.LDU)
     x = g.DeviceVector(h.nrows, g.TWOFLOAT)
     x.fill(0.0)
@@ -150,22 +150,22 @@ def test_cfg1_mpir_twofloat(cfg1):
     rel = true_rel(A, dev(x64), dev(b))
     phi = dev(np.zeros(h.nrows, np.float64))
     st64 = g.Solver(A, mpir_cfg(ILU_INNER)).solve(phi, dev(b))
-    print(f"cfg1 MPIR twofloat32: {st} (true rel. residual {rel:.3e}); float64: {st64}",
+    print(f"convdiff128 MPIR twofloat32: {st} (true rel. residual {rel:.3e}); float64: {st64}",
           flush=True)
     assert st.converged and rel <= 1e-10
     assert abs(st.iterations - st64.iterations) <= 1
 
 
-# ---- cfg2 -------------------------------------------------------------------
+# ---- powerlaw4m -------------------------------------------------------------------
 @pytest.fixture(scope="module")
-def cfg2():
+def powerlaw4m():
     h, L = powerlaw_case(4194304, T=4096, seed=2)
     assert h.nrows == 4194304 and h.num_tiles == 4096
     return dict(h=h, L=L)
 
 
-def test_cfg2_spmv_full_size_bit_exact(cfg2):
-    h, L = cfg2["h"], cfg2["L"]
+def test_powerlaw4m_spmv_full_size_bit_exact(powerlaw4m):
+    h, L = powerlaw4m["h"], powerlaw4m["L"]
     A = h.copyToTile(g.CRS)
     x = o.uniform_pm1(51, h.nrows)
     ref = o.spmv(L, x).view(np.uint32)
@@ -177,8 +177,8 @@ def test_cfg2_spmv_full_size_bit_exact(cfg2):
 GS2 = {"type": "GaussSeidel", "numFixedIterations": 2, "symmetric": True}
 
 
-def test_cfg2_solve(cfg2):
-    h, L = cfg2["h"], cfg2["L"]
+def test_powerlaw4m_solve(powerlaw4m):
+    h, L = powerlaw4m["h"], powerlaw4m["L"]
     A = h.copyToTile(g.CRS)
     xt, b = rhs_for(h, L, seed=2)
     # plain FP32 PBiCGStab + SGS(2) against the oracle
@@ -190,7 +190,7 @@ def test_cfg2_solve(cfg2):
     _, sto = S.pbicgstab(np.zeros(h.nrows, np.float32), b,
                          S.pbicgstab_cfg(1e-8, 2000, precond="gs", gsSweeps=2,
                                          gsSymmetric=True))
-    print(f"cfg2: GPU {st}, oracle {sto.iterations} iterations", flush=True)
+    print(f"powerlaw4m: GPU {st}, oracle {sto.iterations} iterations", flush=True)
     assert st.converged and sto.converged
     assert abs(st.iterations - sto.iterations) <= max(1, 0.05 * sto.iterations)
     # the same inner solve under FP64 refinement reaches a true 1e-8
@@ -198,11 +198,11 @@ def test_cfg2_solve(cfg2):
     phi = dev(np.zeros(h.nrows, np.float64))
     sm = g.Solver(A, mpir_cfg(inner, rel=1e-8)).solve(phi, dev(b))
     rel = true_rel(A, phi, dev(b))
-    print(f"cfg2 MPIR: {sm}, true rel. residual {rel:.3e}", flush=True)
+    print(f"powerlaw4m MPIR: {sm}, true rel. residual {rel:.3e}", flush=True)
     assert sm.converged and rel <= 1e-8
 
 
-# ---- cfg3 -------------------------------------------------------------------
+# ---- batch512 -------------------------------------------------------------------
 def make_batch_systems(S=512, seed=3):
     """512 2D 9-point convection-diffusion systems, sizes log-uniform in
     [1e3, 2e5] rows (square grids of side round(sqrt(n))), D ~ U[0.5, 2],
@@ -218,7 +218,7 @@ def make_batch_systems(S=512, seed=3):
     return systems
 
 
-def test_cfg3_batch_512_systems():
+def test_batch_512_systems():
     systems = make_batch_systems()
     h, ts = g.batch_union(systems, (32, 32))
     A = h.copyToTile(g.CRS)
@@ -236,7 +236,7 @@ def test_cfg3_batch_512_systems():
     bb = b.astype(np.float64)
     rel = np.array([np.linalg.norm(r[rs[k]:rs[k + 1]]) / np.linalg.norm(bb[rs[k]:rs[k + 1]])
                     for k in range(512)])
-    print(f"cfg3: {n} rows, solve {1e3 * sec:.1f} ms, max true rel. residual "
+    print(f"batch512: {n} rows, solve {1e3 * sec:.1f} ms, max true rel. residual "
           f"{rel.max():.2e}", flush=True)
     assert (rel <= 1e-10).all(), rel.max()
     # three sampled systems (smallest, largest, one in between) solved alone
@@ -264,8 +264,8 @@ def test_cfg3_batch_512_systems():
         assert err < 1e-7, (k, err)
 
 
-# ---- cfg4 -------------------------------------------------------------------
-def test_cfg4_serving_rates():
+# ---- serving -------------------------------------------------------------------
+def test_serving_rates():
     """Resident systems of the three generators with sizes log-uniform in
     [1e4, 2e6] rows; Poisson arrivals at 50, 200 and 800 requests/s, 1-8
     right-hand sides each, MPIR to 1e-10."""
@@ -303,7 +303,7 @@ def test_cfg4_serving_rates():
         sysv[k], rhs[k], seeds[k] = ids[probe], 3, 99
         res, sol = srv.run([ids[i] for i in sysv], rhs, arrival, seeds,
                            keep_solution=[(k, mats[probe].nrows)])
-        print(f"cfg4 at {rate} req/s: p50 {1e3 * np.median(res['latency']):.1f} ms, "
+        print(f"serving at {rate} req/s: p50 {1e3 * np.median(res['latency']):.1f} ms, "
               f"max rel. residual {res['rel_residual'].max():.2e}", flush=True)
         assert np.all(res["converged"] == 1)
         assert np.all(res["rel_residual"] <= 1e-10)
