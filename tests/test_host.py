@@ -243,3 +243,21 @@ def test_structure_beyond_one_work_chunk(reverse):
     ol = o.levels(n, T, L["tile_start"], L["row_ptr"], L["col"])
     for k in ["level", "color_sort_addr", "color_start", "num_colors"]:
         assert np.array_equal(lv[k], ol[k]), k
+
+
+def test_errors_from_worker_threads():
+    """Errors raised inside the threaded pipeline reach the caller: an
+    out-of-range triplet in the last of several 1 Mi-triplet chunks, and the
+    LDU faces of a pattern that is not structurally symmetric (rows handled
+    by several workers)."""
+    n = 100000
+    rows, cols, vals = g.generatePowerLawTriplets(n, seed=4)
+    bad = cols.copy()
+    bad[-1] = n + 5
+    with pytest.raises(g.GrapheneError) as e:
+        g.HostMatrix.from_coo(n, rows, bad, vals, 16)
+    assert "out of range" in str(e.value)
+    h = g.HostMatrix.from_coo(n, rows, cols, vals, 16)
+    with pytest.raises(g.GrapheneError) as e:
+        h.ldu()
+    assert "symmetric" in str(e.value)
